@@ -1,0 +1,201 @@
+/* sensemap_pm.h -- C-ABI of the MI355X (gfx950) PatchMatch multi-view-stereo
+ * engine.  This is the drop-in boundary for SenseMap's PatchMatch stage:
+ *
+ *   reference (src/mvs/patch_match_cuda.h:154-203)        this ABI
+ *   ------------------------------------------------------------------------
+ *   PatchMatchACMMCuda(options, problem)   (ctor)      pm_problem_create
+ *   bool PatchMatchACMMCuda::Run()                     pm_problem_run
+ *   DepthMap GetDepthMap() const                       pm_problem_get_depth_map
+ *   NormalMap GetNormalMap() const                     pm_problem_get_normal_map
+ *   Mat<float> GetConfMap() const                      pm_problem_get_conf_map
+ *   cost_map_ (no getter in the reference)             pm_problem_get_cost_map
+ *   view_sel_map_ (no getter in the reference)         pm_problem_get_view_sel
+ *   ~PatchMatchACMMCuda()                              pm_problem_destroy
+ *   PatchMatchOptions defaults / Check()               pm_options_init / pm_options_check
+ *     (src/controllers/patch_match_options.h:42-304, patch_match_options.cc:75-101)
+ *   SetBestCudaDevice(gpu_index) (src/util/cuda.cc:36-61)  pm_engine_create(device)
+ *
+ * Plain C: fixed-width scalars and pointers only.  Every call returns a
+ * pm_status and never exits or aborts; pm_last_error() returns the message of
+ * the calling thread's last failure.  Inputs are borrowed for the duration of
+ * the call that receives them; outputs go to caller-allocated buffers.
+ */
+#ifndef SENSEMAP_PM_H_
+#define SENSEMAP_PM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_ABI_VERSION 1
+#define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
+#define PM_MAX_WINDOW_RADIUS 32
+#define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
+
+typedef enum pm_status {
+  PM_OK = 0,
+  PM_ERR_INVALID_ARG = 1,
+  PM_ERR_OUT_OF_MEMORY = 2,
+  PM_ERR_DEVICE = 3,
+  PM_ERR_TOO_MANY_SOURCES = 4,
+  PM_ERR_NOT_READY = 5,
+  PM_ERR_OVER_BUDGET = 6,
+  PM_ERR_INTERNAL = 7
+} pm_status;
+
+enum { PM_STORAGE_U8 = 0, PM_STORAGE_F16 = 1 };
+enum { PM_ACCUM_PLAIN = 0, PM_ACCUM_COMPENSATED = 1 };
+enum { PM_RCP_EXACT = 0, PM_RCP_FAST = 1 };
+
+/* Engine-relevant fields of PatchMatchOptions, same names and defaults
+ * (src/controllers/patch_match_options.h:42-304).  Fields the engine does not
+ * read (fusion, meshing, lidar, cache sizes, ...) are not mirrored. */
+typedef struct pm_options {
+  int32_t window_radius;              /* 5 */
+  int32_t window_step;                /* 2 */
+  float sigma_spatial;                /* = window_radius (5) */
+  float sigma_color;                  /* 0.2 */
+  int32_t num_samples;                /* 15 */
+  int32_t num_iterations;             /* 5 */
+  int32_t geom_consistency;           /* 1 */
+  float geom_consistency_regularizer; /* 0.3 */
+  float geom_consistency_max_cost;    /* 3.0 */
+  int32_t random_optimization;        /* 1 */
+  int32_t local_optimization;         /* 1 */
+  float random_depth_ratio;           /* 0.004 */
+  float random_angle1_range;          /* 20 deg, in radians */
+  float random_angle2_range;          /* 12 deg, in radians */
+  float init_ncc_matching_cost;       /* 0.8 */
+  float max_ncc_matching_cost;        /* 1.2 */
+  float alpha;                        /* 90 */
+  float beta;                         /* 0.3 */
+  int32_t thk;                        /* 4 (1..4) */
+  int32_t num_good_hypothesis;        /* 2 */
+  int32_t num_bad_hypothesis;         /* 3 */
+  int32_t init_depth_random;          /* = !geom_consistency */
+  int32_t filter;                     /* 1 */
+  int32_t conf_filter;                /* 0: FilterDepthMap runs when filter && conf_filter */
+  float conf_threshold;               /* 0.5 */
+  float depth_diff_threshold;         /* 0.02 */
+  /* ---- extensions (no counterpart in the reference) ---- */
+  int32_t init_samples;               /* 1 = reference semantics; K > 1 keeps the best of K random inits */
+  int32_t compat_lo_geo_bug;          /* 1 = reproduce kernel_functions.cu:2924 (ref_invK passed as K) */
+  int32_t image_storage;              /* PM_STORAGE_U8 or PM_STORAGE_F16 (source images in HBM) */
+  int32_t accum_mode;                 /* PM_ACCUM_PLAIN or PM_ACCUM_COMPENSATED (NCC moments) */
+  int32_t rcp_mode;                   /* PM_RCP_EXACT (bitwise with the CPU transcription) or PM_RCP_FAST */
+  int32_t reserved0;
+  uint64_t seed;                      /* Philox key; streams are keyed by (seed, view_id, pixel, pass) */
+} pm_options;
+
+/* Pinhole camera: K = {fx, cx, fy, cy}; world->camera rotation R (row-major)
+ * and translation T, as mvs::Image::GetK/GetR/GetT (src/mvs/image.h). */
+typedef struct pm_camera {
+  float K[4];
+  float R[9];
+  float T[3];
+} pm_camera;
+
+/* 8-bit greyscale image, row 0 at the top (src/util/bitmap.cc:201-205). */
+typedef struct pm_image {
+  const uint8_t* data;
+  int32_t width;
+  int32_t height;
+  int32_t stride;  /* bytes between rows, >= width */
+  int32_t reserved0;
+} pm_image;
+
+/* One reference view with its source views: the fields of Problem
+ * (src/controllers/patch_match_options.h:306-353) the engine reads, with the
+ * images, cameras and maps given directly instead of by index into
+ * controller-owned vectors. */
+typedef struct pm_view_desc {
+  uint32_t view_id;                 /* Problem::ref_image_idx; keys the random streams */
+  int32_t num_src;                  /* Problem::src_image_idxs.size(), 1..PM_MAX_NUM_SRC */
+  pm_image ref;
+  pm_camera ref_cam;
+  const pm_image* src;              /* num_src entries */
+  const pm_camera* src_cams;        /* num_src entries */
+  float depth_min;                  /* PatchMatchOptions::depth_min as set per problem */
+  float depth_max;
+  /* Geometric pass (geom_consistency): per-source depth maps, row-major,
+   * src[i].width x src[i].height floats each (Problem::depth_maps). */
+  const float* const* src_depth_maps;
+  /* Initial state when !init_depth_random: depth W x H and normal 3 x W x H
+   * planar (Problem::depth_maps / normal_maps at ref_image_idx). */
+  const float* init_depth;
+  const float* init_normal;
+} pm_view_desc;
+
+/* Caller-allocated outputs; a NULL pointer skips that map. */
+typedef struct pm_outputs {
+  float* depth;        /* W x H */
+  float* normal;       /* 3 x W x H planar (x, y, z), the NormalMap layout */
+  float* conf;         /* W x H */
+  float* cost;         /* num_src x W x H planar: per-source cost of the winning hypothesis */
+  uint32_t* view_sel;  /* W x H bitmask of selected source views */
+} pm_outputs;
+
+/* Timings of the last pm_problem_run, measured with hipEvents on the
+ * engine's stream. */
+typedef struct pm_run_stats {
+  double total_ms;          /* whole Run() on the device */
+  double sweep_ms;          /* sum over all checkerboard sweep launches */
+  double local_opt_ms;      /* sum over local-optimisation launches */
+  double init_ms;           /* moments + init + init-conf launches */
+  int32_t sweep_launches;
+  int32_t local_opt_launches;
+  uint64_t ncc_evaluations; /* (hypothesis, source) NCC evaluations, when counting was enabled */
+  uint64_t ncc_samples;     /* window samples of those evaluations */
+  uint64_t geo_evaluations; /* geometric-consistency terms */
+} pm_run_stats;
+
+typedef struct pm_engine pm_engine;
+typedef struct pm_problem pm_problem;
+
+/* Fill the PatchMatchOptions defaults (patch_match_options.h:42-304). */
+pm_status pm_options_init(pm_options* options);
+/* PatchMatchOptions::Check() (patch_match_options.cc:75-101) plus the limits
+ * of this engine (thk 1..4, sources, radius).  PM_OK when valid. */
+pm_status pm_options_check(const pm_options* options);
+
+const char* pm_last_error(void);
+int32_t pm_abi_version(void);
+
+/* Select device `device` and create a HIP stream owned by the engine
+ * (SetBestCudaDevice, src/util/cuda.cc:36-61). */
+pm_status pm_engine_create(int32_t device, pm_engine** out);
+void pm_engine_destroy(pm_engine* engine);
+/* Turn on per-run counting of NCC evaluations and window samples
+ * (calibration runs only; off by default). */
+pm_status pm_engine_set_counting(pm_engine* engine, int32_t enable);
+
+/* Upload the view, pack poses and allocate the workspace: the
+ * PatchMatchACMMCuda constructor (patch_match_cuda.cu:422-432). */
+pm_status pm_problem_create(pm_engine* engine, const pm_options* options,
+                            const pm_view_desc* view, pm_problem** out);
+/* Run the iteration schedule (patch_match_cuda.cu:748-1057); blocks until done. */
+pm_status pm_problem_run(pm_problem* problem);
+pm_status pm_problem_get_depth_map(const pm_problem* problem, float* out);
+pm_status pm_problem_get_normal_map(const pm_problem* problem, float* out);
+pm_status pm_problem_get_conf_map(const pm_problem* problem, float* out);
+pm_status pm_problem_get_cost_map(const pm_problem* problem, float* out);
+pm_status pm_problem_get_view_sel(const pm_problem* problem, uint32_t* out);
+pm_status pm_problem_get_stats(const pm_problem* problem, pm_run_stats* out);
+/* Packed per-source pose block (PM_POSE_STRIDE floats per source) and the
+ * reference {fx,cx,fy,cy, 1/fx,-cx/fx,1/fy,-cy/fy} (InitTransforms,
+ * patch_match_cuda.cu:493-580). */
+pm_status pm_problem_get_poses(const pm_problem* problem, float* poses, float ref_k8[8]);
+void pm_problem_destroy(pm_problem* problem);
+
+/* create + run + read back + destroy in one call. */
+pm_status pm_run_view(pm_engine* engine, const pm_options* options,
+                      const pm_view_desc* view, const pm_outputs* outputs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* SENSEMAP_PM_H_ */

@@ -1,0 +1,87 @@
+"""ctypes binding of oracle/lib/liboracle_pm.so (test infrastructure only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from sensemap_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "liboracle_pm.so")
+_lib = None
+
+
+class pmo_counters(C.Structure):
+    _fields_ = [("ncc_evaluations", C.c_uint64), ("ncc_samples", C.c_uint64), ("geo_evaluations", C.c_uint64)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        P = C.POINTER
+        lib.pmo_run_view.restype = C.c_int
+        lib.pmo_run_view.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(abi.pm_outputs), C.c_int, C.c_int,
+                                     C.c_int, P(pmo_counters)]
+        lib.pmo_pack_poses.restype = C.c_int
+        lib.pmo_pack_poses.argtypes = [P(abi.pm_view_desc), P(C.c_float), P(C.c_float)]
+        lib.pmo_ref_moments.restype = C.c_int
+        lib.pmo_ref_moments.argtypes = [P(abi.pm_options), P(C.c_uint8), C.c_int, C.c_int, P(C.c_float),
+                                        P(C.c_float), P(C.c_float), P(C.c_uint8)]
+        lib.pmo_ncc_cost.restype = C.c_float
+        lib.pmo_ncc_cost.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, P(C.c_float), C.c_int]
+        lib.pmo_ncc_cost_f64.restype = C.c_double
+        lib.pmo_ncc_cost_f64.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, P(C.c_double),
+                                         C.c_int]
+        lib.pmo_geom_cost.restype = C.c_float
+        lib.pmo_geom_cost.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, C.c_float, C.c_int]
+        for name in ("pmo_expf", "pmo_sinf", "pmo_cosf", "pmo_acosf"):
+            getattr(lib, name).restype = C.c_float
+            getattr(lib, name).argtypes = [C.c_float]
+        lib.pmo_atan2f.restype = C.c_float
+        lib.pmo_atan2f.argtypes = [C.c_float, C.c_float]
+        lib.pmo_philox.restype = None
+        lib.pmo_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        lib.pmo_uniform.restype = C.c_float
+        lib.pmo_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        _lib = lib
+    return _lib
+
+
+def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):
+    """Run the transcription on a sensemap_amd.patch_match.ViewDesc; returns a
+    dict of depth, normal, conf, cost, view_sel (and counters if count)."""
+    lib = load()
+    W, H, N = view.width, view.height, view.num_src
+    out = {
+        "depth": np.zeros((H, W), np.float32),
+        "normal": np.zeros((3, H, W), np.float32),
+        "conf": np.zeros((H, W), np.float32),
+        "cost": np.zeros((N, H, W), np.float32),
+        "view_sel": np.zeros((H, W), np.uint32),
+    }
+    o = abi.pm_outputs()
+    o.depth = out["depth"].ctypes.data_as(C.POINTER(C.c_float))
+    o.normal = out["normal"].ctypes.data_as(C.POINTER(C.c_float))
+    o.conf = out["conf"].ctypes.data_as(C.POINTER(C.c_float))
+    o.cost = out["cost"].ctypes.data_as(C.POINTER(C.c_float))
+    o.view_sel = out["view_sel"].ctypes.data_as(C.POINTER(C.c_uint32))
+    copt = options.to_c() if hasattr(options, "to_c") else options
+    cnt = pmo_counters()
+    rc = lib.pmo_run_view(C.byref(copt), C.byref(view.desc), C.byref(o), int(row_begin), int(row_end), int(threads),
+                          C.byref(cnt) if count else None)
+    if rc != 0:
+        raise RuntimeError("pmo_run_view failed")
+    if count:
+        out["counters"] = (cnt.ncc_evaluations, cnt.ncc_samples, cnt.geo_evaluations)
+    return out

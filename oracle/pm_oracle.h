@@ -1,0 +1,114 @@
+/* pm_oracle.h -- CPU transcription of SenseMap's ACMM PatchMatch stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is the checker that the gfx950 HIP
+ * engine is compared against, and the CPU baseline that bench.py times; the
+ * product never links or calls it.
+ *
+ * It restates, in plain C with the canonical FP32 evaluation order shared
+ * with the HIP kernels, the semantics of:
+ *   src/mvs/gpu_mat_ref_image.cu:16-56      FilterKernel (ref moments)
+ *   src/mvs/kernel_functions.cu:70-306      random draws, perturbation, CorrectNormal
+ *   src/mvs/kernel_functions.cu:409-568     PropagateDepth, ComposeHomography,
+ *                                           ComputeGeomConsistencyCostShared
+ *   src/mvs/kernel_functions.cu:636-686     FindMinCost, TransformPDFToCDF
+ *   src/mvs/kernel_functions.cu:689-737     InitNormalMap, InitDepthMap
+ *   src/mvs/kernel_functions.cu:1245-1340   ComputeNCCCost<R,S>
+ *   src/mvs/kernel_functions.cu:1447-1601   ComputeMultiViewCostVector, InitConfMapKernel
+ *   src/mvs/kernel_functions.cu:1655-1675   ComputeGradientMap
+ *   src/mvs/kernel_functions.cu:2136-2560   SweepFromCheckerBoardOpt
+ *   src/mvs/kernel_functions.cu:2736-2931   LocalOptimization
+ *   src/mvs/kernel_functions.cu:3282-3354   FilterDepthMap
+ *   src/mvs/patch_match_cuda.cu:493-580     InitTransforms (pose packing)
+ *   src/mvs/patch_match_cuda.cu:582-698     InitWorkspace
+ *   src/mvs/patch_match_cuda.cu:748-1057    Run() iteration schedule
+ *   src/mvs/utils.cc:14-52                  relative pose / projection matrices
+ *
+ * plus an FP64 restatement of the NCC cost (pmo_ncc_cost_f64) used as the
+ * precision reference.
+ */
+#ifndef SENSEMAP_PM_ORACLE_H_
+#define SENSEMAP_PM_ORACLE_H_
+
+#include <stdint.h>
+#include "../include/sensemap_pm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-view precomputed data, filled by pmo_prepare. */
+typedef struct pmo_view {
+  int W, H, N;
+  const uint8_t* ref;          /* W*H, tightly packed copy */
+  const uint8_t* src[PM_MAX_NUM_SRC];
+  int srcW[PM_MAX_NUM_SRC], srcH[PM_MAX_NUM_SRC];
+  const float* src_depth[PM_MAX_NUM_SRC];
+  float refK[4];               /* fx, cx, fy, cy */
+  float refInvK[4];            /* 1/fx, -cx/fx, 1/fy, -cy/fy */
+  float pose[PM_MAX_NUM_SRC][PM_POSE_STRIDE];
+  float depth_min, depth_max;
+  uint32_t view_id;
+  /* bilateral weights: spatial table (per window sample) and colour table */
+  int nwin;                    /* samples per window row */
+  float* sp;                   /* nwin*nwin */
+  float ct[256];
+  /* per-pixel ref moments */
+  float* mu_r;
+  float* var_r;
+  float* inv_sw;
+  float* err_r;                /* sum w (r - mu) / sum w, compensated mode */
+  uint8_t* grad_ok;            /* ComputeGradientMap >= 1.0 */
+} pmo_view;
+
+typedef struct pmo_state {
+  float* depth;                /* W*H */
+  float* normal;               /* 3*W*H planar */
+  float* conf;                 /* W*H */
+  float* cost;                 /* N*W*H */
+  uint32_t* view_sel;          /* W*H */
+} pmo_state;
+
+typedef struct pmo_counters {
+  uint64_t ncc_evaluations;
+  uint64_t ncc_samples;
+  uint64_t geo_evaluations;
+} pmo_counters;
+
+/* Full Run() of one view (photometric or geometric per options).  Pixels in
+ * rows [row_begin, row_end) are updated; the others keep their initial state
+ * (row_end <= 0 means all rows).  num_threads <= 0 uses the OpenMP default.
+ * Returns 0 on success. */
+int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out,
+                 int row_begin, int row_end, int num_threads, pmo_counters* counters);
+
+/* Pose packing exactly as the engine does it (see pm_problem_get_poses). */
+int pmo_pack_poses(const pm_view_desc* view, float* poses, float ref_k8[8]);
+
+/* Ref moments for a whole image: mu, var, 1/sum_w, gradient gate. */
+int pmo_ref_moments(const pm_options* opt, const uint8_t* img, int W, int H,
+                    float* mu, float* var, float* inv_sw, uint8_t* grad_ok);
+
+/* One NCC evaluation (FP32 canonical) and its FP64 restatement, for the
+ * hypothesis {nx, ny, nz, depth} at (row, col) against source `src`. */
+float pmo_ncc_cost(const pm_options* opt, const pm_view_desc* view, int row, int col,
+                   const float hyp[4], int src);
+double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row, int col,
+                        const double hyp[4], int src);
+/* Geometric-consistency term (FP32 canonical), K = ref {fx, cx, fy, cy}. */
+float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, int col,
+                    float depth, int src);
+
+/* Deterministic math and RNG, exported for the tests. */
+float pmo_expf(float x);
+float pmo_sinf(float x);
+float pmo_cosf(float x);
+float pmo_atan2f(float y, float x);
+float pmo_acosf(float x);
+void pmo_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float pmo_uniform(uint64_t seed, uint32_t view, uint32_t pixel, uint32_t pass, uint32_t draw);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,182 @@
+"""ctypes mirror of include/sensemap_pm.h and the loader of libsensemap_pm.so.
+
+The product path is the in-tree HIP library; there is no CPU fallback.  If
+the library is missing, ``load_engine()`` raises instead of degrading.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+PM_MAX_NUM_SRC = 32
+PM_POSE_STRIDE = 44
+
+PM_OK = 0
+STATUS_NAMES = {
+    0: "PM_OK", 1: "PM_ERR_INVALID_ARG", 2: "PM_ERR_OUT_OF_MEMORY", 3: "PM_ERR_DEVICE",
+    4: "PM_ERR_TOO_MANY_SOURCES", 5: "PM_ERR_NOT_READY", 6: "PM_ERR_OVER_BUDGET", 7: "PM_ERR_INTERNAL",
+}
+PM_STORAGE_U8, PM_STORAGE_F16 = 0, 1
+PM_ACCUM_PLAIN, PM_ACCUM_COMPENSATED = 0, 1
+PM_RCP_EXACT, PM_RCP_FAST = 0, 1
+
+
+class pm_options(C.Structure):
+    _fields_ = [
+        ("window_radius", C.c_int32),
+        ("window_step", C.c_int32),
+        ("sigma_spatial", C.c_float),
+        ("sigma_color", C.c_float),
+        ("num_samples", C.c_int32),
+        ("num_iterations", C.c_int32),
+        ("geom_consistency", C.c_int32),
+        ("geom_consistency_regularizer", C.c_float),
+        ("geom_consistency_max_cost", C.c_float),
+        ("random_optimization", C.c_int32),
+        ("local_optimization", C.c_int32),
+        ("random_depth_ratio", C.c_float),
+        ("random_angle1_range", C.c_float),
+        ("random_angle2_range", C.c_float),
+        ("init_ncc_matching_cost", C.c_float),
+        ("max_ncc_matching_cost", C.c_float),
+        ("alpha", C.c_float),
+        ("beta", C.c_float),
+        ("thk", C.c_int32),
+        ("num_good_hypothesis", C.c_int32),
+        ("num_bad_hypothesis", C.c_int32),
+        ("init_depth_random", C.c_int32),
+        ("filter", C.c_int32),
+        ("conf_filter", C.c_int32),
+        ("conf_threshold", C.c_float),
+        ("depth_diff_threshold", C.c_float),
+        ("init_samples", C.c_int32),
+        ("compat_lo_geo_bug", C.c_int32),
+        ("image_storage", C.c_int32),
+        ("accum_mode", C.c_int32),
+        ("rcp_mode", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class pm_camera(C.Structure):
+    _fields_ = [("K", C.c_float * 4), ("R", C.c_float * 9), ("T", C.c_float * 3)]
+
+
+class pm_image(C.Structure):
+    _fields_ = [
+        ("data", C.POINTER(C.c_uint8)),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("stride", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
+class pm_view_desc(C.Structure):
+    _fields_ = [
+        ("view_id", C.c_uint32),
+        ("num_src", C.c_int32),
+        ("ref", pm_image),
+        ("ref_cam", pm_camera),
+        ("src", C.POINTER(pm_image)),
+        ("src_cams", C.POINTER(pm_camera)),
+        ("depth_min", C.c_float),
+        ("depth_max", C.c_float),
+        ("src_depth_maps", C.POINTER(C.POINTER(C.c_float))),
+        ("init_depth", C.POINTER(C.c_float)),
+        ("init_normal", C.POINTER(C.c_float)),
+    ]
+
+
+class pm_outputs(C.Structure):
+    _fields_ = [
+        ("depth", C.POINTER(C.c_float)),
+        ("normal", C.POINTER(C.c_float)),
+        ("conf", C.POINTER(C.c_float)),
+        ("cost", C.POINTER(C.c_float)),
+        ("view_sel", C.POINTER(C.c_uint32)),
+    ]
+
+
+class pm_run_stats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("sweep_ms", C.c_double),
+        ("local_opt_ms", C.c_double),
+        ("init_ms", C.c_double),
+        ("sweep_launches", C.c_int32),
+        ("local_opt_launches", C.c_int32),
+        ("ncc_evaluations", C.c_uint64),
+        ("ncc_samples", C.c_uint64),
+        ("geo_evaluations", C.c_uint64),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_SIGNATURES = {
+    "pm_options_init": (C.c_int, [C.POINTER(pm_options)]),
+    "pm_options_check": (C.c_int, [C.POINTER(pm_options)]),
+    "pm_last_error": (C.c_char_p, []),
+    "pm_abi_version": (C.c_int32, []),
+    "pm_engine_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_engine_destroy": (None, [C.c_void_p]),
+    "pm_engine_set_counting": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pm_problem_create": (C.c_int, [C.c_void_p, C.POINTER(pm_options), C.POINTER(pm_view_desc),
+                                    C.POINTER(C.c_void_p)]),
+    "pm_problem_run": (C.c_int, [C.c_void_p]),
+    "pm_problem_get_depth_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_problem_get_normal_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_problem_get_conf_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_problem_get_cost_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "pm_problem_get_view_sel": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pm_problem_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_run_stats)]),
+    "pm_problem_get_poses": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "pm_problem_destroy": (None, [C.c_void_p]),
+    "pm_run_view": (C.c_int, [C.c_void_p, C.POINTER(pm_options), C.POINTER(pm_view_desc),
+                              C.POINTER(pm_outputs)]),
+}
+
+ENGINE_LIB = os.path.join(LIB_DIR, "libsensemap_pm.so")
+_engine = None
+
+
+class PMError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load_engine() -> C.CDLL:
+    """Load the in-tree HIP engine library; raise if it has not been built."""
+    global _engine
+    if _engine is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise RuntimeError(
+                f"{ENGINE_LIB} is missing: build it with __graft_entry__.build() "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        lib = C.CDLL(ENGINE_LIB)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _engine = lib
+    return _engine
+
+
+def check(status: int) -> None:
+    if status != PM_OK:
+        raise PMError(status, load_engine().pm_last_error().decode(errors="replace"))
+
+
+def default_options(**overrides) -> pm_options:
+    o = pm_options()
+    check(load_engine().pm_options_init(C.byref(o)))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"pm_options has no field {k}")
+        setattr(o, k, v)
+    return o

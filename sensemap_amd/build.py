@@ -1,0 +1,77 @@
+"""Build recipe for the native parts of sensemap_amd.
+
+  libsensemap_pm.so  gfx950 HIP kernels + host engine + C-ABI (hipcc)
+  libpm_scene.so     synthetic scene renderer (gcc, host only)
+
+Both are built in-tree under sensemap_amd/lib/ so they travel with the
+repository snapshot.  Every HIP translation unit is compiled with
+-ffp-contract=off (the canonical FP32 evaluation order) and with packed FP32
+instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PM_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-fvisibility=hidden", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}", "-Wall", "-Wno-unused-function",
+]
+
+HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip"]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def build_engine() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(LIB, src.replace(".hip", ".o"))
+        objs.append(obj)
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for f in [ex.submit(_run, j) for j in jobs]:
+            f.result()
+    out = os.path.join(LIB, "libsensemap_pm.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_scene() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libpm_scene.so")
+    _run(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-Wall", "-o", out,
+          os.path.join(CSRC, "pm_scene.c"), "-lm"])
+    return out
+
+
+def build_all() -> None:
+    build_scene()
+    build_engine()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", os.listdir(LIB), file=sys.stderr)

@@ -1,0 +1,100 @@
+// pm_device.h -- device-side view descriptor and launch interface of the
+// gfx950 PatchMatch kernels.  Shared by the kernels (pm_kernels.hip) and the
+// host engine (pm_engine.hip); not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sensemap_pm.h"
+
+namespace pm {
+
+constexpr int kMaxHyp = 11;          // 1 current + 2 neighbours + 8 directional
+constexpr int kNumRefine = 5;        // current + 4 refinement hypotheses
+constexpr int kQuadPad = 4;          // zero texels around every source image
+constexpr int kTileW = 32;           // sweep tile: 32 columns x 16 rows,
+constexpr int kTileH = 16;           //   256 threads = one colour of the tile
+constexpr int kSweepThreads = 256;
+constexpr float kMaxCost = 2.0f;
+constexpr float kVarThreshold = 0.65025f;  // 1e-5 * 255^2
+constexpr float kLowTexVar = 25.0f;        // compensated-mode cut (0..255 units)
+
+// Per-source device data.
+struct SrcDev {
+  const void* tex;          // quad texels: uint32 (u8 x4) or uint2 (f16 x4)
+  const float4* depth_q;    // quad texels of the source depth map (geometric pass)
+  int32_t width, height;    // source image size
+  int32_t pitch;            // width + 2*kQuadPad (texels per padded row)
+  int32_t pad0;
+};
+
+// One view as the kernels see it: lives in device memory, read through
+// uniform (scalar) loads.
+struct ViewDev {
+  int32_t W, H, N;
+  int32_t nwin;             // samples per window row
+  int32_t radius, step;
+  int32_t view_id;
+  int32_t geom;             // geometric-consistency term on
+  float refK[4];
+  float refInvK[4];
+  float depth_min, depth_max, depth_max_eff;  // eff = 1.2 * depth_max (sweep random depth)
+  float geom_regularizer, geom_max_cost;
+  float random_depth_ratio, angle1, angle2;
+  float max_ncc_matching_cost, inv_beta2;
+  int32_t num_good, num_bad, num_samples, random_optimization;
+  int32_t thk, init_samples, init_random, compat_lo_geo_bug;
+  int32_t accum_mode, slots_per_view;
+  float conf_threshold, depth_diff_threshold;
+  uint64_t seed;
+  const uint8_t* ref;       // W*H
+  const float4* refm;       // W*H {mu_r, var_r, 1/sum_w, err_r}
+  const uint8_t* grad_ok;   // W*H
+  const float* poses;       // N * PM_POSE_STRIDE
+  const float* sp;          // nwin*nwin spatial weights
+  const float* ct;          // 256 colour weights
+  float* depth;             // W*H
+  float* normal;            // 3*W*H planar
+  float* conf;              // W*H
+  float* cost;              // N*W*H planar
+  uint32_t* view_sel;       // W*H
+  float* ws;                // workspace, slots_per_view slots, SoA
+  unsigned long long* counters;  // [ncc_eval, ncc_samples, geo_eval] or null
+  SrcDev src[PM_MAX_NUM_SRC];
+};
+
+// Workspace fields (SoA: field f of slot s at ws[f * slots + s]).
+//   hyp   : 4 * (kMaxHyp + kNumRefine) floats  (nx, ny, nz, d per hypothesis)
+//   M     : kMaxHyp * N floats      (candidate x source costs)
+//   RM    : kNumRefine * N floats   (refinement x source costs)
+//   W     : N floats                (view weights, later CDF)
+inline int ws_floats_per_slot(int N) {
+  return 4 * (kMaxHyp + kNumRefine) + kMaxHyp * N + kNumRefine * N + N;
+}
+
+struct LaunchCfg {
+  int storage;   // PM_STORAGE_*
+  int rcp;       // PM_RCP_*
+  int radius, step;
+};
+
+// Kernel launch wrappers (pm_kernels.hip).  Every launch checks the grid it
+// builds against the view sizes first.
+hipError_t launch_build_quads_u8(const uint8_t* img, int W, int H, int stride, uint32_t* out,
+                                 hipStream_t s);
+hipError_t launch_build_quads_f16(const uint8_t* img, int W, int H, int stride, uint2* out,
+                                  hipStream_t s);
+hipError_t launch_build_depth_quads(const float* depth, int W, int H, float4* out, hipStream_t s);
+hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, float4* refm,
+                              uint8_t* grad_ok, hipStream_t s);
+hipError_t launch_init(const ViewDev* v, const LaunchCfg& cfg, int W, int H, uint32_t pass,
+                       hipStream_t s);
+hipError_t launch_sweep(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, float th_mc,
+                        uint32_t pass, hipStream_t s);
+hipError_t launch_local_opt(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color,
+                            uint32_t pass, hipStream_t s);
+hipError_t launch_filter(const ViewDev* v, int W, int H, int color, hipStream_t s);
+int sweep_slots(int W, int H);   // threads launched per colour pass
+
+}  // namespace pm

@@ -1,0 +1,666 @@
+// pm_engine.hip -- host side of the gfx950 PatchMatch engine and its C-ABI
+// (include/sensemap_pm.h).
+//
+// One pm_engine owns a device and a HIP stream.  One pm_problem is one
+// reference view with its sources resident in HBM: the constructor of
+// SenseMap's PatchMatchACMMCuda (src/mvs/patch_match_cuda.cu:422-432) becomes
+// pm_problem_create, Run() (:748-1057) becomes pm_problem_run and the
+// Get*Map() getters become pm_problem_get_*.  Errors are returned as
+// pm_status with a per-thread message, never by exit() as the reference's
+// CUDA_SAFE_CALL does (src/util/cudacc.cc:30-38).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sensemap_pm.h"
+#include "pm_device.h"
+#include "pm_math.h"
+
+#define PM_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_last_error;
+
+pm_status fail(pm_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define PM_HIP(call)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                          \
+    }                                                                                          \
+  } while (0)
+
+}  // namespace
+
+struct pm_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int counting = 0;
+};
+
+struct pm_problem {
+  pm_engine* engine = nullptr;
+  pm_options opt{};
+  int W = 0, H = 0, N = 0;
+  uint32_t view_id = 0;
+  float depth_min = 0.f, depth_max = 0.f;
+  std::vector<void*> allocs;
+  pm::ViewDev host{};
+  pm::ViewDev* dev = nullptr;
+  float* init_depth = nullptr;   // device copies of the given initial state
+  float* init_normal = nullptr;
+  unsigned long long* counters = nullptr;
+  std::vector<float> poses;      // N * PM_POSE_STRIDE
+  float refk8[8] = {};
+  pm_run_stats stats{};
+  size_t device_bytes = 0;
+
+  template <typename T>
+  pm_status alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 1);
+    if (e != hipSuccess) {
+      return fail(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
+                  "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    allocs.push_back(q);
+    device_bytes += bytes;
+    *p = (T*)q;
+    return PM_OK;
+  }
+  ~pm_problem() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// options
+
+PM_API pm_status pm_options_init(pm_options* o) {
+  if (!o) return fail(PM_ERR_INVALID_ARG, "pm_options_init: null options");
+  std::memset(o, 0, sizeof(*o));
+  o->window_radius = 5;
+  o->window_step = 2;
+  o->sigma_spatial = 5.0f;  // sigma_spatial = window_radius (patch_match_options.h:89)
+  o->sigma_color = 0.2f;
+  o->num_samples = 15;
+  o->num_iterations = 5;
+  o->geom_consistency = 1;
+  o->geom_consistency_regularizer = 0.3f;
+  o->geom_consistency_max_cost = 3.0f;
+  o->random_optimization = 1;
+  o->local_optimization = 1;
+  o->random_depth_ratio = 0.004f;
+  o->random_angle1_range = (float)(20.0 * M_PI / 180.0);
+  o->random_angle2_range = (float)(12.0 * M_PI / 180.0);
+  o->init_ncc_matching_cost = 0.8f;
+  o->max_ncc_matching_cost = 1.2f;
+  o->alpha = 90.0f;
+  o->beta = 0.3f;
+  o->thk = 4;
+  o->num_good_hypothesis = 2;
+  o->num_bad_hypothesis = 3;
+  o->init_depth_random = 0;  // = !geom_consistency
+  o->filter = 1;
+  o->conf_filter = 0;
+  o->conf_threshold = 0.5f;
+  o->depth_diff_threshold = 0.02f;
+  o->init_samples = 1;
+  o->compat_lo_geo_bug = 1;
+  o->image_storage = PM_STORAGE_U8;
+  o->accum_mode = PM_ACCUM_PLAIN;
+  o->rcp_mode = PM_RCP_EXACT;
+  o->seed = 0;
+  return PM_OK;
+}
+
+PM_API pm_status pm_options_check(const pm_options* o) {
+  if (!o) return fail(PM_ERR_INVALID_ARG, "null options");
+  // PatchMatchOptions::Check (patch_match_options.cc:75-101)
+  if (o->window_radius <= 0 || o->window_radius > PM_MAX_WINDOW_RADIUS)
+    return fail(PM_ERR_INVALID_ARG, "window_radius must be in [1, 32]");
+  if (o->window_step <= 0 || o->window_step > 2) return fail(PM_ERR_INVALID_ARG, "window_step must be 1 or 2");
+  if (!(o->sigma_color > 0.0f)) return fail(PM_ERR_INVALID_ARG, "sigma_color must be > 0");
+  if (!(o->sigma_spatial > 0.0f)) return fail(PM_ERR_INVALID_ARG, "sigma_spatial must be > 0");
+  if (o->num_samples <= 0) return fail(PM_ERR_INVALID_ARG, "num_samples must be > 0");
+  if (o->num_iterations < 0) return fail(PM_ERR_INVALID_ARG, "num_iterations must be >= 0");
+  if (!(o->geom_consistency_regularizer >= 0.0f))
+    return fail(PM_ERR_INVALID_ARG, "geom_consistency_regularizer must be >= 0");
+  if (!(o->geom_consistency_max_cost >= 0.0f)) return fail(PM_ERR_INVALID_ARG, "geom_consistency_max_cost must be >= 0");
+  // limits of this engine
+  if (o->thk < 1 || o->thk > 4) return fail(PM_ERR_INVALID_ARG, "thk must be in [1, 4] (mweights has 5 entries)");
+  if (!(o->beta > 0.0f)) return fail(PM_ERR_INVALID_ARG, "beta must be > 0");
+  if (!(o->alpha > 0.0f)) return fail(PM_ERR_INVALID_ARG, "alpha must be > 0");
+  if (o->init_samples < 1 || o->init_samples > 64) return fail(PM_ERR_INVALID_ARG, "init_samples must be in [1, 64]");
+  if (o->image_storage != PM_STORAGE_U8 && o->image_storage != PM_STORAGE_F16)
+    return fail(PM_ERR_INVALID_ARG, "image_storage must be PM_STORAGE_U8 or PM_STORAGE_F16");
+  if (o->accum_mode != PM_ACCUM_PLAIN && o->accum_mode != PM_ACCUM_COMPENSATED)
+    return fail(PM_ERR_INVALID_ARG, "accum_mode must be PM_ACCUM_PLAIN or PM_ACCUM_COMPENSATED");
+  if (o->rcp_mode != PM_RCP_EXACT && o->rcp_mode != PM_RCP_FAST)
+    return fail(PM_ERR_INVALID_ARG, "rcp_mode must be PM_RCP_EXACT or PM_RCP_FAST");
+  return PM_OK;
+}
+
+PM_API const char* pm_last_error(void) { return g_last_error.c_str(); }
+PM_API int32_t pm_abi_version(void) { return PM_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// engine
+
+PM_API pm_status pm_engine_create(int32_t device, pm_engine** out) {
+  if (!out) return fail(PM_ERR_INVALID_ARG, "null out");
+  *out = nullptr;
+  int count = 0;
+  PM_HIP(hipGetDeviceCount(&count));
+  if (device < 0) device = 0;
+  if (device >= count) return fail(PM_ERR_INVALID_ARG, "device index out of range");
+  PM_HIP(hipSetDevice(device));
+  pm_engine* e = new (std::nothrow) pm_engine();
+  if (!e) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  e->device = device;
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete e;
+    return fail(PM_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  *out = e;
+  return PM_OK;
+}
+
+PM_API void pm_engine_destroy(pm_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+PM_API pm_status pm_engine_set_counting(pm_engine* e, int32_t enable) {
+  if (!e) return fail(PM_ERR_INVALID_ARG, "null engine");
+  e->counting = enable ? 1 : 0;
+  return PM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// problem
+
+namespace {
+
+// InitTransforms (patch_match_cuda.cu:493-580) with ComputeRelativePose,
+// ComposeProjectionMatrix and ComposeInverseProjectionMatrix
+// (utils.cc:14-52), in FP64, rounded once to FP32.  inv_P is the closed
+// form [R^T K^-1 | -R^T T] of the inverse of [K R | K T; 0 0 0 1].
+void pack_poses(const pm_view_desc* view, float* poses, float k8[8]) {
+  const pm_camera* rc = &view->ref_cam;
+  const float fx = rc->K[0], cx = rc->K[1], fy = rc->K[2], cy = rc->K[3];
+  k8[0] = fx;
+  k8[1] = cx;
+  k8[2] = fy;
+  k8[3] = cy;
+  k8[4] = 1.0f / fx;
+  k8[5] = -cx / fx;
+  k8[6] = 1.0f / fy;
+  k8[7] = -cy / fy;
+  for (int s = 0; s < view->num_src; ++s) {
+    const pm_camera* sc = &view->src_cams[s];
+    double R[9], T[3];
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) acc += (double)sc->R[3 * i + k] * (double)rc->R[3 * j + k];
+        R[3 * i + j] = acc;
+      }
+    }
+    for (int i = 0; i < 3; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) acc += R[3 * i + k] * (double)rc->T[k];
+      T[i] = (double)sc->T[i] - acc;
+    }
+    const double sfx = sc->K[0], scx = sc->K[1], sfy = sc->K[2], scy = sc->K[3];
+    double P[12];
+    for (int j = 0; j < 4; ++j) {
+      const double r0 = j < 3 ? R[j] : T[0];
+      const double r1 = j < 3 ? R[3 + j] : T[1];
+      const double r2 = j < 3 ? R[6 + j] : T[2];
+      P[j] = sfx * r0 + scx * r2;
+      P[4 + j] = sfy * r1 + scy * r2;
+      P[8 + j] = r2;
+    }
+    const double Kinv[9] = {1.0 / sfx, 0.0, -scx / sfx, 0.0, 1.0 / sfy, -scy / sfy, 0.0, 0.0, 1.0};
+    double iP[12];
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) acc += R[3 * k + i] * Kinv[3 * k + j];
+        iP[4 * i + j] = acc;
+      }
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) acc += R[3 * k + i] * T[k];
+      iP[4 * i + 3] = -acc;
+    }
+    float* o = poses + (size_t)s * PM_POSE_STRIDE;
+    std::memset(o, 0, sizeof(float) * PM_POSE_STRIDE);
+    o[0] = sc->K[0];
+    o[1] = sc->K[1];
+    o[2] = sc->K[2];
+    o[3] = sc->K[3];
+    for (int i = 0; i < 9; ++i) o[4 + i] = (float)R[i];
+    for (int i = 0; i < 3; ++i) o[13 + i] = (float)T[i];
+    for (int i = 0; i < 12; ++i) o[16 + i] = (float)P[i];
+    for (int i = 0; i < 12; ++i) o[28 + i] = (float)iP[i];
+    o[40] = (float)view->src[s].width;
+    o[41] = (float)view->src[s].height;
+  }
+}
+
+// BilateralWeightComputer (utils.h:72-92) as spatial x colour tables, FP64.
+void weight_tables(const pm_options* o, std::vector<float>& sp, std::vector<float>& ct, int* nwin) {
+  const int R = o->window_radius, S = o->window_step;
+  int n = 0;
+  for (int d = -R; d <= R; d += S) ++n;
+  *nwin = n;
+  sp.resize((size_t)n * n);
+  const double ss = (double)o->sigma_spatial;
+  const double sn = 1.0 / (2.0 * ss * ss);
+  const double sc = (double)o->sigma_color;
+  const double cn = 1.0 / (2.0 * sc * sc);
+  int k = 0;
+  for (int dr = -R; dr <= R; dr += S)
+    for (int dc = -R; dc <= R; dc += S) sp[k++] = (float)std::exp(-(double)(dr * dr + dc * dc) * sn);
+  ct.resize(256);
+  for (int d = 0; d < 256; ++d) {
+    const double x = (double)d / 255.0;
+    ct[d] = (float)std::exp(-x * x * cn);
+  }
+}
+
+bool finite_camera(const pm_camera& c) {
+  for (float k : c.K)
+    if (!std::isfinite(k)) return false;
+  for (float r : c.R)
+    if (!std::isfinite(r)) return false;
+  for (float t : c.T)
+    if (!std::isfinite(t)) return false;
+  return c.K[0] != 0.0f && c.K[2] != 0.0f;
+}
+
+pm_status check_view(const pm_options* o, const pm_view_desc* v) {
+  if (!v) return fail(PM_ERR_INVALID_ARG, "null view");
+  if (v->num_src < 1) return fail(PM_ERR_INVALID_ARG, "view needs at least one source image");
+  if (v->num_src > PM_MAX_NUM_SRC) return fail(PM_ERR_TOO_MANY_SOURCES, "more than 32 source images");
+  if (!v->ref.data || v->ref.width <= 0 || v->ref.height <= 0 || v->ref.stride < v->ref.width)
+    return fail(PM_ERR_INVALID_ARG, "invalid reference image");
+  if ((int64_t)v->ref.width * v->ref.height > (int64_t)1 << 28)
+    return fail(PM_ERR_INVALID_ARG, "reference image too large");
+  if (!v->src || !v->src_cams) return fail(PM_ERR_INVALID_ARG, "null source arrays");
+  if (!finite_camera(v->ref_cam)) return fail(PM_ERR_INVALID_ARG, "non-finite or degenerate reference camera");
+  for (int i = 0; i < v->num_src; ++i) {
+    const pm_image& s = v->src[i];
+    if (!s.data || s.width <= 0 || s.height <= 0 || s.stride < s.width)
+      return fail(PM_ERR_INVALID_ARG, "invalid source image " + std::to_string(i));
+    if ((int64_t)s.width * s.height > (int64_t)1 << 28) return fail(PM_ERR_INVALID_ARG, "source image too large");
+    if (!finite_camera(v->src_cams[i])) return fail(PM_ERR_INVALID_ARG, "non-finite source camera " + std::to_string(i));
+  }
+  if (!(v->depth_min > 0.0f) || !(v->depth_max >= v->depth_min) || !std::isfinite(v->depth_max))
+    return fail(PM_ERR_INVALID_ARG, "depth range must satisfy 0 < depth_min <= depth_max");
+  if (o->geom_consistency) {
+    if (!v->src_depth_maps) return fail(PM_ERR_INVALID_ARG, "geom_consistency needs src_depth_maps");
+    for (int i = 0; i < v->num_src; ++i)
+      if (!v->src_depth_maps[i]) return fail(PM_ERR_INVALID_ARG, "missing source depth map " + std::to_string(i));
+  }
+  if (!o->init_depth_random && (!v->init_depth || !v->init_normal))
+    return fail(PM_ERR_INVALID_ARG, "init_depth_random = 0 needs init_depth and init_normal");
+  return PM_OK;
+}
+
+}  // namespace
+
+PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_view_desc* v, pm_problem** out) {
+  if (!e || !o || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  pm_status st = pm_options_check(o);
+  if (st != PM_OK) return st;
+  st = check_view(o, v);
+  if (st != PM_OK) return st;
+  PM_HIP(hipSetDevice(e->device));
+  pm_problem* p = new (std::nothrow) pm_problem();
+  if (!p) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  struct Guard {
+    pm_problem*& p;
+    bool keep = false;
+    ~Guard() {
+      if (!keep) delete p;
+    }
+  } guard{p};
+  p->engine = e;
+  p->opt = *o;
+  p->W = v->ref.width;
+  p->H = v->ref.height;
+  p->N = v->num_src;
+  p->view_id = v->view_id;
+  p->depth_min = v->depth_min;
+  p->depth_max = v->depth_max;
+  const int W = p->W, H = p->H, N = p->N;
+  const size_t plane = (size_t)W * H;
+  hipStream_t s = e->stream;
+  pm::ViewDev& h = p->host;
+  std::memset(&h, 0, sizeof(h));
+
+  // reference image and tables
+  uint8_t* ref = nullptr;
+  if ((st = p->alloc(&ref, plane)) != PM_OK) return st;
+  PM_HIP(hipMemcpy2DAsync(ref, W, v->ref.data, v->ref.stride, W, H, hipMemcpyHostToDevice, s));
+  std::vector<float> sp, ct;
+  int nwin = 0;
+  weight_tables(o, sp, ct, &nwin);
+  float *d_sp = nullptr, *d_ct = nullptr;
+  if ((st = p->alloc(&d_sp, sp.size())) != PM_OK) return st;
+  if ((st = p->alloc(&d_ct, ct.size())) != PM_OK) return st;
+  PM_HIP(hipMemcpyAsync(d_sp, sp.data(), sp.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  PM_HIP(hipMemcpyAsync(d_ct, ct.data(), ct.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  p->poses.assign((size_t)N * PM_POSE_STRIDE, 0.0f);
+  pack_poses(v, p->poses.data(), p->refk8);
+  float* d_poses = nullptr;
+  if ((st = p->alloc(&d_poses, p->poses.size())) != PM_OK) return st;
+  PM_HIP(hipMemcpyAsync(d_poses, p->poses.data(), p->poses.size() * sizeof(float), hipMemcpyHostToDevice, s));
+
+  // source images: plain upload, then the padded quad layout
+  size_t max_src = 0;
+  for (int i = 0; i < N; ++i) max_src = std::max(max_src, (size_t)v->src[i].width * v->src[i].height);
+  uint8_t* staging = nullptr;
+  PM_HIP(hipMalloc(&staging, max_src));
+  for (int i = 0; i < N; ++i) {
+    const pm_image& si = v->src[i];
+    const int pw = si.width + 2 * pm::kQuadPad, ph = si.height + 2 * pm::kQuadPad;
+    pm::SrcDev& sd = h.src[i];
+    sd.width = si.width;
+    sd.height = si.height;
+    sd.pitch = pw;
+    hipError_t he = hipMemcpy2DAsync(staging, si.width, si.data, si.stride, si.width, si.height,
+                                     hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) {
+      if (o->image_storage == PM_STORAGE_U8) {
+        uint32_t* q = nullptr;
+        if ((st = p->alloc(&q, (size_t)pw * ph)) != PM_OK) {
+          (void)hipFree(staging);
+          return st;
+        }
+        he = pm::launch_build_quads_u8(staging, si.width, si.height, si.width, q, s);
+        sd.tex = q;
+      } else {
+        uint2* q = nullptr;
+        if ((st = p->alloc(&q, (size_t)pw * ph)) != PM_OK) {
+          (void)hipFree(staging);
+          return st;
+        }
+        he = pm::launch_build_quads_f16(staging, si.width, si.height, si.width, q, s);
+        sd.tex = q;
+      }
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(s);  // staging is reused by the next source
+    if (he != hipSuccess) {
+      (void)hipFree(staging);
+      return fail(PM_ERR_DEVICE, std::string("source upload: ") + hipGetErrorString(he));
+    }
+  }
+  (void)hipFree(staging);
+
+  // source depth maps (geometric pass)
+  if (o->geom_consistency) {
+    for (int i = 0; i < N; ++i) {
+      const pm_image& si = v->src[i];
+      const size_t n = (size_t)si.width * si.height;
+      float* tmp = nullptr;
+      PM_HIP(hipMalloc(&tmp, n * sizeof(float)));
+      float4* dq = nullptr;
+      const size_t pn = (size_t)(si.width + 2 * pm::kQuadPad) * (si.height + 2 * pm::kQuadPad);
+      if ((st = p->alloc(&dq, pn)) != PM_OK) {
+        (void)hipFree(tmp);
+        return st;
+      }
+      hipError_t he = hipMemcpyAsync(tmp, v->src_depth_maps[i], n * sizeof(float), hipMemcpyHostToDevice, s);
+      if (he == hipSuccess) he = pm::launch_build_depth_quads(tmp, si.width, si.height, dq, s);
+      if (he == hipSuccess) he = hipStreamSynchronize(s);
+      (void)hipFree(tmp);
+      if (he != hipSuccess) return fail(PM_ERR_DEVICE, std::string("depth upload: ") + hipGetErrorString(he));
+      h.src[i].depth_q = dq;
+    }
+  }
+
+  // state, workspace, moments
+  float4* refm = nullptr;
+  uint8_t* grad = nullptr;
+  if ((st = p->alloc(&refm, plane)) != PM_OK) return st;
+  if ((st = p->alloc(&grad, plane)) != PM_OK) return st;
+  if ((st = p->alloc(&h.depth, plane)) != PM_OK) return st;
+  if ((st = p->alloc(&h.normal, 3 * plane)) != PM_OK) return st;
+  if ((st = p->alloc(&h.conf, plane)) != PM_OK) return st;
+  if ((st = p->alloc(&h.cost, (size_t)N * plane)) != PM_OK) return st;
+  if ((st = p->alloc(&h.view_sel, plane)) != PM_OK) return st;
+  const int slots = pm::sweep_slots(W, H);
+  if ((st = p->alloc(&h.ws, (size_t)slots * pm::ws_floats_per_slot(N))) != PM_OK) return st;
+  if ((st = p->alloc(&p->counters, 3)) != PM_OK) return st;
+  if (!o->init_depth_random) {
+    if ((st = p->alloc(&p->init_depth, plane)) != PM_OK) return st;
+    if ((st = p->alloc(&p->init_normal, 3 * plane)) != PM_OK) return st;
+    PM_HIP(hipMemcpyAsync(p->init_depth, v->init_depth, plane * sizeof(float), hipMemcpyHostToDevice, s));
+    PM_HIP(hipMemcpyAsync(p->init_normal, v->init_normal, 3 * plane * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+
+  h.W = W;
+  h.H = H;
+  h.N = N;
+  h.nwin = nwin;
+  h.radius = o->window_radius;
+  h.step = o->window_step;
+  h.view_id = (int32_t)v->view_id;
+  h.geom = o->geom_consistency ? 1 : 0;
+  std::memcpy(h.refK, p->refk8, 4 * sizeof(float));
+  std::memcpy(h.refInvK, p->refk8 + 4, 4 * sizeof(float));
+  h.depth_min = v->depth_min;
+  h.depth_max = v->depth_max;
+  h.depth_max_eff = (float)(1.2 * (double)v->depth_max);
+  h.geom_regularizer = o->geom_consistency_regularizer;
+  h.geom_max_cost = o->geom_consistency_max_cost;
+  h.random_depth_ratio = o->random_depth_ratio;
+  h.angle1 = std::fmin(o->random_angle1_range, PM_PI_F);
+  h.angle2 = std::fmin(o->random_angle2_range, PM_HALF_PI_F);
+  h.max_ncc_matching_cost = o->max_ncc_matching_cost;
+  h.inv_beta2 = 0.5f / (o->beta * o->beta);
+  h.num_good = o->num_good_hypothesis;
+  h.num_bad = o->num_bad_hypothesis;
+  h.num_samples = o->num_samples;
+  h.random_optimization = o->random_optimization ? 1 : 0;
+  h.thk = o->thk;
+  h.init_samples = o->init_samples;
+  h.init_random = o->init_depth_random ? 1 : 0;
+  h.compat_lo_geo_bug = o->compat_lo_geo_bug ? 1 : 0;
+  h.accum_mode = o->accum_mode;
+  h.slots_per_view = slots;
+  h.conf_threshold = o->conf_threshold;
+  h.depth_diff_threshold = o->depth_diff_threshold;
+  h.seed = o->seed;
+  h.ref = ref;
+  h.refm = refm;
+  h.grad_ok = grad;
+  h.poses = d_poses;
+  h.sp = d_sp;
+  h.ct = d_ct;
+  h.counters = nullptr;
+  if ((st = p->alloc(&p->dev, 1)) != PM_OK) return st;
+  PM_HIP(hipMemcpyAsync(p->dev, &h, sizeof(h), hipMemcpyHostToDevice, s));
+  PM_HIP(pm::launch_ref_moments(p->dev, W, H, o->accum_mode, refm, grad, s));
+  PM_HIP(hipStreamSynchronize(s));
+  guard.keep = true;
+  *out = p;
+  return PM_OK;
+}
+
+PM_API void pm_problem_destroy(pm_problem* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->engine->device);
+  (void)hipStreamSynchronize(p->engine->stream);
+  delete p;
+}
+
+namespace {
+
+struct EventTimer {
+  std::vector<hipEvent_t> ev;
+  pm_status rec(hipStream_t s) {
+    hipEvent_t e;
+    PM_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+    PM_HIP(hipEventRecord(e, s));
+    return PM_OK;
+  }
+  float between(size_t a, size_t b) const {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) return 0.0f;
+    return ms;
+  }
+  ~EventTimer() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+PM_API pm_status pm_problem_run(pm_problem* p) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  pm_engine* e = p->engine;
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  const int W = p->W, H = p->H;
+  const size_t plane = (size_t)W * H;
+  const pm_options& o = p->opt;
+  pm::LaunchCfg cfg{o.image_storage, o.rcp_mode, o.window_radius, o.window_step};
+  // counters on/off for this run
+  unsigned long long* cnt = e->counting ? p->counters : nullptr;
+  if (cnt != p->host.counters) {
+    p->host.counters = cnt;
+    PM_HIP(hipMemcpyAsync(p->dev, &p->host, sizeof(p->host), hipMemcpyHostToDevice, s));
+  }
+  if (cnt) PM_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), s));
+  if (!o.init_depth_random) {
+    PM_HIP(hipMemcpyAsync(p->host.depth, p->init_depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+    PM_HIP(hipMemcpyAsync(p->host.normal, p->init_normal, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  const uint32_t goff = o.geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
+  EventTimer t;
+  pm_status st;
+  std::vector<std::pair<size_t, int>> sweeps, locals;  // event index pairs
+  if ((st = t.rec(s)) != PM_OK) return st;                      // 0: start
+  PM_HIP(pm::launch_init(p->dev, cfg, W, H, PM_PASS_INIT + goff, s));
+  if ((st = t.rec(s)) != PM_OK) return st;                      // 1: init done
+  for (int iter = 0; iter < o.num_iterations; ++iter) {
+    const float th_mc =
+        (float)((double)o.init_ncc_matching_cost * std::exp((double)(-(float)(iter * iter) / o.alpha)));
+    for (int color = 0; color < 2; ++color) {
+      const size_t a = t.ev.size();
+      if ((st = t.rec(s)) != PM_OK) return st;
+      PM_HIP(pm::launch_sweep(p->dev, cfg, W, H, color, th_mc, PM_PASS_SWEEP(iter, color) + goff, s));
+      if ((st = t.rec(s)) != PM_OK) return st;
+      sweeps.push_back({a, 0});
+    }
+    if (o.local_optimization && iter > 2) {
+      for (int color = 0; color < 2; ++color) {
+        const size_t a = t.ev.size();
+        if ((st = t.rec(s)) != PM_OK) return st;
+        PM_HIP(pm::launch_local_opt(p->dev, cfg, W, H, color, PM_PASS_LOCAL(iter, color) + goff, s));
+        if ((st = t.rec(s)) != PM_OK) return st;
+        locals.push_back({a, 0});
+      }
+    }
+  }
+  if (o.filter && o.conf_filter) {
+    for (int color = 0; color < 2; ++color) PM_HIP(pm::launch_filter(p->dev, W, H, color, s));
+  }
+  if ((st = t.rec(s)) != PM_OK) return st;
+  PM_HIP(hipStreamSynchronize(s));
+  std::memset(&p->stats, 0, sizeof(p->stats));
+  p->stats.total_ms = t.between(0, t.ev.size() - 1);
+  p->stats.init_ms = t.between(0, 1);
+  for (auto& pr : sweeps) p->stats.sweep_ms += t.between(pr.first, pr.first + 1);
+  for (auto& pr : locals) p->stats.local_opt_ms += t.between(pr.first, pr.first + 1);
+  p->stats.sweep_launches = (int32_t)sweeps.size();
+  p->stats.local_opt_launches = (int32_t)locals.size();
+  if (cnt) {
+    unsigned long long c[3];
+    PM_HIP(hipMemcpy(c, cnt, sizeof(c), hipMemcpyDeviceToHost));
+    p->stats.ncc_evaluations = c[0];
+    p->stats.ncc_samples = c[1];
+    p->stats.geo_evaluations = c[2];
+  }
+  return PM_OK;
+}
+
+namespace {
+
+pm_status copy_out(const pm_problem* p, void* dst, const void* src, size_t bytes) {
+  if (!p || !dst) return fail(PM_ERR_INVALID_ARG, "null argument");
+  PM_HIP(hipSetDevice(p->engine->device));
+  PM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->engine->stream));
+  PM_HIP(hipStreamSynchronize(p->engine->stream));
+  return PM_OK;
+}
+
+}  // namespace
+
+PM_API pm_status pm_problem_get_depth_map(const pm_problem* p, float* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return copy_out(p, out, p->host.depth, (size_t)p->W * p->H * sizeof(float));
+}
+PM_API pm_status pm_problem_get_normal_map(const pm_problem* p, float* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return copy_out(p, out, p->host.normal, 3 * (size_t)p->W * p->H * sizeof(float));
+}
+PM_API pm_status pm_problem_get_conf_map(const pm_problem* p, float* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return copy_out(p, out, p->host.conf, (size_t)p->W * p->H * sizeof(float));
+}
+PM_API pm_status pm_problem_get_cost_map(const pm_problem* p, float* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return copy_out(p, out, p->host.cost, (size_t)p->N * p->W * p->H * sizeof(float));
+}
+PM_API pm_status pm_problem_get_view_sel(const pm_problem* p, uint32_t* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return copy_out(p, out, p->host.view_sel, (size_t)p->W * p->H * sizeof(uint32_t));
+}
+PM_API pm_status pm_problem_get_stats(const pm_problem* p, pm_run_stats* out) {
+  if (!p || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
+  *out = p->stats;
+  return PM_OK;
+}
+PM_API pm_status pm_problem_get_poses(const pm_problem* p, float* poses, float ref_k8[8]) {
+  if (!p || !poses || !ref_k8) return fail(PM_ERR_INVALID_ARG, "null argument");
+  std::memcpy(poses, p->poses.data(), p->poses.size() * sizeof(float));
+  std::memcpy(ref_k8, p->refk8, sizeof(p->refk8));
+  return PM_OK;
+}
+
+PM_API pm_status pm_run_view(pm_engine* e, const pm_options* o, const pm_view_desc* v, const pm_outputs* out) {
+  if (!out) return fail(PM_ERR_INVALID_ARG, "null outputs");
+  pm_problem* p = nullptr;
+  pm_status st = pm_problem_create(e, o, v, &p);
+  if (st != PM_OK) return st;
+  st = pm_problem_run(p);
+  if (st == PM_OK && out->depth) st = pm_problem_get_depth_map(p, out->depth);
+  if (st == PM_OK && out->normal) st = pm_problem_get_normal_map(p, out->normal);
+  if (st == PM_OK && out->conf) st = pm_problem_get_conf_map(p, out->conf);
+  if (st == PM_OK && out->cost) st = pm_problem_get_cost_map(p, out->cost);
+  if (st == PM_OK && out->view_sel) st = pm_problem_get_view_sel(p, out->view_sel);
+  pm_problem_destroy(p);
+  return st;
+}

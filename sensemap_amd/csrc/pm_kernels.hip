@@ -1,0 +1,1185 @@
+// pm_kernels.hip -- gfx950 kernels of the PatchMatch MVS engine.
+//
+// One thread per pixel of one checkerboard colour.  A 256-thread workgroup
+// owns a 32x16 pixel tile (its 256 pixels of the colour); each wave64 covers
+// 4 rows x 16 same-colour pixels, so the 64 windows of a wave overlap and
+// their source footprints hit the same cache lines.  Per workgroup the
+// reference tile (+ window halo, u8) and the 256-entry bilateral colour
+// table are staged in LDS; the spatial weight table and the per-source pose
+// blocks are read with uniform (scalar) loads because the source loop and
+// the window position are uniform across the wave.  Source images live in
+// HBM in a zero-padded "quad" layout: texel (x, y) holds the 2x2 bilinear
+// footprint {(x,y), (x+1,y), (x,y+1), (x+1,y+1)}, so one 32-bit (u8) or
+// 64-bit (f16) load feeds one bilinear sample with no bounds checks.
+//
+// Per-pixel arithmetic follows the canonical FP32 order of the CPU
+// transcription (oracle/pm_oracle.c): -ffp-contract=off, explicit fmaf,
+// correctly rounded division and sqrt, project math from pm_math.h.  With
+// PM_RCP_EXACT the maps are bitwise equal to the transcription's.
+//
+// Reference semantics (src/mvs/kernel_functions.cu):
+//   k_ref_moments   <- FilterKernel (gpu_mat_ref_image.cu:16-56) + ComputeGradientMap (:1655-1675)
+//   k_init          <- FillWithRandomNumbers/InitNormalMap/InitDepthMap (:689-737) + InitConfMapKernel (:1497-1601)
+//   k_sweep         <- Black/RedPixelUpdateOpt -> SweepFromCheckerBoardOpt (:2136-2560, :2634-2734)
+//   k_local_opt     <- Black/RedPixelLocalOptimization -> LocalOptimization (:2736-3017)
+//   k_filter        <- FilterDepthMap (:3282-3354)
+//   ncc_eval        <- ComputeNCCCost<R,S> (:1245-1340) with ComposeHomography (:431-503)
+//   geom_eval       <- ComputeGeomConsistencyCostShared (:505-568)
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "pm_device.h"
+#include "pm_math.h"
+
+namespace pm {
+namespace {
+
+// Pointers stored in ViewDev are generic; cast them to the global address
+// space so loads and stores compile to global_* (or s_load for uniform
+// addresses) instead of flat_*.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+// 16-byte load of a float4 array element through the global address space.
+__device__ __forceinline__ f4v ld_f4(const float4* p, int idx) {
+  return *((const __attribute__((address_space(1))) f4v*)p + idx);
+}
+
+__constant__ int c_neighbor_offs[4][2] = {{0, -1}, {-1, 0}, {0, 1}, {1, 0}};
+__constant__ int c_dir_samples[8][20] = {
+    {-1, -2, 1, -2, -2, -3, 2, -3, -3, -4, 3, -4, -4, -5, 4, -5, -5, -6, 5, -6},
+    {-1, 2, 1, 2, -2, 3, 2, 3, -3, 4, 3, 4, -4, 5, 4, 5, -5, 6, 5, 6},
+    {-2, -1, -2, 1, -3, -2, -3, 2, -4, -3, -4, 3, -5, -4, -5, 4, -6, -5, -6, 5},
+    {2, -1, 2, 1, 3, -2, 3, 2, 4, -3, 4, 3, 5, -4, 5, 4, 6, -5, 6, 5},
+    {0, -3, 0, -5, 0, -7, 0, -9, 0, -11, 0, -13, 0, -15, 0, -17, 0, -19, 0, -21},
+    {0, 3, 0, 5, 0, 7, 0, 9, 0, 11, 0, 13, 0, 15, 0, 17, 0, 19, 0, 21},
+    {-3, 0, -5, 0, -7, 0, -9, 0, -11, 0, -13, 0, -15, 0, -17, 0, -19, 0, -21, 0},
+    {3, 0, 5, 0, 7, 0, 9, 0, 11, 0, 13, 0, 15, 0, 17, 0, 19, 0, 21, 0}};
+__constant__ float c_mweights[5] = {0.f, 1.f, 2.01f, 3.03f, 4.06f};
+
+// ---------------------------------------------------------------------------
+// per-lane pixel context
+
+struct Pix {
+  int row, col, lrow, lcol;
+  int p;                 // row * W + col
+  int c;                 // ref centre intensity (u8)
+  float mu_r, var_r, inv_sw, err_r;
+  float np0, np1;        // normalized point (x, y, 1)
+  bool shifted;          // compensated moments for this pixel
+};
+
+// Workspace accessors (SoA over slots).
+struct Ws {
+  float* base;
+  int slots, slot, N;
+  __device__ __attribute__((address_space(1))) float& hyp(int k, int c) const { return G(base)[(size_t)(4 * k + c) * slots + slot]; }
+  __device__ __attribute__((address_space(1))) float& M(int k, int i) const {
+    return G(base)[(size_t)(4 * (kMaxHyp + kNumRefine) + k * N + i) * slots + slot];
+  }
+  __device__ __attribute__((address_space(1))) float& RM(int k, int i) const {
+    return G(base)[(size_t)(4 * (kMaxHyp + kNumRefine) + kMaxHyp * N + k * N + i) * slots + slot];
+  }
+  __device__ __attribute__((address_space(1))) float& Wt(int i) const {
+    return G(base)[(size_t)(4 * (kMaxHyp + kNumRefine) + (kMaxHyp + kNumRefine) * N + i) * slots + slot];
+  }
+};
+
+struct Counters {
+  unsigned long long ncc_eval = 0, ncc_samples = 0, geo_eval = 0;
+};
+
+__device__ __forceinline__ void flush_counters(const ViewDev* v, const Counters& c) {
+  if (v->counters) {
+    atomicAdd(&v->counters[0], c.ncc_eval);
+    atomicAdd(&v->counters[1], c.ncc_samples);
+    atomicAdd(&v->counters[2], c.geo_eval);
+  }
+}
+
+template <int RCP>
+__device__ __forceinline__ float rcp_z(float z) {
+  if constexpr (RCP == PM_RCP_EXACT) {
+    return 1.0f / z;
+  } else {
+    float r = __builtin_amdgcn_rcpf(z);
+    const float e = fmaf(-z, r, 1.0f);
+    return fmaf(r, e, r);
+  }
+}
+
+// Fetch the 2x2 footprint of source texel (x0, y0) from the quad layout.
+// tex is uniform across the wave (the source loop is uniform), off is the
+// 32-bit texel index, so the load uses the scalar-base + 32-bit-offset form.
+template <int STORE>
+__device__ __forceinline__ void fetch_quad(const void* tex, uint32_t off, float& t0, float& t1, float& t2,
+                                           float& t3) {
+  const __attribute__((address_space(1))) char* b = G((const char*)tex);
+  if constexpr (STORE == PM_STORAGE_U8) {
+    const uint32_t q = *(const __attribute__((address_space(1))) uint32_t*)(b + (off << 2));
+    t0 = (float)(q & 0xffu);
+    t1 = (float)((q >> 8) & 0xffu);
+    t2 = (float)((q >> 16) & 0xffu);
+    t3 = (float)(q >> 24);
+  } else {
+    const u2v q = *(const __attribute__((address_space(1))) u2v*)(b + (off << 3));
+    t0 = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
+    t1 = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
+    t2 = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
+    t3 = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
+  }
+}
+
+__device__ __forceinline__ void neumaier_add(float& s, float& c, float x) {
+  const float t = s + x;
+  c = c + ((fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s));
+  s = t;
+}
+
+// ComputeNCCCost (kernel_functions.cu:1245-1340) for source i (uniform) and
+// the lane's hypothesis.  tile points at the LDS ref tile entry of the
+// window's top-left sample; tw is the tile row length.
+template <int R_, int S_, int STORE, int RCP>
+__device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, const Pix& px,
+                                          const uint8_t* tile, int tw, const float* ct, const float* sp,
+                                          float nx, float ny, float nz, float d, Counters& cnt) {
+  const int R = R_ >= 0 ? R_ : v->radius;
+  const int S = R_ >= 0 ? S_ : v->step;
+  const int n = R_ >= 0 ? (2 * R_) / S_ + 1 : v->nwin;
+  cnt.ncc_eval++;
+  const __attribute__((address_space(1))) float* pose = G(v->poses) + i * PM_POSE_STRIDE;
+  const float K0 = pose[0], K1 = pose[1], K2 = pose[2], K3 = pose[3];
+  const __attribute__((address_space(1))) float* Rm = pose + 4;
+  const __attribute__((address_space(1))) float* T = pose + 13;
+  const float* iK = v->refInvK;
+  const float fcol = (float)px.col, frow = (float)px.row;
+  const float ray0 = iK[0] * fcol + iK[1];
+  const float ray1 = iK[2] * frow + iK[3];
+  const float dist = d * (nx * ray0 + ny * ray1 + nz);
+  const float inv_dist = 1.0f / dist;
+  const float nd0 = inv_dist * nx, nd1 = inv_dist * ny, nd2 = inv_dist * nz;
+  float A[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    A[3 * r + 0] = Rm[3 * r + 0] + nd0 * T[r];
+    A[3 * r + 1] = Rm[3 * r + 1] + nd1 * T[r];
+    A[3 * r + 2] = Rm[3 * r + 2] + nd2 * T[r];
+  }
+  float B[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    B[j] = K0 * A[j] + K1 * A[6 + j];
+    B[3 + j] = K2 * A[3 + j] + K3 * A[6 + j];
+    B[6 + j] = A[6 + j];
+  }
+  float H[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    H[3 * r + 0] = B[3 * r + 0] * iK[0];
+    H[3 * r + 1] = B[3 * r + 1] * iK[2];
+    H[3 * r + 2] = B[3 * r + 0] * iK[1] + B[3 * r + 1] * iK[3] + B[3 * r + 2];
+  }
+  const float zc = H[6] * fcol + H[7] * frow + H[8];
+  const float izc = 1.0f / zc;
+  const float xf = (H[0] * fcol + H[1] * frow + H[2]) * izc;
+  const float yf = (H[3] * fcol + H[4] * frow + H[5]) * izc;
+  const SrcDev& s = v->src[i];
+  const float Wsf = (float)s.width, Hsf = (float)s.height;
+  if (!(xf > -1.0f && xf < Wsf && yf > -1.0f && yf < Hsf)) return kMaxCost;
+  if (px.var_r < kVarThreshold) return kMaxCost;
+  const float c0 = (float)(px.col - R), r0 = (float)(px.row - R);
+  float bx = H[0] * c0 + H[1] * r0 + H[2];
+  float by = H[3] * c0 + H[4] * r0 + H[5];
+  float bz = H[6] * c0 + H[7] * r0 + H[8];
+  const float fs = (float)S;
+  const float hxc = fs * H[0], hyc = fs * H[3], hzc = fs * H[6];
+  const float hxr = fs * H[1], hyr = fs * H[4], hzr = fs * H[7];
+  const float umax = Wsf + 2.0f, vmax = Hsf + 2.0f;
+  const int pitch = s.pitch;
+  const void* tex = s.tex;
+  const int c = px.c;
+  float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  float e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
+  const float mu_r = px.mu_r;
+  const uint32_t upitch = (uint32_t)pitch;
+  // One window walk; SHIFTED selects the compensated moment update.
+  auto walk = [&](auto shifted_tag) __attribute__((always_inline)) {
+    constexpr bool kShifted = decltype(shifted_tag)::value;
+    for (int ir = 0; ir < n; ++ir) {
+      float x = bx, y = by, z = bz;
+      const uint8_t* trow = tile + ir * S * tw;
+      const float* sprow = sp + ir * n;
+      auto sample = [&](int j) __attribute__((always_inline)) {
+        const float rz = rcp_z<RCP>(z);
+        float u = x * rz, w_ = y * rz;
+        u = fminf(fmaxf(u, -3.0f), umax);
+        w_ = fminf(fmaxf(w_, -3.0f), vmax);
+        const float x0f = floorf(u), y0f = floorf(w_);
+        const float fx = u - x0f, fy = w_ - y0f;
+        const uint32_t off = __umul24((uint32_t)((int)y0f + kQuadPad), upitch) + (uint32_t)((int)x0f + kQuadPad);
+        float t0, t1, t2, t3;
+        fetch_quad<STORE>(tex, off, t0, t1, t2, t3);
+        const float top = fmaf(fx, t1 - t0, t0);
+        const float bot = fmaf(fx, t3 - t2, t2);
+        const float sv = fmaf(fy, bot - top, top);
+        const uint32_t ri = trow[j * S];
+        const float r = (float)ri;
+        const float w = sprow[j] * ct[__sad((uint32_t)c, ri, 0u)];
+        if constexpr (!kShifted) {
+          const float ws = w * sv;
+          s1 = s1 + ws;
+          s2 = fmaf(ws, sv, s2);
+          s3 = fmaf(ws, r, s3);
+        } else {
+          const float dd = sv - mu_r;
+          const float rd = r - mu_r;
+          const float wd = w * dd;
+          neumaier_add(s1, e1, wd);
+          neumaier_add(s2, e2, wd * dd);
+          neumaier_add(s3, e3, wd * rd);
+        }
+        x = x + hxc;
+        y = y + hyc;
+        z = z + hzc;
+      };
+      if constexpr (R_ >= 0) {
+        constexpr int kN = R_ >= 0 ? (2 * R_) / S_ + 1 : 1;
+#pragma unroll
+        for (int j = 0; j < kN; ++j) sample(j);
+      } else {
+        for (int j = 0; j < n; ++j) sample(j);
+      }
+      bx = bx + hxr;
+      by = by + hyr;
+      bz = bz + hzr;
+    }
+  };
+  const bool shifted = px.shifted;
+  if (!shifted) {
+    walk(std::false_type{});
+  } else {
+    walk(std::true_type{});
+  }
+  cnt.ncc_samples += (unsigned long long)(n * n);
+  const float inv_sw = px.inv_sw;
+  float var_s, cov;
+  if (!shifted) {
+    const float mu_s = s1 * inv_sw;
+    const float m2_s = s2 * inv_sw;
+    const float m_rs = s3 * inv_sw;
+    var_s = m2_s - mu_s * mu_s;
+    cov = m_rs - mu_r * mu_s;
+  } else {
+    const float m1 = (s1 + e1) * inv_sw;
+    const float m2 = (s2 + e2) * inv_sw;
+    const float m3 = (s3 + e3) * inv_sw;
+    var_s = m2 - m1 * m1;
+    cov = m3 - px.err_r * m1;
+  }
+  if (var_s < kVarThreshold) return kMaxCost;
+  const float val = 1.0f - cov / sqrtf(px.var_r * var_s);
+  return fmaxf(0.0f, fminf(kMaxCost, val));
+}
+
+// ComputeGeomConsistencyCostShared (kernel_functions.cu:505-568).
+__device__ __forceinline__ float geom_eval(const ViewDev* __restrict__ v, int i, const Pix& px, float depth,
+                                           const float* K, Counters& cnt) {
+  cnt.geo_eval++;
+  const __attribute__((address_space(1))) float* P = G(v->poses) + i * PM_POSE_STRIDE + 16;
+  const __attribute__((address_space(1))) float* iP = G(v->poses) + i * PM_POSE_STRIDE + 28;
+  const float fx = depth * px.np0, fy = depth * px.np1, fz = depth * 1.0f;
+  const float inv_fz = 1.0f / (P[8] * fx + P[9] * fy + P[10] * fz + P[11]);
+  float sc = inv_fz * (P[0] * fx + P[1] * fy + P[2] * fz + P[3]);
+  float sr = inv_fz * (P[4] * fx + P[5] * fy + P[6] * fz + P[7]);
+  const SrcDev& s = v->src[i];
+  float u = fminf(fmaxf(sc, -3.0f), (float)s.width + 2.0f);
+  float w_ = fminf(fmaxf(sr, -3.0f), (float)s.height + 2.0f);
+  const float x0f = floorf(u), y0f = floorf(w_);
+  const float fxx = u - x0f, fyy = w_ - y0f;
+  const f4v q = ld_f4(s.depth_q, ((int)y0f + kQuadPad) * s.pitch + ((int)x0f + kQuadPad));
+  const float top = fmaf(fxx, q.y - q.x, q.x);
+  const float bot = fmaf(fxx, q.w - q.z, q.z);
+  const float sd = fmaf(fyy, bot - top, top);
+  if (sd == 0.0f) return v->geom_max_cost;
+  sc = sc * sd;
+  sr = sr * sd;
+  const float bx = iP[0] * sc + iP[1] * sr + iP[2] * sd + iP[3];
+  const float by = iP[4] * sc + iP[5] * sr + iP[6] * sd + iP[7];
+  const float bz = iP[8] * sc + iP[9] * sr + iP[10] * sd + iP[11];
+  const float ibz = 1.0f / bz;
+  const float bcol = ibz * K[0] * bx + K[1];
+  const float brow = ibz * K[2] * by + K[3];
+  const float dc = (float)px.col - bcol;
+  const float dr = (float)px.row - brow;
+  return fminf(v->geom_max_cost, sqrtf(dc * dc + dr * dr));
+}
+
+// ComputeMultiViewCostVector (kernel_functions.cu:1447-1495), priors off.
+template <int R_, int S_, int STORE, int RCP>
+__device__ __forceinline__ float mv_cost(const ViewDev* __restrict__ v, int i, const Pix& px, const uint8_t* tile,
+                                         int tw, const float* ct, const float* sp, float nx, float ny, float nz,
+                                         float d, const float* Kgeo, Counters& cnt) {
+  float cost = ncc_eval<R_, S_, STORE, RCP>(v, i, px, tile, tw, ct, sp, nx, ny, nz, d, cnt);
+  if (v->geom) cost = cost + v->geom_regularizer * geom_eval(v, i, px, d, Kgeo, cnt);
+  return cost;
+}
+
+// ---------------------------------------------------------------------------
+// random hypotheses (identical to the transcription)
+
+__device__ __forceinline__ void random_normal(pm_rng* g, float r0, float r1, float n[3]) {
+  float v1 = 0.0f, v2 = 0.0f, s = 2.0f;
+  for (int t = 0; t < 64 && s >= 1.0f; ++t) {
+    v1 = 2.0f * pm_rng_uniform(g) - 1.0f;
+    v2 = 2.0f * pm_rng_uniform(g) - 1.0f;
+    s = v1 * v1 + v2 * v2;
+  }
+  if (s >= 1.0f) {
+    v1 = 0.0f;
+    v2 = 0.0f;
+    s = 0.0f;
+  }
+  const float s_norm = 2.0f * sqrtf(1.0f - s);
+  n[0] = v1 * s_norm;
+  n[1] = v2 * s_norm;
+  n[2] = 1.0f - 2.0f * s;
+  if (n[0] * r0 + n[1] * r1 + n[2] * 1.0f > 0.0f) {
+    n[0] = -n[0];
+    n[1] = -n[1];
+    n[2] = -n[2];
+  }
+}
+
+__device__ __forceinline__ float random_depth(pm_rng* g, float dmin, float dmax) {
+  return pm_rng_uniform(g) * (dmax - dmin) + dmin;
+}
+
+__device__ __forceinline__ void perturb_normal(pm_rng* g, const float n[3], float a1, float a2, float out[3]) {
+  const float y = pm_atan2f(sqrtf(n[0] * n[0] + n[1] * n[1]), n[2]);
+  const float x = pm_atan2f(n[1], n[0]);
+  const float nx = a1 * (2.0f * pm_rng_uniform(g) - 1.0f) + x;
+  const float ny = a2 * (2.0f * pm_rng_uniform(g) - 1.0f) + y;
+  const float siny = pm_sinf(ny);
+  out[0] = pm_cosf(nx) * siny;
+  out[1] = pm_sinf(nx) * siny;
+  out[2] = pm_cosf(ny);
+}
+
+// CorrectNormal (kernel_functions.cu:227-306) with the reference's matrix.
+__device__ __forceinline__ void correct_normal(float n[3], const float vd[3], float inv_view_len) {
+  const float cosl = n[0] * vd[0] + n[1] * vd[1] + n[2] * vd[2];
+  if (!(cosl >= 0.0f)) return;
+  float c[3];
+  c[0] = n[1] * vd[2] - n[2] * vd[1];
+  c[1] = n[2] * vd[0] - n[0] * vd[2];
+  c[2] = n[0] * vd[1] - n[1] * vd[0];
+  const float a = cosl * inv_view_len;
+  float phi;
+  if (!(a <= 1.0f)) {
+    phi = -0.001f;
+  } else {
+    phi = pm_acosf(a) - PM_HALF_PI_F;
+    phi = fminf(phi * 1.01f, -0.001f);
+  }
+  const float inv_len = 1.0f / sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  c[0] = c[0] * inv_len;
+  c[1] = c[1] * inv_len;
+  c[2] = c[2] * inv_len;
+  const float o[9] = {0.0f, -c[2], c[1], c[2], 0.0f, -c[0], -c[1], c[0], 0.0f};
+  const float sp = pm_sinf(phi);
+  const float cp = 1.0f - pm_cosf(phi);
+  float oo[9];
+  oo[0] = cp * (o[0] * o[0] + o[1] * o[3] + o[2] * o[6]);
+  oo[1] = cp * (o[0] * o[1] + o[1] * o[4] + o[2] * o[7]);
+  oo[2] = cp * (o[0] * o[2] + o[1] * o[5] + o[2] * o[8]);
+  oo[3] = cp * (o[3] * o[0] + o[4] * o[3] + o[5] * o[6]);
+  oo[4] = oo[3];
+  oo[5] = oo[3];
+  oo[6] = cp * (o[6] * o[0] + o[7] * o[3] + o[8] * o[6]);
+  oo[7] = oo[6];
+  oo[8] = oo[6];
+  float Rr[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+    Rr[k] = id + o[k] * sp + oo[k];
+  }
+  const float r0 = Rr[0] * n[0] + Rr[1] * n[1] + Rr[2] * n[2];
+  const float r1 = Rr[3] * n[0] + Rr[4] * n[1] + Rr[5] * n[2];
+  const float r2 = Rr[6] * n[0] + Rr[7] * n[1] + Rr[8] * n[2];
+  n[0] = r0;
+  n[1] = r1;
+  n[2] = r2;
+}
+
+__device__ __forceinline__ float propagate_depth(float d1, const float n1[3], const float* iK, int row1, int col1,
+                                                 float np0, float np1) {
+  const float x1 = d1 * (iK[0] * (float)col1 + iK[1]);
+  const float y1 = d1 * (iK[2] * (float)row1 + iK[3]);
+  const float z1 = d1;
+  const float dd = n1[0] * x1 + n1[1] * y1 + n1[2] * z1;
+  const float t = dd / (n1[0] * np0 + n1[1] * np1 + n1[2] * 1.0f);
+  if (fabsf(t) < 1e-5f) return d1;
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// workgroup set-up shared by the colour-pass kernels
+
+template <int R_, int S_>
+struct SpSize {
+  static constexpr int kFloats = R_ >= 0 ? ((2 * R_) / S_ + 1) * ((2 * R_) / S_ + 1)
+                                         : (2 * PM_MAX_WINDOW_RADIUS + 1) * (2 * PM_MAX_WINDOW_RADIUS + 1);
+};
+
+template <int R_>
+struct TileSize {
+  static constexpr int kBytes = R_ >= 0 ? (kTileH + 2 * R_) * (kTileW + 2 * R_)
+                                        : (kTileH + 2 * PM_MAX_WINDOW_RADIUS) * (kTileW + 2 * PM_MAX_WINDOW_RADIUS);
+};
+
+// Stage the colour table and the ref tile (zero outside the image) into
+// LDS, then map the thread to its pixel.  Returns false for threads whose
+// pixel lies outside the image (after the barrier).
+__device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int color, int R, uint8_t* tile, float* ct,
+                                           float* spl, Pix& px, int& tw) {
+  const int t = threadIdx.x;
+  ct[t] = G(v->ct)[t];
+  const int nsp = v->nwin * v->nwin;
+  for (int e = t; e < nsp; e += kSweepThreads) spl[e] = G(v->sp)[e];
+  const int W = v->W, H = v->H;
+  const int row0 = blockIdx.y * kTileH, col0 = blockIdx.x * kTileW;
+  tw = kTileW + 2 * R;
+  const int th = kTileH + 2 * R;
+  for (int e = t; e < tw * th; e += kSweepThreads) {
+    const int rr = row0 - R + e / tw, cc = col0 - R + e % tw;
+    tile[e] = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? (uint8_t)0 : G(v->ref)[rr * W + cc];
+  }
+  __syncthreads();
+  px.lrow = t >> 4;
+  px.row = row0 + px.lrow;
+  px.lcol = 2 * (t & 15) + ((px.row + color) & 1);
+  px.col = col0 + px.lcol;
+  if (px.row >= H || px.col >= W) return false;
+  px.p = px.row * W + px.col;
+  const f4v m = ld_f4(v->refm, px.p);
+  px.mu_r = m.x;
+  px.var_r = m.y;
+  px.inv_sw = m.z;
+  px.err_r = m.w;
+  px.c = tile[(px.lrow + R) * tw + px.lcol + R];
+  px.shifted = (v->accum_mode == PM_ACCUM_COMPENSATED) && (px.var_r < kLowTexVar);
+  px.np0 = v->refInvK[0] * (float)px.col + v->refInvK[1];
+  px.np1 = v->refInvK[2] * (float)px.row + v->refInvK[3];
+  return true;
+}
+
+__device__ __forceinline__ int slot_index() {
+  return (blockIdx.y * gridDim.x + blockIdx.x) * kSweepThreads + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// init: random (best of init_samples) or given depth/normal, then InitConf
+
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restrict__ v, int color, uint32_t pass) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const int R = R_ >= 0 ? R_ : v->radius;
+  Pix px;
+  int tw;
+  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const int N = v->N;
+  const int plane = v->W * v->H;
+  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Counters cnt;
+  pm_rng g;
+  pm_rng_init(&g, v->seed, (uint32_t)v->view_id, (uint32_t)px.p, pass);
+  const int thk = v->thk < N ? v->thk : N;
+  float hd, hn[3];
+  float best_conf = 0.0f;
+  uint32_t best_sel = 0;
+  float bd = 0.0f, bn[3] = {0.0f, 0.0f, 0.0f};
+  const int K = v->init_random ? (v->init_samples > 0 ? v->init_samples : 1) : 1;
+  for (int smp = 0; smp < K; ++smp) {
+    if (v->init_random) {
+      hd = random_depth(&g, v->depth_min, v->depth_max);
+      random_normal(&g, px.np0, px.np1, hn);
+    } else {
+      hd = G(v->depth)[px.p];
+      hn[0] = G(v->normal)[px.p];
+      hn[1] = G(v->normal)[plane + px.p];
+      hn[2] = G(v->normal)[2 * plane + px.p];
+      if (hd <= 0.0f) {
+        hd = random_depth(&g, v->depth_min, v->depth_max);
+        random_normal(&g, px.np0, px.np1, hn);
+      }
+      if (hn[0] * px.np0 + hn[1] * px.np1 + hn[2] * 1.0f >= 0.0f) random_normal(&g, px.np0, px.np1, hn);
+    }
+    for (int i = 0; i < N; ++i) {
+      ws.M(0, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, hn[0], hn[1], hn[2], hd, v->refK, cnt);
+    }
+    // thk smallest costs in stable ascending order (= the reference's
+    // bubble sort, kernel_functions.cu:1588-1597)
+    float conf = 0.0f;
+    uint32_t taken = 0;
+    for (int j = 0; j < thk; ++j) {
+      int bi = -1;
+      float bv = 0.0f;
+      for (int i = 0; i < N; ++i) {
+        if ((taken >> i) & 1u) continue;
+        const float cv = ws.M(0, i);
+        if (bi < 0 || cv < bv) {
+          bi = i;
+          bv = cv;
+        }
+      }
+      taken |= 1u << bi;
+      conf = conf + bv;
+    }
+    conf = conf / c_mweights[thk];
+    if (smp == 0 || conf < best_conf) {
+      best_conf = conf;
+      best_sel = taken;
+      bd = hd;
+      bn[0] = hn[0];
+      bn[1] = hn[1];
+      bn[2] = hn[2];
+      for (int i = 0; i < N; ++i) ws.RM(0, i) = ws.M(0, i);
+    }
+  }
+  G(v->depth)[px.p] = bd;
+  G(v->normal)[px.p] = bn[0];
+  G(v->normal)[plane + px.p] = bn[1];
+  G(v->normal)[2 * plane + px.p] = bn[2];
+  G(v->conf)[px.p] = best_conf;
+  G(v->view_sel)[px.p] = best_sel;
+  for (int i = 0; i < N; ++i) G(v->cost)[i * plane + px.p] = ws.RM(0, i);
+  flush_counters(v, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// checkerboard sweep
+
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restrict__ v, int color, float th_mc,
+                                                         uint32_t pass) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const int R = R_ >= 0 ? R_ : v->radius;
+  Pix px;
+  int tw;
+  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const int W = v->W, H = v->H, N = v->N;
+  const int plane = W * H;
+  const int p = px.p, row = px.row, col = px.col;
+  const float* iK = v->refInvK;
+  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Counters cnt;
+
+  const float vd[3] = {px.np0, px.np1, 1.0f};
+  const float inv_view_len = 1.0f / sqrtf(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+
+  // candidates
+  int nh = 0;
+  ws.hyp(0, 0) = G(v->normal)[p];
+  ws.hyp(0, 1) = G(v->normal)[plane + p];
+  ws.hyp(0, 2) = G(v->normal)[2 * plane + p];
+  ws.hyp(0, 3) = G(v->depth)[p];
+  nh = 1;
+  if (G(v->grad_ok)[p]) {
+    float nconf[4];
+    int order[4];
+    int nvalid = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      order[k] = k;
+      const int u = col + c_neighbor_offs[k][1], w = row + c_neighbor_offs[k][0];
+      if (u < 0 || u >= W || w < 0 || w >= H) {
+        nconf[k] = PM_FLT_MAX;
+        continue;
+      }
+      nvalid++;
+      nconf[k] = G(v->conf)[w * W + u];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < 3 - a; ++b) {
+        if (nconf[b] > nconf[b + 1]) {
+          const float t = nconf[b];
+          nconf[b] = nconf[b + 1];
+          nconf[b + 1] = t;
+          const int o = order[b];
+          order[b] = order[b + 1];
+          order[b + 1] = o;
+        }
+      }
+    }
+    if (nvalid > 2) nvalid = 2;
+    for (int k = 0; k < nvalid; ++k) {
+      const int idx = order[k];
+      const int u = col + c_neighbor_offs[idx][1], w = row + c_neighbor_offs[idx][0];
+      const int q = w * W + u;
+      float nn[3] = {G(v->normal)[q], G(v->normal)[plane + q], G(v->normal)[2 * plane + q]};
+      correct_normal(nn, vd, inv_view_len);
+      const float dd = propagate_depth(G(v->depth)[q], nn, iK, w, u, px.np0, px.np1);
+      ws.hyp(nh, 0) = nn[0];
+      ws.hyp(nh, 1) = nn[1];
+      ws.hyp(nh, 2) = nn[2];
+      ws.hyp(nh, 3) = dd;
+      nh++;
+    }
+  }
+  for (int gi = 0; gi < 8; ++gi) {
+    float best_conf = PM_FLT_MAX;
+    int br = 0, bc = 0;
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+      const int nr = row + c_dir_samples[gi][2 * f];
+      const int ncn = col + c_dir_samples[gi][2 * f + 1];
+      if (nr < 0 || nr >= H || ncn < 0 || ncn >= W) continue;
+      const float cv = G(v->conf)[nr * W + ncn];
+      if (best_conf > cv) {
+        best_conf = cv;
+        br = nr;
+        bc = ncn;
+      }
+    }
+    if (best_conf < PM_FLT_MAX) {
+      const int q = br * W + bc;
+      float nn[3] = {G(v->normal)[q], G(v->normal)[plane + q], G(v->normal)[2 * plane + q]};
+      correct_normal(nn, vd, inv_view_len);
+      const float dd = propagate_depth(G(v->depth)[q], nn, iK, br, bc, px.np0, px.np1);
+      ws.hyp(nh, 0) = nn[0];
+      ws.hyp(nh, 1) = nn[1];
+      ws.hyp(nh, 2) = nn[2];
+      ws.hyp(nh, 3) = dd;
+      nh++;
+    }
+  }
+
+  // cost matrix, source-major so consecutive evaluations share a source
+  for (int i = 0; i < N; ++i) {
+    ws.M(0, i) = G(v->cost)[i * plane + p];
+    for (int k = 1; k < kMaxHyp; ++k) {
+      if (k < nh) {
+        ws.M(k, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, ws.hyp(k, 0), ws.hyp(k, 1), ws.hyp(k, 2),
+                                                 ws.hyp(k, 3), v->refK, cnt);
+      }
+    }
+  }
+
+  // view selection (kernel_functions.cu:2364-2426)
+  uint32_t flags = 0;
+  for (int i = 0; i < N; ++i) {
+    float pr = 0.0f;
+    int n1 = 0, n2 = 0;
+    for (int k = 0; k < nh; ++k) {
+      const float cv = ws.M(k, i);
+      pr = pr + pm_expf(-(cv * cv) * v->inv_beta2);
+      n1 += (cv < th_mc);
+      n2 += (cv > v->max_ncc_matching_cost);
+    }
+    ws.Wt(i) = pr;
+    if (n1 >= v->num_good && n2 <= v->num_bad) flags |= 1u << i;
+  }
+  if (flags == 0) {
+    flush_counters(v, cnt);
+    return;
+  }
+  float psum = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    const float pr = ((flags >> i) & 1u) ? ws.Wt(i) : 0.0f;
+    psum = psum + pr;
+  }
+  const float inv_psum = 1.0f / psum;
+  float cum = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    const float pr = ((flags >> i) & 1u) ? ws.Wt(i) : 0.0f;
+    cum = cum + pr * inv_psum;
+    ws.Wt(i) = cum;
+    ws.RM(0, i) = 0.0f;
+  }
+  pm_rng g;
+  pm_rng_init(&g, v->seed, (uint32_t)v->view_id, (uint32_t)p, pass);
+  for (int smp = 0; smp < v->num_samples; ++smp) {
+    const float rp = pm_rng_uniform(&g) - PM_FLT_EPSILON;
+    for (int i = 0; i < N; ++i) {
+      if (ws.Wt(i) > rp) {
+        ws.RM(0, i) = v->random_optimization ? 1.0f : ws.RM(0, i) + 1.0f;
+        break;
+      }
+    }
+  }
+  float wnorm = 0.0f;
+  for (int i = 0; i < N; ++i) wnorm = wnorm + ws.RM(0, i);
+  if (wnorm <= 1e-5f) {
+    flush_counters(v, cnt);
+    return;
+  }
+  const float inv_wnorm = 1.0f / wnorm;
+  uint32_t sel = 0;
+  for (int i = 0; i < N; ++i) {
+    const float wv = ws.RM(0, i) * inv_wnorm;
+    ws.Wt(i) = wv;
+    if (wv > 0.0f) sel |= 1u << i;
+  }
+  int best = -1;
+  float minc = PM_FLT_MAX;
+  for (int k = 0; k < nh; ++k) {
+    float cc = 0.0f;
+    for (int i = 0; i < N; ++i) cc = cc + ws.Wt(i) * ws.M(k, i);
+    if (cc <= minc) {
+      minc = cc;
+      best = k;
+    }
+  }
+  G(v->view_sel)[p] = sel;
+  if (best < 0) best = 0;
+  float depth = ws.hyp(best, 3);
+  float normal[3] = {ws.hyp(best, 0), ws.hyp(best, 1), ws.hyp(best, 2)};
+
+  // refinement hypotheses (kernel_functions.cu:2449-2486)
+  const float dlo = (1.0f - v->random_depth_ratio) * depth;
+  const float dhi = (1.0f + v->random_depth_ratio) * depth;
+  const float nd = random_depth(&g, dlo, dhi);
+  float nn[3];
+  perturb_normal(&g, normal, v->angle1, v->angle2, nn);
+  correct_normal(nn, vd, inv_view_len);
+  const float rd = random_depth(&g, v->depth_min, v->depth_max_eff);
+  float rn[3];
+  random_normal(&g, vd[0], vd[1], rn);
+  correct_normal(rn, vd, inv_view_len);
+  const int hb = kMaxHyp;  // refinement hypotheses stored after the candidates
+  ws.hyp(hb + 0, 0) = normal[0];
+  ws.hyp(hb + 0, 1) = normal[1];
+  ws.hyp(hb + 0, 2) = normal[2];
+  ws.hyp(hb + 0, 3) = depth;
+  ws.hyp(hb + 1, 0) = nn[0];
+  ws.hyp(hb + 1, 1) = nn[1];
+  ws.hyp(hb + 1, 2) = nn[2];
+  ws.hyp(hb + 1, 3) = depth;
+  ws.hyp(hb + 2, 0) = normal[0];
+  ws.hyp(hb + 2, 1) = normal[1];
+  ws.hyp(hb + 2, 2) = normal[2];
+  ws.hyp(hb + 2, 3) = nd;
+  ws.hyp(hb + 3, 0) = rn[0];
+  ws.hyp(hb + 3, 1) = rn[1];
+  ws.hyp(hb + 3, 2) = rn[2];
+  ws.hyp(hb + 3, 3) = depth;
+  ws.hyp(hb + 4, 0) = normal[0];
+  ws.hyp(hb + 4, 1) = normal[1];
+  ws.hyp(hb + 4, 2) = normal[2];
+  ws.hyp(hb + 4, 3) = rd;
+
+  float rc0 = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    const float m = ws.M(best, i);
+    ws.RM(0, i) = m;
+    rc0 = rc0 + ws.Wt(i) * m;
+  }
+  for (int i = 0; i < N; ++i) {
+    for (int k = 1; k < kNumRefine; ++k) {
+      ws.RM(k, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, ws.hyp(hb + k, 0), ws.hyp(hb + k, 1),
+                                                ws.hyp(hb + k, 2), ws.hyp(hb + k, 3), v->refK, cnt);
+    }
+  }
+  int b2 = -1;
+  minc = PM_FLT_MAX;
+  for (int k = 0; k < kNumRefine; ++k) {
+    float cc = rc0;
+    if (k > 0) {
+      cc = 0.0f;
+      for (int i = 0; i < N; ++i) cc = cc + ws.Wt(i) * ws.RM(k, i);
+    }
+    if (cc <= minc) {
+      minc = cc;
+      b2 = k;
+    }
+  }
+  if (b2 < 0) {  // every refined cost NaN: the reference writes nothing
+    flush_counters(v, cnt);
+    return;
+  }
+  G(v->conf)[p] = minc;
+  G(v->depth)[p] = ws.hyp(hb + b2, 3);
+  G(v->normal)[p] = ws.hyp(hb + b2, 0);
+  G(v->normal)[plane + p] = ws.hyp(hb + b2, 1);
+  G(v->normal)[2 * plane + p] = ws.hyp(hb + b2, 2);
+  for (int i = 0; i < N; ++i) G(v->cost)[i * plane + p] = ws.RM(b2, i);
+  flush_counters(v, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// local optimisation
+
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) void k_local_opt(const ViewDev* __restrict__ v, int color, uint32_t pass) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const int R = R_ >= 0 ? R_ : v->radius;
+  Pix px;
+  int tw;
+  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const int W = v->W, H = v->H, N = v->N;
+  const int plane = W * H;
+  const int p = px.p, row = px.row, col = px.col;
+  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Counters cnt;
+
+  float conf = G(v->conf)[p];
+  const float depth = G(v->depth)[p];
+  const float normal[3] = {G(v->normal)[p], G(v->normal)[plane + p], G(v->normal)[2 * plane + p]};
+  const uint32_t sel = G(v->view_sel)[p];
+  const float angle1 = (float)(4.0 * 0.0174532925199432);
+  const float angle2 = (float)(2.0 * 0.0174532925199432);
+  float low = depth, up = depth;
+  float m0 = normal[0], m1 = normal[1], m2 = normal[2];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int u = col + c_neighbor_offs[k][1], w = row + c_neighbor_offs[k][0];
+    if (u < 0 || u >= W || w < 0 || w >= H) continue;
+    const int q = w * W + u;
+    const float ndp = G(v->depth)[q];
+    m0 = m0 + G(v->normal)[q];
+    m1 = m1 + G(v->normal)[plane + q];
+    m2 = m2 + G(v->normal)[2 * plane + q];
+    if (ndp <= low) low = ndp;
+    if (ndp > up) up = ndp;
+  }
+  float mn[3] = {m0, m1, m2};
+  const float inv_norm = 1.0f / sqrtf(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
+  mn[0] = mn[0] * inv_norm;
+  mn[1] = mn[1] * inv_norm;
+  mn[2] = mn[2] * inv_norm;
+  if (sel == 0) return;
+  pm_rng g;
+  pm_rng_init(&g, v->seed, (uint32_t)v->view_id, (uint32_t)p, pass);
+  for (int t = 0; t < 5; ++t) {
+    float hn[3];
+    const float hd = random_depth(&g, low, up);
+    perturb_normal(&g, mn, angle1, angle2, hn);
+    ws.hyp(t, 0) = hn[0];
+    ws.hyp(t, 1) = hn[1];
+    ws.hyp(t, 2) = hn[2];
+    ws.hyp(t, 3) = hd;
+  }
+  for (int i = 0; i < N; ++i) {
+    if (!((sel >> i) & 1u)) continue;
+    for (int t = 0; t < 5; ++t) {
+      ws.M(t, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, ws.hyp(t, 0), ws.hyp(t, 1), ws.hyp(t, 2),
+                                               ws.hyp(t, 3), v->refK, cnt);
+    }
+  }
+  int best = -1;
+  for (int t = 0; t < 5; ++t) {
+    float mconf = 0.0f, mw = 0.0f;
+    for (int i = 0; i < N; ++i) {
+      if (!((sel >> i) & 1u)) continue;
+      mconf = mconf + ws.M(t, i);
+      mw = mw + 1.0f;
+    }
+    mconf = mconf / mw;
+    if (mconf < conf) {
+      conf = mconf;
+      best = t;
+    }
+  }
+  if (best < 0) {
+    flush_counters(v, cnt);
+    return;
+  }
+  const float bd = ws.hyp(best, 3);
+  const float bn[3] = {ws.hyp(best, 0), ws.hyp(best, 1), ws.hyp(best, 2)};
+  G(v->conf)[p] = conf;
+  G(v->depth)[p] = bd;
+  G(v->normal)[p] = bn[0];
+  G(v->normal)[plane + p] = bn[1];
+  G(v->normal)[2 * plane + p] = bn[2];
+  const float* Kgeo = v->compat_lo_geo_bug ? v->refInvK : v->refK;
+  for (int i = 0; i < N; ++i) {
+    float cv;
+    if ((sel >> i) & 1u) {
+      cv = ws.M(best, i);
+    } else {
+      cv = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, bn[0], bn[1], bn[2], bd, Kgeo, cnt);
+    }
+    G(v->cost)[i * plane + p] = cv;
+  }
+  flush_counters(v, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// FilterDepthMap (kernel_functions.cu:3282-3354)
+
+__global__ __launch_bounds__(kSweepThreads) void k_filter(const ViewDev* __restrict__ v, int color) {
+  const int t = threadIdx.x;
+  const int row = blockIdx.y * kTileH + (t >> 4);
+  const int col = blockIdx.x * kTileW + 2 * (t & 15) + ((row + color) & 1);
+  const int W = v->W, H = v->H;
+  if (row >= H || col >= W) return;
+  const int plane = W * H;
+  const int p = row * W + col;
+  const float* iK = v->refInvK;
+  const float depth = G(v->depth)[p];
+  if (depth <= 0.0f) return;
+  const float n[3] = {G(v->normal)[p], G(v->normal)[plane + p], G(v->normal)[2 * plane + p]};
+  const float conf = G(v->conf)[p];
+  const float pt[3] = {depth * (iK[0] * (float)col + iK[1]), depth * (iK[2] * (float)row + iK[3]), depth};
+  int good = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int u = col + c_neighbor_offs[k][1], w = row + c_neighbor_offs[k][0];
+    if (u < 0 || u >= W || w < 0 || w >= H) continue;
+    const int q = w * W + u;
+    const float nd = G(v->depth)[q];
+    const float nn[3] = {G(v->normal)[q], G(v->normal)[plane + q], G(v->normal)[2 * plane + q]};
+    const float nc = G(v->conf)[q];
+    const float npt[3] = {nd * (iK[0] * (float)u + iK[1]), nd * (iK[2] * (float)w + iK[3]), nd};
+    const float e0 = pt[0] - npt[0], e1 = pt[1] - npt[1], e2 = pt[2] - npt[2];
+    const float d1 = fabsf(e0 * n[0] + e1 * n[1] + e2 * n[2]);
+    const float d2 = fabsf(e0 * nn[0] + e1 * nn[1] + e2 * nn[2]);
+    if ((conf < v->conf_threshold && d1 / depth < v->depth_diff_threshold) ||
+        (nc < v->conf_threshold && d2 / nd < v->depth_diff_threshold)) {
+      good++;
+    }
+  }
+  if (good < 2) {
+    G(v->depth)[p] = 0.0f;
+    G(v->normal)[p] = 0.0f;
+    G(v->normal)[plane + p] = 0.0f;
+    G(v->normal)[2 * plane + p] = 0.0f;
+    G(v->conf)[p] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// preprocessing
+
+__global__ void k_quads_u8(const uint8_t* __restrict__ img, int W, int H, int stride, uint32_t* __restrict__ out) {
+  const int pw = W + 2 * kQuadPad, ph = H + 2 * kQuadPad;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= pw || py >= ph) return;
+  const int x = px - kQuadPad, y = py - kQuadPad;
+  uint32_t q = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int xx = x + (k & 1), yy = y + (k >> 1);
+    const uint32_t t = (xx < 0 || xx >= W || yy < 0 || yy >= H) ? 0u : (uint32_t)img[yy * stride + xx];
+    q |= t << (8 * k);
+  }
+  out[py * pw + px] = q;
+}
+
+__global__ void k_quads_f16(const uint8_t* __restrict__ img, int W, int H, int stride, uint2* __restrict__ out) {
+  const int pw = W + 2 * kQuadPad, ph = H + 2 * kQuadPad;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= pw || py >= ph) return;
+  const int x = px - kQuadPad, y = py - kQuadPad;
+  unsigned short h[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int xx = x + (k & 1), yy = y + (k >> 1);
+    const float t = (xx < 0 || xx >= W || yy < 0 || yy >= H) ? 0.0f : (float)img[yy * stride + xx];
+    h[k] = __half_as_ushort(__float2half(t));
+  }
+  out[py * pw + px] = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+}
+
+__global__ void k_depth_quads(const float* __restrict__ d, int W, int H, float4* __restrict__ out) {
+  const int pw = W + 2 * kQuadPad, ph = H + 2 * kQuadPad;
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= pw || py >= ph) return;
+  const int x = px - kQuadPad, y = py - kQuadPad;
+  float t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int xx = x + (k & 1), yy = y + (k >> 1);
+    t[k] = (xx < 0 || xx >= W || yy < 0 || yy >= H) ? 0.0f : d[yy * W + xx];
+  }
+  out[py * pw + px] = make_float4(t[0], t[1], t[2], t[3]);
+}
+
+// FilterKernel + ComputeGradientMap
+__global__ void k_ref_moments(const ViewDev* __restrict__ v, int accum_mode, float4* __restrict__ refm,
+                              uint8_t* __restrict__ grad_ok) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y * blockDim.y + threadIdx.y;
+  const int W = v->W, H = v->H;
+  if (col >= W || row >= H) return;
+  const int R = v->radius, S = v->step;
+  const uint8_t* ref = v->ref;
+  auto px_at = [&](int r, int c) -> int { return (r < 0 || r >= H || c < 0 || c >= W) ? 0 : (int)ref[r * W + c]; };
+  const int c = px_at(row, col);
+  float sw = 0.0f, sr = 0.0f, srr = 0.0f;
+  int k = 0;
+  for (int dr = -R; dr <= R; dr += S) {
+    for (int dc = -R; dc <= R; dc += S) {
+      const int ri = px_at(row + dr, col + dc);
+      const float r = (float)ri;
+      const float w = G(v->sp)[k++] * G(v->ct)[abs(c - ri)];
+      const float wr = w * r;
+      sw = sw + w;
+      sr = sr + wr;
+      srr = fmaf(wr, r, srr);
+    }
+  }
+  const float mu = sr / sw;
+  const float m2 = srr / sw;
+  float var = m2 - mu * mu;
+  float err = 0.0f;
+  if (accum_mode == PM_ACCUM_COMPENSATED) {
+    float s1 = 0.0f, s2 = 0.0f;
+    k = 0;
+    for (int dr = -R; dr <= R; dr += S) {
+      for (int dc = -R; dc <= R; dc += S) {
+        const int ri = px_at(row + dr, col + dc);
+        const float d = (float)ri - mu;
+        const float w = G(v->sp)[k++] * G(v->ct)[abs(c - ri)];
+        const float wd = w * d;
+        s1 = s1 + wd;
+        s2 = fmaf(wd, d, s2);
+      }
+    }
+    err = s1 / sw;
+    var = s2 / sw - err * err;
+  }
+  refm[row * W + col] = make_float4(mu, var, 1.0f / sw, err);
+  const float kx[3][3] = {{-3, 0, 3}, {-10, 0, 10}, {-3, 0, 3}};
+  const float ky[3][3] = {{-3, -10, -3}, {0, 0, 0}, {3, 10, 3}};
+  float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+  for (int dr = -1; dr <= 1; ++dr) {
+#pragma unroll
+    for (int dc = -1; dc <= 1; ++dc) {
+      const float color = (float)px_at(row + dr, col + dc) / 255.0f;
+      gx = gx + color * kx[dr + 1][dc + 1];
+      gy = gy + color * ky[dr + 1][dc + 1];
+    }
+  }
+  const float grad = sqrtf(gx * gx + gy * gy);
+  grad_ok[row * W + col] = grad >= 1.0f ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over the compiled (radius, step) shapes
+
+template <int R, int S, int ST, int RC>
+struct SweepK {
+  static void* fn() { return (void*)&k_sweep<R, S, ST, RC>; }
+};
+template <int R, int S, int ST, int RC>
+struct InitK {
+  static void* fn() { return (void*)&k_init<R, S, ST, RC>; }
+};
+template <int R, int S, int ST, int RC>
+struct LocalK {
+  static void* fn() { return (void*)&k_local_opt<R, S, ST, RC>; }
+};
+
+template <template <int, int, int, int> class Kn, int R, int S>
+void* pick_sr(int storage, int rcp) {
+  if (storage == PM_STORAGE_U8) {
+    return rcp == PM_RCP_EXACT ? Kn<R, S, PM_STORAGE_U8, PM_RCP_EXACT>::fn() : Kn<R, S, PM_STORAGE_U8, PM_RCP_FAST>::fn();
+  }
+  return rcp == PM_RCP_EXACT ? Kn<R, S, PM_STORAGE_F16, PM_RCP_EXACT>::fn() : Kn<R, S, PM_STORAGE_F16, PM_RCP_FAST>::fn();
+}
+
+template <template <int, int, int, int> class Kn>
+void* pick_kernel(const LaunchCfg& cfg) {
+  const int r = cfg.radius, st = cfg.step;
+  if (r == 5 && st == 2) return pick_sr<Kn, 5, 2>(cfg.storage, cfg.rcp);
+  if (r == 7 && st == 1) return pick_sr<Kn, 7, 1>(cfg.storage, cfg.rcp);
+  if (r == 5 && st == 1) return pick_sr<Kn, 5, 1>(cfg.storage, cfg.rcp);
+  if (r == 7 && st == 2) return pick_sr<Kn, 7, 2>(cfg.storage, cfg.rcp);
+  return pick_sr<Kn, -1, -1>(cfg.storage, cfg.rcp);
+}
+
+dim3 sweep_grid(int W, int H) { return dim3((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, 1); }
+
+}  // namespace
+
+int sweep_slots(int W, int H) {
+  const dim3 g = sweep_grid(W, H);
+  return (int)(g.x * g.y) * kSweepThreads;
+}
+
+hipError_t launch_build_quads_u8(const uint8_t* img, int W, int H, int stride, uint32_t* out, hipStream_t s) {
+  if (W <= 0 || H <= 0 || stride < W) return hipErrorInvalidValue;
+  const dim3 blk(32, 8), grd((W + 2 * kQuadPad + 31) / 32, (H + 2 * kQuadPad + 7) / 8);
+  hipLaunchKernelGGL(k_quads_u8, grd, blk, 0, s, img, W, H, stride, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_quads_f16(const uint8_t* img, int W, int H, int stride, uint2* out, hipStream_t s) {
+  if (W <= 0 || H <= 0 || stride < W) return hipErrorInvalidValue;
+  const dim3 blk(32, 8), grd((W + 2 * kQuadPad + 31) / 32, (H + 2 * kQuadPad + 7) / 8);
+  hipLaunchKernelGGL(k_quads_f16, grd, blk, 0, s, img, W, H, stride, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_depth_quads(const float* depth, int W, int H, float4* out, hipStream_t s) {
+  if (W <= 0 || H <= 0) return hipErrorInvalidValue;
+  const dim3 blk(32, 8), grd((W + 2 * kQuadPad + 31) / 32, (H + 2 * kQuadPad + 7) / 8);
+  hipLaunchKernelGGL(k_depth_quads, grd, blk, 0, s, depth, W, H, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, float4* refm, uint8_t* grad_ok,
+                              hipStream_t s) {
+  if (W <= 0 || H <= 0) return hipErrorInvalidValue;
+  const dim3 blk(32, 8), grd((W + 31) / 32, (H + 7) / 8);
+  hipLaunchKernelGGL(k_ref_moments, grd, blk, 0, s, v, accum_mode, refm, grad_ok);
+  return hipGetLastError();
+}
+
+hipError_t launch_init(const ViewDev* v, const LaunchCfg& cfg, int W, int H, uint32_t pass, hipStream_t s) {
+  if (W <= 0 || H <= 0) return hipErrorInvalidValue;
+  void* fn = pick_kernel<InitK>(cfg);
+  for (int color = 0; color < 2; ++color) {
+    void* args[] = {(void*)&v, (void*)&color, (void*)&pass};
+    hipError_t e = hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, float th_mc, uint32_t pass,
+                        hipStream_t s) {
+  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
+  void* fn = pick_kernel<SweepK>(cfg);
+  void* args[] = {(void*)&v, (void*)&color, (void*)&th_mc, (void*)&pass};
+  return hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
+}
+
+hipError_t launch_local_opt(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, uint32_t pass,
+                            hipStream_t s) {
+  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
+  void* fn = pick_kernel<LocalK>(cfg);
+  void* args[] = {(void*)&v, (void*)&color, (void*)&pass};
+  return hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
+}
+
+hipError_t launch_filter(const ViewDev* v, int W, int H, int color, hipStream_t s) {
+  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_filter, sweep_grid(W, H), dim3(kSweepThreads), 0, s, v, color);
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -1,0 +1,268 @@
+"""Python mirror of SenseMap's PatchMatch engine API over the C-ABI.
+
+Names follow the reference (src/controllers/patch_match_options.h,
+src/mvs/patch_match_cuda.h): ``PatchMatchOptions``, ``Problem``, ``Image``
+and ``PatchMatchACMMCuda`` with ``Run()``, ``GetDepthMap()``,
+``GetNormalMap()``, ``GetConfMap()``.  The engine itself is the gfx950 HIP
+library; this module only marshals numpy arrays into the C structs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, fields
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class PatchMatchOptions:
+    """Engine fields of PatchMatchOptions (patch_match_options.h:42-304), same defaults."""
+    window_radius: int = 5
+    window_step: int = 2
+    sigma_spatial: float = 5.0
+    sigma_color: float = 0.2
+    num_samples: int = 15
+    num_iterations: int = 5
+    geom_consistency: bool = True
+    geom_consistency_regularizer: float = 0.3
+    geom_consistency_max_cost: float = 3.0
+    random_optimization: bool = True
+    local_optimization: bool = True
+    random_depth_ratio: float = 0.004
+    random_angle1_range: float = math.radians(20.0)
+    random_angle2_range: float = math.radians(12.0)
+    init_ncc_matching_cost: float = 0.8
+    max_ncc_matching_cost: float = 1.2
+    alpha: float = 90.0
+    beta: float = 0.3
+    thk: int = 4
+    num_good_hypothesis: int = 2
+    num_bad_hypothesis: int = 3
+    init_depth_random: Optional[bool] = None  # None = !geom_consistency, as the reference
+    filter: bool = True
+    conf_filter: bool = False
+    conf_threshold: float = 0.5
+    depth_diff_threshold: float = 0.02
+    # extensions
+    init_samples: int = 1
+    compat_lo_geo_bug: bool = True
+    image_storage: int = abi.PM_STORAGE_U8
+    accum_mode: int = abi.PM_ACCUM_PLAIN
+    rcp_mode: int = abi.PM_RCP_EXACT
+    seed: int = 0
+
+    def to_c(self) -> abi.pm_options:
+        o = abi.pm_options()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "init_depth_random" and v is None:
+                v = not self.geom_consistency
+            setattr(o, f.name, int(v) if isinstance(v, bool) else v)
+        return o
+
+    def Check(self) -> bool:
+        st = abi.load_engine().pm_options_check(C.byref(self.to_c()))
+        return st == abi.PM_OK
+
+
+@dataclass
+class Image:
+    """mvs::Image: u8 grey bitmap, K = (fx, cx, fy, cy), world->camera R, T."""
+    bitmap: np.ndarray
+    K: Sequence[float]
+    R: np.ndarray
+    T: np.ndarray
+
+    def GetWidth(self) -> int:
+        return int(self.bitmap.shape[1])
+
+    def GetHeight(self) -> int:
+        return int(self.bitmap.shape[0])
+
+
+@dataclass
+class Problem:
+    """Problem (patch_match_options.h:306-353) with the controller-owned
+    vectors given directly."""
+    ref_image_idx: int
+    src_image_idxs: List[int]
+    images: List[Image]
+    depth_min: float
+    depth_max: float
+    depth_maps: Optional[list] = None    # per image index, float32 (H, W) (geometric pass)
+    normal_maps: Optional[list] = None   # per image index, float32 (3, H, W)
+
+
+def _camera(img: Image) -> abi.pm_camera:
+    c = abi.pm_camera()
+    for i in range(4):
+        c.K[i] = float(img.K[i])
+    R = np.asarray(img.R, dtype=np.float64).reshape(9)
+    T = np.asarray(img.T, dtype=np.float64).reshape(3)
+    for i in range(9):
+        c.R[i] = float(R[i])
+    for i in range(3):
+        c.T[i] = float(T[i])
+    return c
+
+
+class ViewDesc:
+    """Owns the numpy buffers behind one pm_view_desc."""
+
+    def __init__(self, problem: Problem, options: PatchMatchOptions):
+        imgs = problem.images
+        ref = imgs[problem.ref_image_idx]
+        self._keep = []
+        self.ref_bitmap = np.ascontiguousarray(ref.bitmap, dtype=np.uint8)
+        self._keep.append(self.ref_bitmap)
+        d = abi.pm_view_desc()
+        d.view_id = int(problem.ref_image_idx)
+        d.num_src = len(problem.src_image_idxs)
+        d.ref = self._image(self.ref_bitmap)
+        d.ref_cam = _camera(ref)
+        n = len(problem.src_image_idxs)
+        srcs = (abi.pm_image * max(n, 1))()
+        cams = (abi.pm_camera * max(n, 1))()
+        for k, idx in enumerate(problem.src_image_idxs):
+            b = np.ascontiguousarray(imgs[idx].bitmap, dtype=np.uint8)
+            self._keep.append(b)
+            srcs[k] = self._image(b)
+            cams[k] = _camera(imgs[idx])
+        self._keep += [srcs, cams]
+        d.src = C.cast(srcs, C.POINTER(abi.pm_image))
+        d.src_cams = C.cast(cams, C.POINTER(abi.pm_camera))
+        d.depth_min = float(problem.depth_min)
+        d.depth_max = float(problem.depth_max)
+        if options.geom_consistency:
+            ptrs = (C.POINTER(C.c_float) * max(n, 1))()
+            for k, idx in enumerate(problem.src_image_idxs):
+                dm = np.ascontiguousarray(problem.depth_maps[idx], dtype=np.float32)
+                self._keep.append(dm)
+                ptrs[k] = dm.ctypes.data_as(C.POINTER(C.c_float))
+            self._keep.append(ptrs)
+            d.src_depth_maps = C.cast(ptrs, C.POINTER(C.POINTER(C.c_float)))
+        init_random = options.init_depth_random
+        if init_random is None:
+            init_random = not options.geom_consistency
+        if not init_random:
+            dm = np.ascontiguousarray(problem.depth_maps[problem.ref_image_idx], dtype=np.float32)
+            nm = np.ascontiguousarray(problem.normal_maps[problem.ref_image_idx], dtype=np.float32)
+            self._keep += [dm, nm]
+            d.init_depth = dm.ctypes.data_as(C.POINTER(C.c_float))
+            d.init_normal = nm.ctypes.data_as(C.POINTER(C.c_float))
+        self.desc = d
+        self.width = ref.GetWidth()
+        self.height = ref.GetHeight()
+        self.num_src = n
+
+    @staticmethod
+    def _image(b: np.ndarray) -> abi.pm_image:
+        im = abi.pm_image()
+        im.data = b.ctypes.data_as(C.POINTER(C.c_uint8))
+        im.height, im.width = int(b.shape[0]), int(b.shape[1])
+        im.stride = int(b.strides[0])
+        return im
+
+
+class Engine:
+    """A device and its HIP stream (pm_engine)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load_engine()
+        h = C.c_void_p()
+        abi.check(self.lib.pm_engine_create(int(device), C.byref(h)))
+        self.handle = h
+
+    def set_counting(self, enable: bool) -> None:
+        abi.check(self.lib.pm_engine_set_counting(self.handle, int(enable)))
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.pm_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_engine: Optional[Engine] = None
+
+
+def default_engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+class PatchMatchACMMCuda:
+    """Same surface as SenseMap's PatchMatchACMMCuda (patch_match_cuda.h:154-203):
+    the constructor uploads the problem, Run() runs the iteration schedule,
+    Get*Map() read the results back."""
+
+    def __init__(self, options: PatchMatchOptions, problem: Problem, engine: Optional[Engine] = None):
+        self.engine = engine or default_engine()
+        self.lib = self.engine.lib
+        self.options = options
+        self.view = ViewDesc(problem, options)
+        self._copt = options.to_c()
+        h = C.c_void_p()
+        abi.check(self.lib.pm_problem_create(self.engine.handle, C.byref(self._copt), C.byref(self.view.desc),
+                                             C.byref(h)))
+        self.handle = h
+        self.W, self.H, self.N = self.view.width, self.view.height, self.view.num_src
+
+    def Run(self) -> bool:
+        abi.check(self.lib.pm_problem_run(self.handle))
+        return True
+
+    def _get(self, fn, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        ctype = C.c_float if dtype == np.float32 else C.c_uint32
+        abi.check(fn(self.handle, out.ctypes.data_as(C.POINTER(ctype))))
+        return out
+
+    def GetDepthMap(self) -> np.ndarray:
+        return self._get(self.lib.pm_problem_get_depth_map, (self.H, self.W), np.float32)
+
+    def GetNormalMap(self) -> np.ndarray:
+        return self._get(self.lib.pm_problem_get_normal_map, (3, self.H, self.W), np.float32)
+
+    def GetConfMap(self) -> np.ndarray:
+        return self._get(self.lib.pm_problem_get_conf_map, (self.H, self.W), np.float32)
+
+    def GetCostMap(self) -> np.ndarray:
+        return self._get(self.lib.pm_problem_get_cost_map, (self.N, self.H, self.W), np.float32)
+
+    def GetViewSelMap(self) -> np.ndarray:
+        return self._get(self.lib.pm_problem_get_view_sel, (self.H, self.W), np.uint32)
+
+    def GetStats(self) -> abi.pm_run_stats:
+        s = abi.pm_run_stats()
+        abi.check(self.lib.pm_problem_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def GetPoses(self):
+        poses = np.zeros((self.N, abi.PM_POSE_STRIDE), dtype=np.float32)
+        k8 = np.zeros(8, dtype=np.float32)
+        abi.check(self.lib.pm_problem_get_poses(self.handle, poses.ctypes.data_as(C.POINTER(C.c_float)),
+                                                k8.ctypes.data_as(C.POINTER(C.c_float))))
+        return poses, k8
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.pm_problem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,264 @@
+"""Seeded synthetic multi-view scenes with exact ground-truth depth and normals.
+
+Scenes are triangle soups rendered by libpm_scene.so (z-buffer, FP64 ray /
+triangle intersection, 3D value-noise texture in world space).  Builders:
+
+  planes_scene       3 seeded planes (a background plane and two bounded
+                     quads), 1 reference + ring of sources
+  heightfield_scene  64x64 heightfield mesh with Gaussian bumps seen by a
+                     pitched camera, depth ~3..30, 1 reference + ring of sources
+  orbit_cameras      cameras on a circle around a scene centre
+
+Cameras are pinhole, world->camera R, T, with K = (fx, cx, fy, cy); pixel i
+sits at coordinate i.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+class _ScnParams(C.Structure):
+    _fields_ = [
+        ("K", C.c_double * 4), ("R", C.c_double * 9), ("T", C.c_double * 3),
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("tex_spacing", C.c_double), ("tex_amplitude", C.c_double), ("tex_offset", C.c_double),
+        ("tex_seed", C.c_uint64), ("noise_sigma", C.c_double), ("noise_seed", C.c_uint64),
+    ]
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "lib", "libpm_scene.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        lib.scn_render.restype = C.c_int
+        lib.scn_render.argtypes = [C.POINTER(C.c_double), C.c_int32, C.POINTER(_ScnParams),
+                                   C.POINTER(C.c_uint8), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_int32)]
+        _lib = lib
+    return _lib
+
+
+@dataclass
+class Camera:
+    K: Tuple[float, float, float, float]  # fx, cx, fy, cy
+    R: np.ndarray                         # 3x3 world -> camera
+    T: np.ndarray                         # 3
+    width: int
+    height: int
+
+    @property
+    def center(self) -> np.ndarray:
+        return -self.R.T @ self.T
+
+
+@dataclass
+class RenderedView:
+    camera: Camera
+    image: np.ndarray   # (H, W) uint8
+    depth: np.ndarray   # (H, W) float32, 0 = no surface
+    normal: np.ndarray  # (3, H, W) float32, camera frame
+
+
+@dataclass
+class Texture:
+    spacing: float
+    amplitude: float = 40.0
+    offset: float = 128.0
+    seed: int = 1
+    noise_sigma: float = 0.0
+    noise_seed: int = 0
+
+
+def look_at(center, target, down=(0.0, 1.0, 0.0)) -> Tuple[np.ndarray, np.ndarray]:
+    """World->camera (R, T) for a camera at `center` looking at `target`
+    (x right, y down, z forward)."""
+    c = np.asarray(center, dtype=np.float64)
+    f = np.asarray(target, dtype=np.float64) - c
+    f /= np.linalg.norm(f)
+    d = np.asarray(down, dtype=np.float64)
+    x = np.cross(d, f)
+    x /= np.linalg.norm(x)
+    y = np.cross(f, x)
+    R = np.stack([x, y, f])
+    return R, -R @ c
+
+
+def render(tris: np.ndarray, cam: Camera, tex: Texture) -> RenderedView:
+    lib = _load()
+    tris = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+    p = _ScnParams()
+    for i in range(4):
+        p.K[i] = cam.K[i]
+    for i in range(9):
+        p.R[i] = float(cam.R.reshape(9)[i])
+    for i in range(3):
+        p.T[i] = float(cam.T[i])
+    p.width, p.height = cam.width, cam.height
+    p.tex_spacing, p.tex_amplitude, p.tex_offset = tex.spacing, tex.amplitude, tex.offset
+    p.tex_seed, p.noise_sigma, p.noise_seed = tex.seed, tex.noise_sigma, tex.noise_seed
+    img = np.zeros((cam.height, cam.width), dtype=np.uint8)
+    depth = np.zeros((cam.height, cam.width), dtype=np.float32)
+    normal = np.zeros((3, cam.height, cam.width), dtype=np.float32)
+    lib.scn_render(tris.ctypes.data_as(C.POINTER(C.c_double)), int(tris.shape[0]), C.byref(p),
+                   img.ctypes.data_as(C.POINTER(C.c_uint8)), depth.ctypes.data_as(C.POINTER(C.c_float)),
+                   normal.ctypes.data_as(C.POINTER(C.c_float)), None)
+    return RenderedView(cam, img, depth, normal)
+
+
+def _quad(center, normal, half, up_hint=(0.0, 1.0, 0.0)) -> np.ndarray:
+    n = np.asarray(normal, dtype=np.float64)
+    n /= np.linalg.norm(n)
+    a = np.cross(np.asarray(up_hint, dtype=np.float64), n)
+    a /= np.linalg.norm(a)
+    b = np.cross(n, a)
+    c = np.asarray(center, dtype=np.float64)
+    p = [c + half * (sa * a + sb * b) for sa, sb in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+    return np.array([np.concatenate([p[0], p[1], p[2]]), np.concatenate([p[0], p[2], p[3]])])
+
+
+def _tilted_normal(rng: np.random.Generator, max_deg: float) -> np.ndarray:
+    """Unit normal facing the camera (-z) within max_deg of the optical axis."""
+    t = math.radians(max_deg) * math.sqrt(rng.uniform())
+    ph = rng.uniform(0, 2 * math.pi)
+    return np.array([math.sin(t) * math.cos(ph), math.sin(t) * math.sin(ph), -math.cos(t)])
+
+
+def ring_cameras(ref: Camera, n: int, radius: float, target, phase: float = 0.0, f=None,
+                 size=None) -> List[Camera]:
+    """n cameras on a circle of `radius` in the reference camera's x-y plane,
+    all looking at `target` (world coordinates)."""
+    cams = []
+    C0 = ref.center
+    ax = ref.R[0]
+    ay = ref.R[1]
+    down = ref.R[1]
+    for k in range(n):
+        a = phase + 2 * math.pi * k / n
+        c = C0 + radius * (math.cos(a) * ax + math.sin(a) * ay)
+        R, T = look_at(c, target, down)
+        w, h = size or (ref.width, ref.height)
+        K = ref.K if f is None else (f, (w - 1) / 2.0, f, (h - 1) / 2.0)
+        cams.append(Camera(tuple(K), R, T, w, h))
+    return cams
+
+
+@dataclass
+class Scene:
+    ref: RenderedView
+    srcs: List[RenderedView]
+    depth_min: float
+    depth_max: float
+    tris: np.ndarray
+    texture: Texture
+
+
+def _depth_range(depth: np.ndarray) -> Tuple[float, float]:
+    v = depth[depth > 0]
+    return float(v.min()) * 0.9, float(v.max()) * 1.1
+
+
+def planes_scene(seed: int = 1, width: int = 640, height: int = 480, f: float = 500.0, num_src: int = 4,
+                 tex_amplitude: float = 40.0, noise_sigma: float = 0.0) -> Scene:
+    """3 seeded planes at depths U[2, 10]: a background plane and two bounded
+    quads, normals within 35 deg of the optical axis; sources on a ring of
+    radius 0.1 x median depth looking at the median-depth point."""
+    rng = np.random.default_rng(seed)
+    depths = np.sort(rng.uniform(2.0, 10.0, size=3))[::-1]
+    cx, cy = (width - 1) / 2.0, (height - 1) / 2.0
+    tris = []
+    # background plane through the optical axis at the largest depth
+    tris.append(_quad([0, 0, depths[0]], _tilted_normal(rng, 35.0), 20.0 * depths[0]))
+    for d in depths[1:]:
+        u, v = rng.uniform(0.15, 0.85) * width, rng.uniform(0.15, 0.85) * height
+        centre = np.array([(u - cx) / f * d, (v - cy) / f * d, d])
+        half = d * rng.uniform(0.12, 0.22) * width / f
+        tris.append(_quad(centre, _tilted_normal(rng, 35.0), half))
+    tris = np.concatenate(tris)
+    ref_cam = Camera((f, cx, f, cy), np.eye(3), np.zeros(3), width, height)
+    dmed = float(np.median(depths))
+    tex = Texture(spacing=8.0 * dmed / f, amplitude=tex_amplitude, seed=seed * 7919 + 1,
+                  noise_sigma=noise_sigma, noise_seed=seed * 104729 + 3)
+    ref = render(tris, ref_cam, tex)
+    gt = ref.depth[ref.depth > 0]
+    med = float(np.median(gt))
+    srcs_c = ring_cameras(ref_cam, num_src, 0.1 * med, [0.0, 0.0, med], phase=rng.uniform(0, 2 * math.pi))
+    srcs = [render(tris, c, Texture(tex.spacing, tex.amplitude, tex.offset, tex.seed, noise_sigma,
+                                    tex.noise_seed + 1000 * (k + 1)))
+            for k, c in enumerate(srcs_c)]
+    dmin, dmax = _depth_range(ref.depth)
+    return Scene(ref, srcs, dmin, dmax, tris, tex)
+
+
+def heightfield_tris(rng: np.random.Generator, x_range, z_range, ground_y: float, grid: int = 64,
+                     num_bumps: int = 40, sigma: float = 0.5, amp: float = 0.6) -> np.ndarray:
+    """Heightfield mesh Y = ground_y - sum of Gaussian bumps over (X, Z)
+    (Y points down), grid x grid vertices."""
+    xs = np.linspace(x_range[0], x_range[1], grid)
+    zs = np.linspace(z_range[0], z_range[1], grid)
+    X, Z = np.meshgrid(xs, zs)
+    bx = rng.uniform(x_range[0], x_range[1], num_bumps)
+    bz = rng.uniform(z_range[0], z_range[1], num_bumps)
+    ba = rng.uniform(-amp, amp, num_bumps)
+    Hh = np.zeros_like(X)
+    for a, x0, z0 in zip(ba, bx, bz):
+        Hh += a * np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (2 * sigma * sigma))
+    Y = ground_y - Hh
+    V = np.stack([X, Y, Z], axis=-1)
+    tris = []
+    for i in range(grid - 1):
+        for j in range(grid - 1):
+            a, b, c, d = V[i, j], V[i, j + 1], V[i + 1, j + 1], V[i + 1, j]
+            tris.append(np.concatenate([a, b, c]))
+            tris.append(np.concatenate([a, c, d]))
+    return np.array(tris)
+
+
+def heightfield_scene(seed: int = 2, width: int = 1920, height: int = 1080, f: float = 1500.0,
+                      num_src: int = 10, tex_amplitude: float = 40.0, grid: int = 64) -> Scene:
+    """Heightfield (grid x grid, Gaussian bumps sigma 0.5 m) seen by a camera
+    pitched so that the depth spans about 3..30; sources on a ring of radius
+    0.1 x median depth looking at the median-depth point."""
+    rng = np.random.default_rng(seed)
+    cy = (height - 1) / 2.0
+    cx = (width - 1) / 2.0
+    t = cy / f
+    phi = math.atan(11.0 * t / 9.0)            # bottom row depth 3, top row depth 30
+    h_cam = 3.0 * (math.sin(phi) + t * math.cos(phi))
+    tris = heightfield_tris(rng, (-26.0, 26.0), (0.5, 40.0), h_cam, grid=grid)
+    # camera at the origin, pitched down by phi about the x axis
+    cp, sp = math.cos(phi), math.sin(phi)
+    R = np.array([[1.0, 0.0, 0.0], [0.0, cp, -sp], [0.0, sp, cp]])
+    ref_cam = Camera((f, cx, f, cy), R, np.zeros(3), width, height)
+    tex = Texture(spacing=8.0 * 10.0 / f, amplitude=tex_amplitude, seed=seed * 7919 + 1)
+    ref = render(tris, ref_cam, tex)
+    med = float(np.median(ref.depth[ref.depth > 0]))
+    target = ref_cam.R.T @ np.array([0.0, 0.0, med])
+    srcs_c = ring_cameras(ref_cam, num_src, 0.1 * med, target, phase=rng.uniform(0, 2 * math.pi))
+    srcs = [render(tris, c, tex) for c in srcs_c]
+    dmin, dmax = _depth_range(ref.depth)
+    return Scene(ref, srcs, dmin, dmax, tris, tex)
+
+
+def to_problem(scene: Scene, with_gt_maps: bool = False):
+    """Problem with image 0 = reference and images 1..N = sources."""
+    from .patch_match import Image, Problem
+    views = [scene.ref] + scene.srcs
+    images = [Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
+    prob = Problem(ref_image_idx=0, src_image_idxs=list(range(1, len(views))), images=images,
+                   depth_min=scene.depth_min, depth_max=scene.depth_max)
+    if with_gt_maps:
+        prob.depth_maps = [v.depth for v in views]
+        prob.normal_maps = [v.normal for v in views]
+    return prob
