@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 PatchMatch engine on one MI355X.
+
+Workload (one step): a 1920x1080 reference view with 10 source views of a
+seeded heightfield scene (64x64 mesh, Gaussian bumps, depth ~3..30), window
+15x15 step 1: the photometric pass (8 iterations, random init) followed by
+the geometric-consistency pass (2 iterations) whose initial state is the
+photometric result and whose source depth maps are the sources' own
+photometric results (computed once before timing; everything is resident in
+HBM when the timed region starts, and the photometric -> geometric hand-off
+stays on the device).
+
+Metric: Mpixel.iter/s = W x H x (8 + 2) per step / time; also views/s, the
+median relative depth error against the rendered ground truth, and p50/p99
+step latency.
+
+roofline: the checkerboard sweep kernel, algorithmic FP32 FLOPs per launch
+(23 per bilateral-NCC window sample + 75 per (hypothesis, source) evaluation
++ 61 per geometric term, counted in a calibration run) over its average
+launch duration measured with hipEvents on the engine's stream during the
+timed steps.  cpu_baseline: the C/OpenMP transcription on a band of rows of
+the same view, photometric pass.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
+FLOP_PER_SAMPLE = 23
+FLOP_PER_EVAL = 75
+FLOP_PER_GEO = 61
+METRIC = "Mpixel·iter/s + views/s at fixed depth error; p50/p99 view latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rcp", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    return ap.parse_args()
+
+
+def c2_options(pmm, geom: bool, rcp: int):
+    return pmm.PatchMatchOptions(window_radius=7, window_step=1, sigma_spatial=3.0, sigma_color=0.2,
+                                 num_iterations=2 if geom else 8, geom_consistency=geom,
+                                 init_depth_random=not geom, thk=2, seed=2, rcp_mode=rcp)
+
+
+def sweep_flops(stats) -> float:
+    return (stats.sweep_ncc_samples * FLOP_PER_SAMPLE + stats.sweep_ncc_evaluations * FLOP_PER_EVAL
+            + stats.sweep_geo_evaluations * FLOP_PER_GEO)
+
+
+def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
+    from oracle import binding
+    prob = scene_mod.to_problem(sc)
+    opts = c2_options(pmm, False, 0)
+    view = pmm.ViewDesc(prob, opts)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    W, H = view.width, view.height
+    mid = H // 2
+    # size the band for ~target_s: the cost of one iteration over 2 rows is
+    # the difference of a 2-iteration and a 1-iteration probe (same init)
+    probe = c2_options(pmm, False, 0)
+    probe_t = []
+    for it in (1, 2):
+        probe.num_iterations = it
+        t = time.perf_counter()
+        binding.run_view(probe, view, mid, mid + 2, threads)
+        probe_t.append(time.perf_counter() - t)
+    per_row_iter = max(probe_t[1] - probe_t[0], 1e-4) / 2.0
+    rows = int(max(2, min(64, target_s / (per_row_iter * opts.num_iterations))))
+    r0 = max(0, mid - rows // 2)
+    r1 = min(H, r0 + rows)
+    t = time.perf_counter()
+    binding.run_view(opts, view, r0, r1, threads)
+    dt = time.perf_counter() - t
+    mpix_iter = (r1 - r0) * W * opts.num_iterations / 1e6
+    return {"value": round(mpix_iter / dt, 4), "unit": "Mpixel·iter/s", "cores": threads, "kind": "port",
+            "sample": f"C/OpenMP transcription, photometric pass of the 1920x1080 view, rows [{r0},{r1}) "
+                      f"x {W} px, {opts.num_iterations} iterations, 10 sources, 15x15 step 1; {dt:.1f} s"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus must be 1 "
+                                   "(multi-GPU sharding by reference view is design-only)"}))
+        return 2
+    from sensemap_amd import abi, scene
+    from sensemap_amd import patch_match as pmm
+
+    rcp = abi.PM_RCP_FAST if args.rcp == "fast" else abi.PM_RCP_EXACT
+    sc = scene.heightfield_scene(seed=2)
+    views = [sc.ref] + sc.srcs
+    n_views = len(views)
+    images = [pmm.Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
+    engine = pmm.default_engine()
+
+    # sources' photometric passes (inputs of the geometric pass), untimed
+    photo_opts = c2_options(pmm, False, rcp)
+    src_photo = []
+    for k in range(1, n_views):
+        prob = pmm.Problem(k, [j for j in range(n_views) if j != k], images, sc.depth_min, sc.depth_max)
+        p = pmm.PatchMatchACMMCuda(photo_opts, prob, engine)
+        p.Run()
+        src_photo.append(p)
+    ref_prob = pmm.Problem(0, list(range(1, n_views)), images, sc.depth_min, sc.depth_max,
+                           depth_maps=[v.depth for v in views], normal_maps=[v.normal for v in views])
+    photo = pmm.PatchMatchACMMCuda(photo_opts, ref_prob, engine)
+    geo = pmm.PatchMatchACMMCuda(c2_options(pmm, True, rcp), ref_prob, engine)
+    for i, p in enumerate(src_photo):
+        geo.SetSrcDepthFrom(i, p)
+
+    def step():
+        photo.Run()
+        geo.InitFrom(photo)
+        geo.Run()
+
+    # calibration step with counting on (deterministic: same evaluations as
+    # every timed step); the atomics it adds are off for the timed steps
+    engine.set_counting(True)
+    step()
+    s1, s2 = photo.GetStats(), geo.GetStats()
+    engine.set_counting(False)
+    flops_sweep = sweep_flops(s1) + sweep_flops(s2)
+    launches_per_step = s1.sweep_launches + s2.sweep_launches
+
+    for _ in range(args.warmup):
+        step()
+    lat = []
+    sweep_ms = 0.0
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        step()
+        lat.append((time.perf_counter() - t) * 1000.0)
+        sweep_ms += photo.GetStats().sweep_ms + geo.GetStats().sweep_ms
+    total_s = time.perf_counter() - t_all
+    ms_per_step = total_s * 1000.0 / args.steps
+    W, H = sc.ref.image.shape[1], sc.ref.image.shape[0]
+    iters = photo_opts.num_iterations + 2
+    mpix_iter = W * H * iters / 1e6
+    value = mpix_iter / (ms_per_step / 1000.0)
+    err = np.abs(geo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
+    err_photo = np.abs(photo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
+    avg_launch_ms = sweep_ms / (args.steps * launches_per_step)
+    achieved = (flops_sweep / launches_per_step) / (avg_launch_ms / 1000.0) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpixel·iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1920x1080 reference + 10 sources, heightfield scene, window 15x15 step 1, "
+                               "8 photometric + 2 geometric iterations, one view per launch",
+                   "width": W, "height": H, "num_src": n_views - 1, "window_radius": 7, "window_step": 1,
+                   "iterations": iters, "rcp_mode": args.rcp, "image_storage": "u8"},
+        "views_per_s": round(1000.0 / ms_per_step, 4),
+        "median_rel_depth_error": float(np.median(err)),
+        "frac_within_1pct": float(np.mean(err < 0.01)),
+        "median_rel_depth_error_photometric": float(np.median(err_photo)),
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "roofline": {
+            "kernel": "k_sweep (checkerboard propagation + view selection + refinement)",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(avg_launch_ms, 3),
+            "flop_per_launch": int(flops_sweep / launches_per_step),
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pmm, scene, sc, args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

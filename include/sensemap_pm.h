@@ -144,12 +144,18 @@ typedef struct pm_run_stats {
   double total_ms;          /* whole Run() on the device */
   double sweep_ms;          /* sum over all checkerboard sweep launches */
   double local_opt_ms;      /* sum over local-optimisation launches */
-  double init_ms;           /* moments + init + init-conf launches */
+  double init_ms;           /* init + init-conf launches */
   int32_t sweep_launches;
   int32_t local_opt_launches;
-  uint64_t ncc_evaluations; /* (hypothesis, source) NCC evaluations, when counting was enabled */
-  uint64_t ncc_samples;     /* window samples of those evaluations */
-  uint64_t geo_evaluations; /* geometric-consistency terms */
+  /* When counting is enabled (pm_engine_set_counting): (hypothesis, source)
+   * NCC evaluations, their window samples and geometric-consistency terms,
+   * in total and for the sweep kernel alone. */
+  uint64_t ncc_evaluations;
+  uint64_t ncc_samples;
+  uint64_t geo_evaluations;
+  uint64_t sweep_ncc_evaluations;
+  uint64_t sweep_ncc_samples;
+  uint64_t sweep_geo_evaluations;
 } pm_run_stats;
 
 typedef struct pm_engine pm_engine;
@@ -189,6 +195,18 @@ pm_status pm_problem_get_stats(const pm_problem* problem, pm_run_stats* out);
  * patch_match_cuda.cu:493-580). */
 pm_status pm_problem_get_poses(const pm_problem* problem, float* poses, float ref_k8[8]);
 void pm_problem_destroy(pm_problem* problem);
+
+/* Device-to-device hand-off between passes (the reference goes through
+ * host DepthMap/NormalMap caches, patch_match_controller.cc:4571-4595,
+ * 4679-4680).  Both problems must live on the same engine.
+ * pm_problem_init_from: use `from`'s current depth and normal maps as the
+ *   initial state of `dst` (dst created with init_depth_random = 0, the
+ *   geometric pass; InitWorkspace, patch_match_cuda.cu:648-665).
+ * pm_problem_set_src_depth: replace the depth map of source `src_index` of
+ *   `dst` (geometric pass input, patch_match_cuda.cu:318-368) with `from`'s
+ *   current depth map; sizes must match. */
+pm_status pm_problem_init_from(pm_problem* dst, const pm_problem* from);
+pm_status pm_problem_set_src_depth(pm_problem* dst, int32_t src_index, const pm_problem* from);
 
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,

@@ -1024,21 +1024,23 @@ typedef void (*pixel_fn)(const pass_ctx*, int, int);
  * not matter. */
 static void run_color(const pass_ctx* x, pixel_fn fn, int color, int r0, int r1) {
   const int W = x->v->W;
-  long r;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (r = r0; r < r1; ++r) {
-    const int row = (int)r;
-    for (int col = (row + color) & 1; col < W; col += 2) fn(x, row, col);
+  const long half = (W + 1) / 2;
+  const long total = (long)(r1 - r0) * half;
+  long idx;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (idx = 0; idx < total; ++idx) {
+    const int row = r0 + (int)(idx / half);
+    const int col = 2 * (int)(idx % half) + ((row + color) & 1);
+    if (col < W) fn(x, row, col);
   }
 }
 
 static void run_all(const pass_ctx* x, pixel_fn fn, int r0, int r1) {
   const int W = x->v->W;
-  long r;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (r = r0; r < r1; ++r) {
-    for (int col = 0; col < W; ++col) fn(x, (int)r, col);
-  }
+  const long total = (long)(r1 - r0) * W;
+  long idx;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (idx = 0; idx < total; ++idx) fn(x, r0 + (int)(idx / W), (int)(idx % W));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -113,6 +113,9 @@ class pm_run_stats(C.Structure):
         ("ncc_evaluations", C.c_uint64),
         ("ncc_samples", C.c_uint64),
         ("geo_evaluations", C.c_uint64),
+        ("sweep_ncc_evaluations", C.c_uint64),
+        ("sweep_ncc_samples", C.c_uint64),
+        ("sweep_geo_evaluations", C.c_uint64),
     ]
 
 
@@ -136,6 +139,8 @@ _SIGNATURES = {
     "pm_problem_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_run_stats)]),
     "pm_problem_get_poses": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pm_problem_destroy": (None, [C.c_void_p]),
+    "pm_problem_init_from": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pm_problem_set_src_depth": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "pm_run_view": (C.c_int, [C.c_void_p, C.POINTER(pm_options), C.POINTER(pm_view_desc),
                               C.POINTER(pm_outputs)]),
 }

@@ -450,7 +450,7 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   if ((st = p->alloc(&h.view_sel, plane)) != PM_OK) return st;
   const int slots = pm::sweep_slots(W, H);
   if ((st = p->alloc(&h.ws, (size_t)slots * pm::ws_floats_per_slot(N))) != PM_OK) return st;
-  if ((st = p->alloc(&p->counters, 3)) != PM_OK) return st;
+  if ((st = p->alloc(&p->counters, 6)) != PM_OK) return st;
   if (!o->init_depth_random) {
     if ((st = p->alloc(&p->init_depth, plane)) != PM_OK) return st;
     if ((st = p->alloc(&p->init_normal, 3 * plane)) != PM_OK) return st;
@@ -552,7 +552,7 @@ PM_API pm_status pm_problem_run(pm_problem* p) {
     p->host.counters = cnt;
     PM_HIP(hipMemcpyAsync(p->dev, &p->host, sizeof(p->host), hipMemcpyHostToDevice, s));
   }
-  if (cnt) PM_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), s));
+  if (cnt) PM_HIP(hipMemsetAsync(cnt, 0, 6 * sizeof(unsigned long long), s));
   if (!o.init_depth_random) {
     PM_HIP(hipMemcpyAsync(p->host.depth, p->init_depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
     PM_HIP(hipMemcpyAsync(p->host.normal, p->init_normal, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -597,11 +597,14 @@ PM_API pm_status pm_problem_run(pm_problem* p) {
   p->stats.sweep_launches = (int32_t)sweeps.size();
   p->stats.local_opt_launches = (int32_t)locals.size();
   if (cnt) {
-    unsigned long long c[3];
+    unsigned long long c[6];
     PM_HIP(hipMemcpy(c, cnt, sizeof(c), hipMemcpyDeviceToHost));
     p->stats.ncc_evaluations = c[0];
     p->stats.ncc_samples = c[1];
     p->stats.geo_evaluations = c[2];
+    p->stats.sweep_ncc_evaluations = c[3];
+    p->stats.sweep_ncc_samples = c[4];
+    p->stats.sweep_geo_evaluations = c[5];
   }
   return PM_OK;
 }
@@ -647,6 +650,32 @@ PM_API pm_status pm_problem_get_poses(const pm_problem* p, float* poses, float r
   if (!p || !poses || !ref_k8) return fail(PM_ERR_INVALID_ARG, "null argument");
   std::memcpy(poses, p->poses.data(), p->poses.size() * sizeof(float));
   std::memcpy(ref_k8, p->refk8, sizeof(p->refk8));
+  return PM_OK;
+}
+
+PM_API pm_status pm_problem_init_from(pm_problem* dst, const pm_problem* from) {
+  if (!dst || !from) return fail(PM_ERR_INVALID_ARG, "null problem");
+  if (dst->engine != from->engine) return fail(PM_ERR_INVALID_ARG, "problems live on different engines");
+  if (dst->W != from->W || dst->H != from->H) return fail(PM_ERR_INVALID_ARG, "map sizes differ");
+  if (!dst->init_depth) return fail(PM_ERR_INVALID_ARG, "destination was created with init_depth_random");
+  const size_t plane = (size_t)dst->W * dst->H;
+  hipStream_t s = dst->engine->stream;
+  PM_HIP(hipSetDevice(dst->engine->device));
+  PM_HIP(hipMemcpyAsync(dst->init_depth, from->host.depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+  PM_HIP(hipMemcpyAsync(dst->init_normal, from->host.normal, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+  return PM_OK;
+}
+
+PM_API pm_status pm_problem_set_src_depth(pm_problem* dst, int32_t src_index, const pm_problem* from) {
+  if (!dst || !from) return fail(PM_ERR_INVALID_ARG, "null problem");
+  if (dst->engine != from->engine) return fail(PM_ERR_INVALID_ARG, "problems live on different engines");
+  if (!dst->opt.geom_consistency) return fail(PM_ERR_INVALID_ARG, "destination has no geometric-consistency term");
+  if (src_index < 0 || src_index >= dst->N) return fail(PM_ERR_INVALID_ARG, "source index out of range");
+  const pm::SrcDev& sd = dst->host.src[src_index];
+  if (sd.width != from->W || sd.height != from->H) return fail(PM_ERR_INVALID_ARG, "depth map size differs from source");
+  PM_HIP(hipSetDevice(dst->engine->device));
+  PM_HIP(pm::launch_build_depth_quads(from->host.depth, from->W, from->H, const_cast<float4*>(sd.depth_q),
+                                      dst->engine->stream));
   return PM_OK;
 }
 

@@ -96,11 +96,17 @@ struct Counters {
   unsigned long long ncc_eval = 0, ncc_samples = 0, geo_eval = 0;
 };
 
-__device__ __forceinline__ void flush_counters(const ViewDev* v, const Counters& c) {
+// Counter slots: [0..2] all kernels, [3..5] the sweep kernel alone.
+__device__ __forceinline__ void flush_counters(const ViewDev* v, const Counters& c, bool sweep = false) {
   if (v->counters) {
     atomicAdd(&v->counters[0], c.ncc_eval);
     atomicAdd(&v->counters[1], c.ncc_samples);
     atomicAdd(&v->counters[2], c.geo_eval);
+    if (sweep) {
+      atomicAdd(&v->counters[3], c.ncc_eval);
+      atomicAdd(&v->counters[4], c.ncc_samples);
+      atomicAdd(&v->counters[5], c.geo_eval);
+    }
   }
 }
 
@@ -696,7 +702,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restri
     if (n1 >= v->num_good && n2 <= v->num_bad) flags |= 1u << i;
   }
   if (flags == 0) {
-    flush_counters(v, cnt);
+    flush_counters(v, cnt, true);
     return;
   }
   float psum = 0.0f;
@@ -726,7 +732,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restri
   float wnorm = 0.0f;
   for (int i = 0; i < N; ++i) wnorm = wnorm + ws.RM(0, i);
   if (wnorm <= 1e-5f) {
-    flush_counters(v, cnt);
+    flush_counters(v, cnt, true);
     return;
   }
   const float inv_wnorm = 1.0f / wnorm;
@@ -810,7 +816,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restri
     }
   }
   if (b2 < 0) {  // every refined cost NaN: the reference writes nothing
-    flush_counters(v, cnt);
+    flush_counters(v, cnt, true);
     return;
   }
   G(v->conf)[p] = minc;
@@ -819,7 +825,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restri
   G(v->normal)[plane + p] = ws.hyp(hb + b2, 1);
   G(v->normal)[2 * plane + p] = ws.hyp(hb + b2, 2);
   for (int i = 0; i < N; ++i) G(v->cost)[i * plane + p] = ws.RM(b2, i);
-  flush_counters(v, cnt);
+  flush_counters(v, cnt, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -256,6 +256,14 @@ class PatchMatchACMMCuda:
                                                 k8.ctypes.data_as(C.POINTER(C.c_float))))
         return poses, k8
 
+    def InitFrom(self, other: "PatchMatchACMMCuda") -> None:
+        """Initial depth/normal from another problem's current maps (on device)."""
+        abi.check(self.lib.pm_problem_init_from(self.handle, other.handle))
+
+    def SetSrcDepthFrom(self, src_index: int, other: "PatchMatchACMMCuda") -> None:
+        """Source `src_index`'s depth map from another problem's current depth map (on device)."""
+        abi.check(self.lib.pm_problem_set_src_depth(self.handle, int(src_index), other.handle))
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.pm_problem_destroy(self.handle)

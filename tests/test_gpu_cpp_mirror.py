@@ -1,0 +1,47 @@
+"""The C++ mirror of the reference interface (sensemap_amd/host/patch_match_cuda.h)
+drives the same engine: its depth and normal maps equal the Python path's
+bitwise on the same view."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from sensemap_amd import scene
+from sensemap_amd import patch_match as pmm
+from tests import helpers
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "bin", "shim_run")
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_matches_python_path(tmp_path):
+    assert os.path.exists(BIN), "tests/cpp/bin/shim_run is built by __graft_entry__.build()"
+    sc = helpers.small_planes()
+    opts = helpers.c1_options(num_iterations=3)
+    views = [sc.ref] + sc.srcs
+    lines = [f"{160} {120} {len(sc.srcs)} {sc.depth_min!r} {sc.depth_max!r} {opts.window_radius} {opts.window_step} "
+             f"{opts.sigma_spatial!r} {opts.sigma_color!r} {opts.num_iterations} {opts.thk} {opts.seed}"]
+    for k, v in enumerate(views):
+        c = v.camera
+        vals = list(c.K) + [float(x) for x in np.asarray(c.R, np.float32).reshape(9)] + \
+            [float(x) for x in np.asarray(c.T, np.float32)]
+        lines.append(" ".join(repr(float(np.float32(x))) for x in vals))
+        v.image.tofile(str(tmp_path / f"img{k}.u8"))
+    (tmp_path / "meta.txt").write_text("\n".join(lines) + "\n")
+    subprocess.run([BIN, str(tmp_path)], check=True, timeout=120)
+    d_cpp = np.fromfile(str(tmp_path / "depth.f32"), np.float32).reshape(120, 160)
+    n_cpp = np.fromfile(str(tmp_path / "normal.f32"), np.float32).reshape(3, 120, 160)
+    prob = scene.to_problem(sc)
+    e = pmm.PatchMatchACMMCuda(opts, prob)
+    e.Run()
+    assert np.array_equal(d_cpp.view(np.uint32), e.GetDepthMap().view(np.uint32))
+    assert np.array_equal(n_cpp.view(np.uint32), e.GetNormalMap().view(np.uint32))
+
+
+def test_cpp_mirror_header_compiles(tmp_path):
+    src = tmp_path / "inc.cpp"
+    src.write_text('#include "sensemap_amd/host/patch_match_cuda.h"\nint main() { '
+                   'sensemap_amd::mvs::PatchMatchOptions o; return o.window_radius == 5 ? 0 : 1; }\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", ROOT, str(src)], check=True)
