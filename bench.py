@@ -49,7 +49,29 @@ def parse():
     ap.add_argument("--rcp", choices=["exact", "fast"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
+    ap.add_argument("--only-photometric", action="store_true",
+                    help="profiling mode: the step is the photometric pass alone, no source passes")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="profiling mode: one device-to-device copy of MIB MiB, then exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> int:
+    """One hipMemcpy device-to-device of a known size: the reference point
+    that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}))
+    return 0
 
 
 def c2_options(pmm, geom: bool, rcp: int):
@@ -99,6 +121,8 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU: bench.py --gpus must be 1 "
                                    "(multi-GPU sharding by reference view is design-only)"}))
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     from sensemap_amd import abi, scene
     from sensemap_amd import patch_match as pmm
 
@@ -112,7 +136,7 @@ def main() -> int:
     # sources' photometric passes (inputs of the geometric pass), untimed
     photo_opts = c2_options(pmm, False, rcp)
     src_photo = []
-    for k in range(1, n_views):
+    for k in range(1, 1 if args.only_photometric else n_views):
         prob = pmm.Problem(k, [j for j in range(n_views) if j != k], images, sc.depth_min, sc.depth_max)
         p = pmm.PatchMatchACMMCuda(photo_opts, prob, engine)
         p.Run()
@@ -126,17 +150,19 @@ def main() -> int:
 
     def step():
         photo.Run()
-        geo.InitFrom(photo)
-        geo.Run()
+        if not args.only_photometric:
+            geo.InitFrom(photo)
+            geo.Run()
 
     # calibration step with counting on (deterministic: same evaluations as
     # every timed step); the atomics it adds are off for the timed steps
     engine.set_counting(True)
     step()
-    s1, s2 = photo.GetStats(), geo.GetStats()
+    s1 = photo.GetStats()
+    s2 = None if args.only_photometric else geo.GetStats()
     engine.set_counting(False)
-    flops_sweep = sweep_flops(s1) + sweep_flops(s2)
-    launches_per_step = s1.sweep_launches + s2.sweep_launches
+    flops_sweep = sweep_flops(s1) + (sweep_flops(s2) if s2 else 0)
+    launches_per_step = s1.sweep_launches + (s2.sweep_launches if s2 else 0)
 
     for _ in range(args.warmup):
         step()
@@ -147,14 +173,15 @@ def main() -> int:
         t = time.perf_counter()
         step()
         lat.append((time.perf_counter() - t) * 1000.0)
-        sweep_ms += photo.GetStats().sweep_ms + geo.GetStats().sweep_ms
+        sweep_ms += photo.GetStats().sweep_ms + (0.0 if args.only_photometric else geo.GetStats().sweep_ms)
     total_s = time.perf_counter() - t_all
     ms_per_step = total_s * 1000.0 / args.steps
     W, H = sc.ref.image.shape[1], sc.ref.image.shape[0]
-    iters = photo_opts.num_iterations + 2
+    iters = photo_opts.num_iterations + (0 if args.only_photometric else 2)
     mpix_iter = W * H * iters / 1e6
     value = mpix_iter / (ms_per_step / 1000.0)
-    err = np.abs(geo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
+    final = photo if args.only_photometric else geo
+    err = np.abs(final.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
     err_photo = np.abs(photo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
     avg_launch_ms = sweep_ms / (args.steps * launches_per_step)
     achieved = (flops_sweep / launches_per_step) / (avg_launch_ms / 1000.0) / 1e12

@@ -274,12 +274,17 @@ static void compose_homography(const float* pose, const float iK[4], int row, in
  * reference's "+0.5" is CUDA's texel-centre convention,
  * kernel_functions.cu:1296-1301).  Coordinates are clamped into
  * [-3, W+2] first, which leaves every in-range sample unchanged and maps any
- * farther one (and NaN) to all-zero texels. */
+ * farther one (and NaN) to all-zero texels.  The fraction is
+ * min(u - floor(u), 1 - 2^-24): u - floor(u) rounds up to 1.0 for tiny
+ * negative u, the canonical fraction stays below 1 (the gfx950 v_fract_f32
+ * definition). */
+static inline float fract_canon(float u) { return fminf(u - floorf(u), 0.99999994f); }
+
 static inline float bilinear_u8(const uint8_t* img, int W, int H, float u, float v) {
   u = fminf(fmaxf(u, -3.0f), (float)W + 2.0f);
   v = fminf(fmaxf(v, -3.0f), (float)H + 2.0f);
   const float x0f = floorf(u), y0f = floorf(v);
-  const float fx = u - x0f, fy = v - y0f;
+  const float fx = fract_canon(u), fy = fract_canon(v);
   const int x0 = (int)x0f, y0 = (int)y0f;
   float t[4];
   for (int q = 0; q < 4; ++q) {

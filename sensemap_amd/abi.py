@@ -145,7 +145,8 @@ _SIGNATURES = {
                               C.POINTER(pm_outputs)]),
 }
 
-ENGINE_LIB = os.path.join(LIB_DIR, "libsensemap_pm.so")
+# SENSEMAP_PM_LIB selects another build of the engine (tuning variants).
+ENGINE_LIB = os.environ.get("SENSEMAP_PM_LIB") or os.path.join(LIB_DIR, "libsensemap_pm.so")
 _engine = None
 
 

@@ -30,6 +30,8 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip"]
+# extra compile flags (tuning knobs such as -DPM_SWEEP_WAVES_PER_EU=6)
+EXTRA = os.environ.get("PM_EXTRA_HIPFLAGS", "").split()
 
 
 def _run(cmd: list[str]) -> None:
@@ -39,23 +41,25 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
-def build_engine() -> str:
+def build_engine(out_path: str | None = None) -> str:
+    import shutil
+    import tempfile
     os.makedirs(LIB, exist_ok=True)
+    objdir = tempfile.mkdtemp(prefix="pm_build_")
     objs = []
     jobs = []
     for src in HIP_SOURCES:
-        obj = os.path.join(LIB, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([HIPCC, *HIP_FLAGS, *EXTRA, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         for f in [ex.submit(_run, j) for j in jobs]:
             f.result()
-    out = os.path.join(LIB, "libsensemap_pm.so")
+    out = out_path or os.path.join(LIB, "libsensemap_pm.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
-    for o in objs:
-        os.remove(o)
+    shutil.rmtree(objdir, ignore_errors=True)
     return out
 
 

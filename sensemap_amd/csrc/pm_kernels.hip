@@ -110,15 +110,24 @@ __device__ __forceinline__ void flush_counters(const ViewDev* v, const Counters&
   }
 }
 
+// 1/z of a window sample.
+// PM_RCP_EXACT: v_rcp_f32 refined by two Newton steps with FMA.  On gfx950
+//   v_rcp_f32 is within one ulp of the correctly rounded reciprocal for every
+//   significand (and scales exactly with exponent and sign), and two Newton
+//   steps from any seed within one ulp land on the correctly rounded value,
+//   so this equals 1.0f / z for every normal z, bit for bit, at about half
+//   the instructions of the IEEE division sequence.
+// PM_RCP_FAST: one Newton step (within one ulp, not always correctly rounded).
 template <int RCP>
 __device__ __forceinline__ float rcp_z(float z) {
+  float r = __builtin_amdgcn_rcpf(z);
+  float e = fmaf(-z, r, 1.0f);
+  r = fmaf(r, e, r);
   if constexpr (RCP == PM_RCP_EXACT) {
-    return 1.0f / z;
-  } else {
-    float r = __builtin_amdgcn_rcpf(z);
-    const float e = fmaf(-z, r, 1.0f);
-    return fmaf(r, e, r);
+    e = fmaf(-z, r, 1.0f);
+    r = fmaf(r, e, r);
   }
+  return r;
 }
 
 // Fetch the 2x2 footprint of source texel (x0, y0) from the quad layout.
@@ -141,6 +150,15 @@ __device__ __forceinline__ void fetch_quad(const void* tex, uint32_t off, float&
     t2 = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
     t3 = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
   }
+}
+
+// floor(x) as int in one instruction (v_cvt_flr_i32_f32); the compiler
+// emits v_floor_f32 + v_cvt_i32_f32 for (int)floorf(x).  Exact for the
+// clamped coordinate range [-3, W+2].
+__device__ __forceinline__ int cvt_flr(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
 }
 
 __device__ __forceinline__ void neumaier_add(float& s, float& c, float x) {
@@ -227,9 +245,10 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
         float u = x * rz, w_ = y * rz;
         u = fminf(fmaxf(u, -3.0f), umax);
         w_ = fminf(fmaxf(w_, -3.0f), vmax);
-        const float x0f = floorf(u), y0f = floorf(w_);
-        const float fx = u - x0f, fy = w_ - y0f;
-        const uint32_t off = __umul24((uint32_t)((int)y0f + kQuadPad), upitch) + (uint32_t)((int)x0f + kQuadPad);
+        // fraction = min(u - floor(u), 1 - 2^-24): v_fract_f32
+        const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(w_);
+        const int x0 = cvt_flr(u), y0 = cvt_flr(w_);
+        const uint32_t off = __umul24((uint32_t)(y0 + kQuadPad), upitch) + (uint32_t)(x0 + kQuadPad);
         float t0, t1, t2, t3;
         fetch_quad<STORE>(tex, off, t0, t1, t2, t3);
         const float top = fmaf(fx, t1 - t0, t0);
@@ -237,7 +256,7 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
         const float sv = fmaf(fy, bot - top, top);
         const uint32_t ri = trow[j * S];
         const float r = (float)ri;
-        const float w = sprow[j] * ct[__sad((uint32_t)c, ri, 0u)];
+        const float w = sprow[j] * ct[__builtin_amdgcn_sad_u8((uint32_t)c, ri, 0u)];
         if constexpr (!kShifted) {
           const float ws = w * sv;
           s1 = s1 + ws;
@@ -267,8 +286,7 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
       bz = bz + hzr;
     }
   };
-  const bool shifted = px.shifted;
-  if (!shifted) {
+  if (!px.shifted) {
     walk(std::false_type{});
   } else {
     walk(std::true_type{});
@@ -276,7 +294,7 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
   cnt.ncc_samples += (unsigned long long)(n * n);
   const float inv_sw = px.inv_sw;
   float var_s, cov;
-  if (!shifted) {
+  if (!px.shifted) {
     const float mu_s = s1 * inv_sw;
     const float m2_s = s2 * inv_sw;
     const float m_rs = s3 * inv_sw;
@@ -454,6 +472,20 @@ struct TileSize {
 // Stage the colour table and the ref tile (zero outside the image) into
 // LDS, then map the thread to its pixel.  Returns false for threads whose
 // pixel lies outside the image (after the barrier).
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Give each XCD a contiguous run of tiles (row-major over the tile
+// grid) so the workgroups resident on one XCD cover one compact band of the
+// image and share source footprints in that XCD's L2.
+__device__ __forceinline__ void tile_of_block(int tiles_x, int& tx, int& ty) {
+  const int B = gridDim.x;
+  const int b = blockIdx.x;
+  const int q = B >> 3, r = B & 7;
+  const int xcd = b & 7, j = b >> 3;
+  const int t = xcd * q + (xcd < r ? xcd : r) + j;
+  ty = t / tiles_x;
+  tx = t - ty * tiles_x;
+}
+
 __device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int color, int R, uint8_t* tile, float* ct,
                                            float* spl, Pix& px, int& tw) {
   const int t = threadIdx.x;
@@ -461,7 +493,9 @@ __device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int co
   const int nsp = v->nwin * v->nwin;
   for (int e = t; e < nsp; e += kSweepThreads) spl[e] = G(v->sp)[e];
   const int W = v->W, H = v->H;
-  const int row0 = blockIdx.y * kTileH, col0 = blockIdx.x * kTileW;
+  int tx, ty;
+  tile_of_block((W + kTileW - 1) / kTileW, tx, ty);
+  const int row0 = ty * kTileH, col0 = tx * kTileW;
   tw = kTileW + 2 * R;
   const int th = kTileH + 2 * R;
   for (int e = t; e < tw * th; e += kSweepThreads) {
@@ -487,9 +521,7 @@ __device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int co
   return true;
 }
 
-__device__ __forceinline__ int slot_index() {
-  return (blockIdx.y * gridDim.x + blockIdx.x) * kSweepThreads + threadIdx.x;
-}
+__device__ __forceinline__ int slot_index() { return blockIdx.x * kSweepThreads + threadIdx.x; }
 
 // ---------------------------------------------------------------------------
 // init: random (best of init_samples) or given depth/normal, then InitConf
@@ -576,8 +608,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
 // ---------------------------------------------------------------------------
 // checkerboard sweep
 
+// Optional occupancy target for the sweep kernel (waves per SIMD), a build
+// knob for tuning register use against latency hiding.
+#ifdef PM_SWEEP_WAVES_PER_EU
+#define PM_SWEEP_OCC __attribute__((amdgpu_waves_per_eu(PM_SWEEP_WAVES_PER_EU, 8)))
+#else
+#define PM_SWEEP_OCC
+#endif
+
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(const ViewDev* __restrict__ v, int color, float th_mc,
+__global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const ViewDev* __restrict__ v, int color, float th_mc,
                                                          uint32_t pass) {
   __shared__ uint8_t tile[TileSize<R_>::kBytes];
   __shared__ float ct[256];
@@ -934,9 +974,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const ViewDev* __re
 
 __global__ __launch_bounds__(kSweepThreads) void k_filter(const ViewDev* __restrict__ v, int color) {
   const int t = threadIdx.x;
-  const int row = blockIdx.y * kTileH + (t >> 4);
-  const int col = blockIdx.x * kTileW + 2 * (t & 15) + ((row + color) & 1);
   const int W = v->W, H = v->H;
+  int tx, ty;
+  tile_of_block((W + kTileW - 1) / kTileW, tx, ty);
+  const int row = ty * kTileH + (t >> 4);
+  const int col = tx * kTileW + 2 * (t & 15) + ((row + color) & 1);
   if (row >= H || col >= W) return;
   const int plane = W * H;
   const int p = row * W + col;
@@ -1117,13 +1159,16 @@ void* pick_kernel(const LaunchCfg& cfg) {
   return pick_sr<Kn, -1, -1>(cfg.storage, cfg.rcp);
 }
 
-dim3 sweep_grid(int W, int H) { return dim3((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, 1); }
+// One-dimensional grid over the tiles; tile_of_block() maps blocks to tiles.
+dim3 sweep_grid(int W, int H) {
+  return dim3((unsigned)(((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH)), 1, 1);
+}
 
 }  // namespace
 
 int sweep_slots(int W, int H) {
   const dim3 g = sweep_grid(W, H);
-  return (int)(g.x * g.y) * kSweepThreads;
+  return (int)g.x * kSweepThreads;
 }
 
 hipError_t launch_build_quads_u8(const uint8_t* img, int W, int H, int stride, uint32_t* out, hipStream_t s) {
