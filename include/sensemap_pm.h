@@ -208,6 +208,26 @@ void pm_problem_destroy(pm_problem* problem);
 pm_status pm_problem_init_from(pm_problem* dst, const pm_problem* from);
 pm_status pm_problem_set_src_depth(pm_problem* dst, int32_t src_index, const pm_problem* from);
 
+/* ---- batched form (no counterpart in the reference) ----
+ * Many reference views, of uneven size, source count and window shape, run
+ * together: at every step each view advances by one phase of its Run()
+ * schedule, and views whose phase uses the same kernel share one launch.
+ * The reference runs one view per engine and serialises launches per host
+ * thread (patch_match_controller.cc:3620-3632, patch_match_cuda.cu:841-880).
+ * options and views hold num_views entries each; every view follows its own
+ * options (iterations, window, geometric pass, ...).  Results are read per
+ * view with pm_problem_get_* on the handle from pm_batch_get_view. */
+typedef struct pm_batch pm_batch;
+pm_status pm_batch_create(pm_engine* engine, int32_t num_views, const pm_options* options,
+                          const pm_view_desc* views, pm_batch** out);
+/* Capture the launch sequence into a hipGraph on the next run and replay it
+ * afterwards (counting runs stay eager). */
+pm_status pm_batch_set_graph(pm_batch* batch, int32_t enable);
+pm_status pm_batch_run(pm_batch* batch);
+pm_status pm_batch_get_view(const pm_batch* batch, int32_t index, pm_problem** view);
+pm_status pm_batch_get_stats(const pm_batch* batch, pm_run_stats* out);
+void pm_batch_destroy(pm_batch* batch);
+
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,
                       const pm_view_desc* view, const pm_outputs* outputs);

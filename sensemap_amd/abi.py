@@ -143,6 +143,13 @@ _SIGNATURES = {
     "pm_problem_set_src_depth": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "pm_run_view": (C.c_int, [C.c_void_p, C.POINTER(pm_options), C.POINTER(pm_view_desc),
                               C.POINTER(pm_outputs)]),
+    "pm_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(pm_options), C.POINTER(pm_view_desc),
+                                  C.POINTER(C.c_void_p)]),
+    "pm_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pm_batch_run": (C.c_int, [C.c_void_p]),
+    "pm_batch_get_view": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pm_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_run_stats)]),
+    "pm_batch_destroy": (None, [C.c_void_p]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

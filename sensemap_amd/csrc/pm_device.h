@@ -45,7 +45,7 @@ struct ViewDev {
   float max_ncc_matching_cost, inv_beta2;
   int32_t num_good, num_bad, num_samples, random_optimization;
   int32_t thk, init_samples, init_random, compat_lo_geo_bug;
-  int32_t accum_mode, slots_per_view;
+  int32_t accum_mode, pad_ws;
   float conf_threshold, depth_diff_threshold;
   uint64_t seed;
   const uint8_t* ref;       // W*H
@@ -59,12 +59,12 @@ struct ViewDev {
   float* conf;              // W*H
   float* cost;              // N*W*H planar
   uint32_t* view_sel;       // W*H
-  float* ws;                // workspace, slots_per_view slots, SoA
   unsigned long long* counters;  // [ncc_eval, ncc_samples, geo_eval] or null
   SrcDev src[PM_MAX_NUM_SRC];
 };
 
-// Workspace fields (SoA: field f of slot s at ws[f * slots + s]).
+// Workspace fields per thread of a launch (SoA over the launch's threads:
+// field f of slot s at ws[f * ws_slots + s], ws_slots = blocks * 256).
 //   hyp   : 4 * (kMaxHyp + kNumRefine) floats  (nx, ny, nz, d per hypothesis)
 //   M     : kMaxHyp * N floats      (candidate x source costs)
 //   RM    : kNumRefine * N floats   (refinement x source costs)
@@ -73,10 +73,48 @@ inline int ws_floats_per_slot(int N) {
   return 4 * (kMaxHyp + kNumRefine) + kMaxHyp * N + kNumRefine * N + N;
 }
 
+// One entry of a launch table: one view advanced by one phase.  A launch
+// covers the entries' blocks back to back; block_start is the first block
+// (after the XCD remap) of the entry.
+struct LaunchEntry {
+  const ViewDev* v;
+  int32_t block_start;
+  int32_t tile_offset;  // first tile of the view this entry covers
+  int32_t color;        // checkerboard colour of the pixels this entry updates
+  uint32_t pass;        // random-stream pass id
+  float th_mc;          // good-matching threshold of the iteration (sweep)
+  int32_t pad0;
+};
+
+// Every block of a launch owns 256 workspace slots (its threads), so no
+// two resident workgroups ever share workspace lines; the launch boundary
+// makes the workspace coherent across the XCDs' L2 caches.
+struct LaunchTable {
+  int32_t n;             // entries
+  int32_t total_blocks;  // blocks of the launch
+  int32_t ws_slots;      // total_blocks * 256
+  int32_t pad0;
+  float* ws;             // workspace of the launch
+  LaunchEntry e[1];      // n entries follow (allocated with the table)
+};
+
+constexpr int kMaxBlocksPerLaunch = 16384;
+
+inline size_t launch_table_bytes(int n) {
+  return sizeof(LaunchTable) + (size_t)(n > 1 ? n - 1 : 0) * sizeof(LaunchEntry);
+}
+
+// Blocks (256-thread workgroups, one 32x16 tile of one colour each) a view
+// needs per colour pass.
+inline int view_blocks(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
+
 struct LaunchCfg {
   int storage;   // PM_STORAGE_*
   int rcp;       // PM_RCP_*
   int radius, step;
+  bool operator==(const LaunchCfg& o) const {
+    return storage == o.storage && rcp == o.rcp && radius == o.radius && step == o.step;
+  }
 };
 
 // Kernel launch wrappers (pm_kernels.hip).  Every launch checks the grid it
@@ -88,13 +126,10 @@ hipError_t launch_build_quads_f16(const uint8_t* img, int W, int H, int stride, 
 hipError_t launch_build_depth_quads(const float* depth, int W, int H, float4* out, hipStream_t s);
 hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, float4* refm,
                               uint8_t* grad_ok, hipStream_t s);
-hipError_t launch_init(const ViewDev* v, const LaunchCfg& cfg, int W, int H, uint32_t pass,
-                       hipStream_t s);
-hipError_t launch_sweep(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, float th_mc,
-                        uint32_t pass, hipStream_t s);
-hipError_t launch_local_opt(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color,
-                            uint32_t pass, hipStream_t s);
-hipError_t launch_filter(const ViewDev* v, int W, int H, int color, hipStream_t s);
-int sweep_slots(int W, int H);   // threads launched per colour pass
+// Table-driven colour-pass launches: `table` is a device copy of a
+// LaunchTable whose total_blocks the caller passes as `blocks`.
+enum PhaseKind { kPhaseInit = 0, kPhaseSweep = 1, kPhaseLocal = 2, kPhaseFilter = 3 };
+hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int blocks,
+                        hipStream_t s);
 
 }  // namespace pm

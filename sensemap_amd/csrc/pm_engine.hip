@@ -65,6 +65,7 @@ struct pm_problem {
   float refk8[8] = {};
   pm_run_stats stats{};
   size_t device_bytes = 0;
+  struct pm_schedule* schedule = nullptr;
 
   template <typename T>
   pm_status alloc(T** p, size_t count) {
@@ -80,9 +81,7 @@ struct pm_problem {
     *p = (T*)q;
     return PM_OK;
   }
-  ~pm_problem() {
-    for (void* p : allocs) (void)hipFree(p);
-  }
+  ~pm_problem();
 };
 
 // ---------------------------------------------------------------------------
@@ -448,8 +447,6 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   if ((st = p->alloc(&h.conf, plane)) != PM_OK) return st;
   if ((st = p->alloc(&h.cost, (size_t)N * plane)) != PM_OK) return st;
   if ((st = p->alloc(&h.view_sel, plane)) != PM_OK) return st;
-  const int slots = pm::sweep_slots(W, H);
-  if ((st = p->alloc(&h.ws, (size_t)slots * pm::ws_floats_per_slot(N))) != PM_OK) return st;
   if ((st = p->alloc(&p->counters, 6)) != PM_OK) return st;
   if (!o->init_depth_random) {
     if ((st = p->alloc(&p->init_depth, plane)) != PM_OK) return st;
@@ -487,7 +484,6 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   h.init_random = o->init_depth_random ? 1 : 0;
   h.compat_lo_geo_bug = o->compat_lo_geo_bug ? 1 : 0;
   h.accum_mode = o->accum_mode;
-  h.slots_per_view = slots;
   h.conf_threshold = o->conf_threshold;
   h.depth_diff_threshold = o->depth_diff_threshold;
   h.seed = o->seed;
@@ -505,13 +501,6 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   guard.keep = true;
   *out = p;
   return PM_OK;
-}
-
-PM_API void pm_problem_destroy(pm_problem* p) {
-  if (!p) return;
-  (void)hipSetDevice(p->engine->device);
-  (void)hipStreamSynchronize(p->engine->stream);
-  delete p;
 }
 
 namespace {
@@ -535,78 +524,352 @@ struct EventTimer {
   }
 };
 
-}  // namespace
+// One phase of one view: the iteration schedule of Run()
+// (patch_match_cuda.cu:774-909): init, then per iteration the black and red
+// sweeps and, from iteration 3, the black and red local optimisation; the
+// optional FilterDepthMap pair at the end (:1007-1033).
+struct PhaseRec {
+  pm::PhaseKind kind;
+  int color;
+  uint32_t pass;
+  float th_mc;
+};
 
-PM_API pm_status pm_problem_run(pm_problem* p) {
-  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
-  pm_engine* e = p->engine;
-  PM_HIP(hipSetDevice(e->device));
-  hipStream_t s = e->stream;
-  const int W = p->W, H = p->H;
-  const size_t plane = (size_t)W * H;
-  const pm_options& o = p->opt;
-  pm::LaunchCfg cfg{o.image_storage, o.rcp_mode, o.window_radius, o.window_step};
-  // counters on/off for this run
-  unsigned long long* cnt = e->counting ? p->counters : nullptr;
-  if (cnt != p->host.counters) {
-    p->host.counters = cnt;
-    PM_HIP(hipMemcpyAsync(p->dev, &p->host, sizeof(p->host), hipMemcpyHostToDevice, s));
-  }
-  if (cnt) PM_HIP(hipMemsetAsync(cnt, 0, 6 * sizeof(unsigned long long), s));
-  if (!o.init_depth_random) {
-    PM_HIP(hipMemcpyAsync(p->host.depth, p->init_depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
-    PM_HIP(hipMemcpyAsync(p->host.normal, p->init_normal, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice, s));
-  }
+std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o) {
+  std::vector<std::vector<PhaseRec>> t;
   const uint32_t goff = o.geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
-  EventTimer t;
-  pm_status st;
-  std::vector<std::pair<size_t, int>> sweeps, locals;  // event index pairs
-  if ((st = t.rec(s)) != PM_OK) return st;                      // 0: start
-  PM_HIP(pm::launch_init(p->dev, cfg, W, H, PM_PASS_INIT + goff, s));
-  if ((st = t.rec(s)) != PM_OK) return st;                      // 1: init done
+  // both colours of the init in one tick (init touches only its own pixel)
+  t.push_back({{pm::kPhaseInit, 0, PM_PASS_INIT + goff, 0.0f}, {pm::kPhaseInit, 1, PM_PASS_INIT + goff, 0.0f}});
   for (int iter = 0; iter < o.num_iterations; ++iter) {
     const float th_mc =
         (float)((double)o.init_ncc_matching_cost * std::exp((double)(-(float)(iter * iter) / o.alpha)));
-    for (int color = 0; color < 2; ++color) {
-      const size_t a = t.ev.size();
-      if ((st = t.rec(s)) != PM_OK) return st;
-      PM_HIP(pm::launch_sweep(p->dev, cfg, W, H, color, th_mc, PM_PASS_SWEEP(iter, color) + goff, s));
-      if ((st = t.rec(s)) != PM_OK) return st;
-      sweeps.push_back({a, 0});
+    for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseSweep, c, PM_PASS_SWEEP(iter, c) + goff, th_mc}});
+    if (o.local_optimization && iter > 2)
+      for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseLocal, c, PM_PASS_LOCAL(iter, c) + goff, 0.0f}});
+  }
+  if (o.filter && o.conf_filter)
+    for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseFilter, c, 0u, 0.0f}});
+  return t;
+}
+
+}  // namespace
+
+// All launches of one run over a set of views: at tick t every view runs its
+// phase t, and views whose phase shares the kernel (kind and window shape)
+// share one launch, described by a LaunchTable.  The tables of all launches
+// are built on the host once and uploaded in one copy.
+struct pm_schedule {
+  struct Launch {
+    pm::PhaseKind kind;
+    pm::LaunchCfg cfg;
+    size_t offset;
+    int blocks;
+  };
+  std::vector<uint8_t> tables;
+  std::vector<Launch> launches;
+  void* dev = nullptr;
+  float* ws = nullptr;   // workspace sized for the largest launch
+  size_t ws_floats = 0;
+  ~pm_schedule() {
+    if (dev) (void)hipFree(dev);
+    if (ws) (void)hipFree(ws);
+  }
+};
+
+pm_problem::~pm_problem() {
+  delete schedule;
+  for (void* q : allocs) (void)hipFree(q);
+}
+
+PM_API void pm_problem_destroy(pm_problem* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->engine->device);
+  (void)hipStreamSynchronize(p->engine->stream);
+  delete p;
+}
+
+namespace {
+
+pm::LaunchCfg cfg_of(const pm_options& o) { return pm::LaunchCfg{o.image_storage, o.rcp_mode, o.window_radius, o.window_step}; }
+
+pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch, hipStream_t s) {
+  std::vector<std::vector<std::vector<PhaseRec>>> ticks;
+  size_t T = 0;
+  for (pm_problem* p : views) {
+    ticks.push_back(view_ticks(p->opt));
+    T = std::max(T, ticks.back().size());
+  }
+  sch.tables.clear();
+  sch.launches.clear();
+  for (size_t t = 0; t < T; ++t) {
+    struct Group {
+      pm::PhaseKind kind;
+      pm::LaunchCfg cfg;
+      std::vector<pm::LaunchEntry> e;
+      std::vector<int> blocks;
+      std::vector<int> nmax;   // sources of the entry's view (workspace size)
+    };
+    std::vector<Group> groups;
+    for (size_t vi = 0; vi < views.size(); ++vi) {
+      if (t >= ticks[vi].size()) continue;
+      pm_problem* p = views[vi];
+      for (const PhaseRec& r : ticks[vi][t]) {
+        pm::LaunchCfg cfg = r.kind == pm::kPhaseFilter ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
+        Group* g = nullptr;
+        for (Group& gg : groups)
+          if (gg.kind == r.kind && gg.cfg == cfg) g = &gg;
+        if (!g) {
+          groups.push_back(Group{r.kind, cfg, {}, {}, {}});
+          g = &groups.back();
+        }
+        // a view larger than one launch is split into tile ranges
+        const int vb = pm::view_blocks(p->W, p->H);
+        for (int t0 = 0; t0 < vb; t0 += pm::kMaxBlocksPerLaunch) {
+          pm::LaunchEntry en{};
+          en.v = p->dev;
+          en.tile_offset = t0;
+          en.color = r.color;
+          en.pass = r.pass;
+          en.th_mc = r.th_mc;
+          g->e.push_back(en);
+          g->blocks.push_back(std::min(pm::kMaxBlocksPerLaunch, vb - t0));
+          g->nmax.push_back(p->N);
+        }
+      }
     }
-    if (o.local_optimization && iter > 2) {
-      for (int color = 0; color < 2; ++color) {
-        const size_t a = t.ev.size();
-        if ((st = t.rec(s)) != PM_OK) return st;
-        PM_HIP(pm::launch_local_opt(p->dev, cfg, W, H, color, PM_PASS_LOCAL(iter, color) + goff, s));
-        if ((st = t.rec(s)) != PM_OK) return st;
-        locals.push_back({a, 0});
+    for (Group& g : groups) {
+      // chunks of at most kMaxBlocksPerLaunch blocks, cut at entry boundaries
+      size_t k = 0;
+      while (k < g.e.size()) {
+        int total = 0, nmax = 1;
+        size_t k1 = k;
+        while (k1 < g.e.size() && (k1 == k || total + g.blocks[k1] <= pm::kMaxBlocksPerLaunch)) {
+          g.e[k1].block_start = total;
+          total += g.blocks[k1];
+          nmax = std::max(nmax, g.nmax[k1]);
+          ++k1;
+        }
+        const size_t off = (sch.tables.size() + 15) & ~(size_t)15;
+        const size_t bytes = pm::launch_table_bytes((int)(k1 - k));
+        sch.tables.resize(off + bytes);
+        pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + off);
+        lt->n = (int32_t)(k1 - k);
+        lt->total_blocks = total;
+        lt->ws_slots = total * pm::kSweepThreads;
+        lt->ws = nullptr;  // patched below, after the workspace is sized
+        std::memcpy(lt->e, g.e.data() + k, (k1 - k) * sizeof(pm::LaunchEntry));
+        sch.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, total});
+        const size_t need = (size_t)lt->ws_slots * pm::ws_floats_per_slot(nmax);
+        sch.ws_floats = std::max(sch.ws_floats, need);
+        k = k1;
       }
     }
   }
-  if (o.filter && o.conf_filter) {
-    for (int color = 0; color < 2; ++color) PM_HIP(pm::launch_filter(p->dev, W, H, color, s));
+  if (sch.ws) {
+    PM_HIP(hipFree(sch.ws));
+    sch.ws = nullptr;
   }
-  if ((st = t.rec(s)) != PM_OK) return st;
-  PM_HIP(hipStreamSynchronize(s));
-  std::memset(&p->stats, 0, sizeof(p->stats));
-  p->stats.total_ms = t.between(0, t.ev.size() - 1);
-  p->stats.init_ms = t.between(0, 1);
-  for (auto& pr : sweeps) p->stats.sweep_ms += t.between(pr.first, pr.first + 1);
-  for (auto& pr : locals) p->stats.local_opt_ms += t.between(pr.first, pr.first + 1);
-  p->stats.sweep_launches = (int32_t)sweeps.size();
-  p->stats.local_opt_launches = (int32_t)locals.size();
-  if (cnt) {
-    unsigned long long c[6];
-    PM_HIP(hipMemcpy(c, cnt, sizeof(c), hipMemcpyDeviceToHost));
-    p->stats.ncc_evaluations = c[0];
-    p->stats.ncc_samples = c[1];
-    p->stats.geo_evaluations = c[2];
-    p->stats.sweep_ncc_evaluations = c[3];
-    p->stats.sweep_ncc_samples = c[4];
-    p->stats.sweep_geo_evaluations = c[5];
+  PM_HIP(hipMalloc(&sch.ws, std::max<size_t>(sch.ws_floats, 1) * sizeof(float)));
+  for (const pm_schedule::Launch& l : sch.launches) ((pm::LaunchTable*)(sch.tables.data() + l.offset))->ws = sch.ws;
+  if (sch.dev) {
+    PM_HIP(hipFree(sch.dev));
+    sch.dev = nullptr;
+  }
+  PM_HIP(hipMalloc(&sch.dev, std::max<size_t>(sch.tables.size(), 16)));
+  PM_HIP(hipMemcpyAsync(sch.dev, sch.tables.data(), sch.tables.size(), hipMemcpyHostToDevice, s));
+  return PM_OK;
+}
+
+// Per-view work before the launches: counting on/off and the initial state
+// of views created with init_depth_random = 0.
+pm_status prepare_views(const std::vector<pm_problem*>& views, bool counting, hipStream_t s, bool allow_memset) {
+  for (pm_problem* p : views) {
+    unsigned long long* cnt = counting ? p->counters : nullptr;
+    if (cnt != p->host.counters) {
+      p->host.counters = cnt;
+      PM_HIP(hipMemcpyAsync(p->dev, &p->host, sizeof(p->host), hipMemcpyHostToDevice, s));
+    }
+    if (cnt && allow_memset) PM_HIP(hipMemsetAsync(cnt, 0, 6 * sizeof(unsigned long long), s));
+    if (!p->opt.init_depth_random) {
+      const size_t plane = (size_t)p->W * p->H;
+      PM_HIP(hipMemcpyAsync(p->host.depth, p->init_depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+      PM_HIP(hipMemcpyAsync(p->host.normal, p->init_normal, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
   }
   return PM_OK;
+}
+
+// Issue every launch of the schedule on stream s, bracketing each with
+// events when t is given.
+pm_status launch_all(const pm_schedule& sch, hipStream_t s, EventTimer* t, pm_run_stats* st) {
+  pm_status rs;
+  for (const pm_schedule::Launch& l : sch.launches) {
+    if (t && (rs = t->rec(s)) != PM_OK) return rs;
+    PM_HIP(pm::launch_phase(l.kind, l.cfg, (const pm::LaunchTable*)((const uint8_t*)sch.dev + l.offset), l.blocks, s));
+    if (t && (rs = t->rec(s)) != PM_OK) return rs;
+    if (st) {
+      if (l.kind == pm::kPhaseSweep) st->sweep_launches++;
+      if (l.kind == pm::kPhaseLocal) st->local_opt_launches++;
+    }
+  }
+  return PM_OK;
+}
+
+void collect_times(const pm_schedule& sch, const EventTimer& t, size_t first, pm_run_stats* st) {
+  size_t k = first;
+  for (const pm_schedule::Launch& l : sch.launches) {
+    const double ms = t.between(k, k + 1);
+    if (l.kind == pm::kPhaseSweep) st->sweep_ms += ms;
+    if (l.kind == pm::kPhaseLocal) st->local_opt_ms += ms;
+    if (l.kind == pm::kPhaseInit) st->init_ms += ms;
+    k += 2;
+  }
+}
+
+pm_status read_counters(const std::vector<pm_problem*>& views, pm_run_stats* st) {
+  for (pm_problem* p : views) {
+    if (!p->host.counters) continue;
+    unsigned long long c[6];
+    PM_HIP(hipMemcpy(c, p->counters, sizeof(c), hipMemcpyDeviceToHost));
+    st->ncc_evaluations += c[0];
+    st->ncc_samples += c[1];
+    st->geo_evaluations += c[2];
+    st->sweep_ncc_evaluations += c[3];
+    st->sweep_ncc_samples += c[4];
+    st->sweep_geo_evaluations += c[5];
+  }
+  return PM_OK;
+}
+
+// Eager run of a set of views with per-launch hipEvent timings.
+pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_schedule& sch, pm_run_stats* st) {
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  pm_status rs;
+  if (!sch.dev && (rs = build_schedule(views, sch, s)) != PM_OK) return rs;
+  if ((rs = prepare_views(views, e->counting != 0, s, true)) != PM_OK) return rs;
+  EventTimer t;
+  std::memset(st, 0, sizeof(*st));
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  if ((rs = launch_all(sch, s, &t, st)) != PM_OK) return rs;
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  PM_HIP(hipStreamSynchronize(s));
+  st->total_ms = t.between(0, t.ev.size() - 1);
+  collect_times(sch, t, 1, st);
+  return read_counters(views, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batch
+
+struct pm_batch {
+  pm_engine* engine = nullptr;
+  std::vector<pm_problem*> views;
+  pm_schedule schedule;
+  pm_run_stats stats{};
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~pm_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (pm_problem* p : views) delete p;
+  }
+};
+
+PM_API pm_status pm_batch_create(pm_engine* e, int32_t num_views, const pm_options* options, const pm_view_desc* views,
+                                 pm_batch** out) {
+  if (!e || !options || !views || !out || num_views <= 0) return fail(PM_ERR_INVALID_ARG, "invalid batch arguments");
+  *out = nullptr;
+  pm_batch* b = new (std::nothrow) pm_batch();
+  if (!b) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  b->engine = e;
+  for (int32_t i = 0; i < num_views; ++i) {
+    pm_problem* p = nullptr;
+    pm_status st = pm_problem_create(e, &options[i], &views[i], &p);
+    if (st != PM_OK) {
+      const std::string msg = "view " + std::to_string(i) + ": " + g_last_error;
+      delete b;
+      return fail(st, msg);
+    }
+    b->views.push_back(p);
+  }
+  *out = b;
+  return PM_OK;
+}
+
+PM_API pm_status pm_batch_set_graph(pm_batch* b, int32_t enable) {
+  if (!b) return fail(PM_ERR_INVALID_ARG, "null batch");
+  b->use_graph = enable != 0;
+  return PM_OK;
+}
+
+PM_API pm_status pm_batch_run(pm_batch* b) {
+  if (!b) return fail(PM_ERR_INVALID_ARG, "null batch");
+  pm_engine* e = b->engine;
+  if (!b->use_graph || e->counting) return run_views(e, b->views, b->schedule, &b->stats);
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  pm_status rs;
+  if (!b->exec) {
+    if (!b->schedule.dev && (rs = build_schedule(b->views, b->schedule, s)) != PM_OK) return rs;
+    // counters off before capture (their pointer lives in the view block)
+    if ((rs = prepare_views(b->views, false, s, false)) != PM_OK) return rs;
+    PM_HIP(hipStreamSynchronize(s));
+    PM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rs = prepare_views(b->views, false, s, false);
+    if (rs == PM_OK) rs = launch_all(b->schedule, s, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (rs != PM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rs;
+    }
+    if (ce != hipSuccess) return fail(PM_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    b->graph = g;
+    PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  EventTimer t;
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  PM_HIP(hipGraphLaunch(b->exec, s));
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  PM_HIP(hipStreamSynchronize(s));
+  std::memset(&b->stats, 0, sizeof(b->stats));
+  b->stats.total_ms = t.between(0, 1);
+  for (const pm_schedule::Launch& l : b->schedule.launches) {
+    if (l.kind == pm::kPhaseSweep) b->stats.sweep_launches++;
+    if (l.kind == pm::kPhaseLocal) b->stats.local_opt_launches++;
+  }
+  return PM_OK;
+}
+
+PM_API pm_status pm_batch_get_view(const pm_batch* b, int32_t index, pm_problem** view) {
+  if (!b || !view) return fail(PM_ERR_INVALID_ARG, "null argument");
+  if (index < 0 || index >= (int32_t)b->views.size()) return fail(PM_ERR_INVALID_ARG, "view index out of range");
+  *view = b->views[index];
+  return PM_OK;
+}
+
+PM_API pm_status pm_batch_get_stats(const pm_batch* b, pm_run_stats* out) {
+  if (!b || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
+  *out = b->stats;
+  return PM_OK;
+}
+
+PM_API void pm_batch_destroy(pm_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->engine->device);
+  (void)hipStreamSynchronize(b->engine->stream);
+  delete b;
+}
+
+PM_API pm_status pm_problem_run(pm_problem* p) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  if (!p->schedule) p->schedule = new (std::nothrow) pm_schedule();
+  if (!p->schedule) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  return run_views(p->engine, {p}, *p->schedule, &p->stats);
 }
 
 namespace {

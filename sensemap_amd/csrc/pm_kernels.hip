@@ -79,7 +79,7 @@ struct Pix {
 // Workspace accessors (SoA over slots).
 struct Ws {
   float* base;
-  int slots, slot, N;
+  int slots, slot, N;  // base/slots: the launch's workspace; slot: this thread
   __device__ __attribute__((address_space(1))) float& hyp(int k, int c) const { return G(base)[(size_t)(4 * k + c) * slots + slot]; }
   __device__ __attribute__((address_space(1))) float& M(int k, int i) const {
     return G(base)[(size_t)(4 * (kMaxHyp + kNumRefine) + k * N + i) * slots + slot];
@@ -472,29 +472,61 @@ struct TileSize {
 // Stage the colour table and the ref tile (zero outside the image) into
 // LDS, then map the thread to its pixel.  Returns false for threads whose
 // pixel lies outside the image (after the barrier).
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Give each XCD a contiguous run of tiles (row-major over the tile
-// grid) so the workgroups resident on one XCD cover one compact band of the
-// image and share source footprints in that XCD's L2.
-__device__ __forceinline__ void tile_of_block(int tiles_x, int& tx, int& ty) {
+// The launch's entry for this workgroup.  Workgroups are dispatched
+// round-robin over the 8 XCDs (block b runs on XCD b % 8); the remap gives
+// each XCD a contiguous run of the launch's tiles, so the workgroups resident
+// on one XCD cover one compact band of a view and share source footprints in
+// that XCD's L2.
+struct Ctx {
+  const ViewDev* v;
+  int color;
+  uint32_t pass;
+  float th_mc;
+  int local_block;  // tile index within the view
+  int launch_block; // block index within the launch (after the XCD remap)
+  float* ws;
+  int ws_slots;
+};
+
+__device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt) {
   const int B = gridDim.x;
   const int b = blockIdx.x;
   const int q = B >> 3, r = B & 7;
   const int xcd = b & 7, j = b >> 3;
   const int t = xcd * q + (xcd < r ? xcd : r) + j;
-  ty = t / tiles_x;
-  tx = t - ty * tiles_x;
+  const __attribute__((address_space(1))) LaunchTable* g = G(lt);
+  int lo = 0, hi = g->n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g->e[mid].block_start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  Ctx c;
+  c.v = g->e[lo].v;
+  c.color = g->e[lo].color;
+  c.pass = g->e[lo].pass;
+  c.th_mc = g->e[lo].th_mc;
+  c.local_block = t - g->e[lo].block_start + g->e[lo].tile_offset;
+  c.launch_block = t;
+  c.ws = g->ws;
+  c.ws_slots = g->ws_slots;
+  return c;
 }
 
-__device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int color, int R, uint8_t* tile, float* ct,
-                                           float* spl, Pix& px, int& tw) {
+// Stage the colour table, the spatial weights and the ref tile (zero outside
+// the image) into LDS, then map the thread to its pixel.  Returns false for
+// threads whose pixel lies outside the image (after the barrier).
+__device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, float* ct, float* spl, Pix& px,
+                                           int& tw) {
+  const ViewDev* __restrict__ v = cx.v;
   const int t = threadIdx.x;
   ct[t] = G(v->ct)[t];
   const int nsp = v->nwin * v->nwin;
   for (int e = t; e < nsp; e += kSweepThreads) spl[e] = G(v->sp)[e];
   const int W = v->W, H = v->H;
-  int tx, ty;
-  tile_of_block((W + kTileW - 1) / kTileW, tx, ty);
+  const int tiles_x = (W + kTileW - 1) / kTileW;
+  const int ty = cx.local_block / tiles_x;
+  const int tx = cx.local_block - ty * tiles_x;
   const int row0 = ty * kTileH, col0 = tx * kTileW;
   tw = kTileW + 2 * R;
   const int th = kTileH + 2 * R;
@@ -505,7 +537,7 @@ __device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int co
   __syncthreads();
   px.lrow = t >> 4;
   px.row = row0 + px.lrow;
-  px.lcol = 2 * (t & 15) + ((px.row + color) & 1);
+  px.lcol = 2 * (t & 15) + ((px.row + cx.color) & 1);
   px.col = col0 + px.lcol;
   if (px.row >= H || px.col >= W) return false;
   px.p = px.row * W + px.col;
@@ -521,24 +553,27 @@ __device__ __forceinline__ bool setup_pass(const ViewDev* __restrict__ v, int co
   return true;
 }
 
-__device__ __forceinline__ int slot_index() { return blockIdx.x * kSweepThreads + threadIdx.x; }
+__device__ __forceinline__ int slot_index(const Ctx& cx) { return cx.launch_block * kSweepThreads + threadIdx.x; }
 
 // ---------------------------------------------------------------------------
 // init: random (best of init_samples) or given depth/normal, then InitConf
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restrict__ v, int color, uint32_t pass) {
+__global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __restrict__ lt) {
   __shared__ uint8_t tile[TileSize<R_>::kBytes];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const Ctx cx = resolve(lt);
+  const ViewDev* __restrict__ v = cx.v;
+  const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
   int tw;
-  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int N = v->N;
   const int plane = v->W * v->H;
-  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
   Counters cnt;
   pm_rng g;
   pm_rng_init(&g, v->seed, (uint32_t)v->view_id, (uint32_t)px.p, pass);
@@ -564,7 +599,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
       if (hn[0] * px.np0 + hn[1] * px.np1 + hn[2] * 1.0f >= 0.0f) random_normal(&g, px.np0, px.np1, hn);
     }
     for (int i = 0; i < N; ++i) {
-      ws.M(0, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, hn[0], hn[1], hn[2], hd, v->refK, cnt);
+      ws.M(cx.color, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, hn[0], hn[1], hn[2], hd, v->refK, cnt);
     }
     // thk smallest costs in stable ascending order (= the reference's
     // bubble sort, kernel_functions.cu:1588-1597)
@@ -575,7 +610,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
       float bv = 0.0f;
       for (int i = 0; i < N; ++i) {
         if ((taken >> i) & 1u) continue;
-        const float cv = ws.M(0, i);
+        const float cv = ws.M(cx.color, i);
         if (bi < 0 || cv < bv) {
           bi = i;
           bv = cv;
@@ -592,7 +627,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
       bn[0] = hn[0];
       bn[1] = hn[1];
       bn[2] = hn[2];
-      for (int i = 0; i < N; ++i) ws.RM(0, i) = ws.M(0, i);
+      for (int i = 0; i < N; ++i) ws.RM(cx.color, i) = ws.M(cx.color, i);
     }
   }
   G(v->depth)[px.p] = bd;
@@ -601,7 +636,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
   G(v->normal)[2 * plane + px.p] = bn[2];
   G(v->conf)[px.p] = best_conf;
   G(v->view_sel)[px.p] = best_sel;
-  for (int i = 0; i < N; ++i) G(v->cost)[i * plane + px.p] = ws.RM(0, i);
+  for (int i = 0; i < N; ++i) G(v->cost)[i * plane + px.p] = ws.RM(cx.color, i);
   flush_counters(v, cnt);
 }
 
@@ -617,21 +652,24 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const ViewDev* __restric
 #endif
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const ViewDev* __restrict__ v, int color, float th_mc,
-                                                         uint32_t pass) {
+__global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const LaunchTable* __restrict__ lt) {
   __shared__ uint8_t tile[TileSize<R_>::kBytes];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const Ctx cx = resolve(lt);
+  const ViewDev* __restrict__ v = cx.v;
+  const uint32_t pass = cx.pass;
+  const float th_mc = cx.th_mc;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
   int tw;
-  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
   const float* iK = v->refInvK;
-  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
   Counters cnt;
 
   const float vd[3] = {px.np0, px.np1, 1.0f};
@@ -872,19 +910,22 @@ __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const View
 // local optimisation
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) void k_local_opt(const ViewDev* __restrict__ v, int color, uint32_t pass) {
+__global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* __restrict__ lt) {
   __shared__ uint8_t tile[TileSize<R_>::kBytes];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
+  const Ctx cx = resolve(lt);
+  const ViewDev* __restrict__ v = cx.v;
+  const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
   int tw;
-  if (!setup_pass(v, color, R, tile, ct, spl, px, tw)) return;
+  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
-  Ws ws{v->ws, v->slots_per_view, slot_index(), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
   Counters cnt;
 
   float conf = G(v->conf)[p];
@@ -972,13 +1013,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const ViewDev* __re
 // ---------------------------------------------------------------------------
 // FilterDepthMap (kernel_functions.cu:3282-3354)
 
-__global__ __launch_bounds__(kSweepThreads) void k_filter(const ViewDev* __restrict__ v, int color) {
+__global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __restrict__ lt) {
+  const Ctx cx = resolve(lt);
+  const ViewDev* __restrict__ v = cx.v;
   const int t = threadIdx.x;
   const int W = v->W, H = v->H;
-  int tx, ty;
-  tile_of_block((W + kTileW - 1) / kTileW, tx, ty);
+  const int tiles_x = (W + kTileW - 1) / kTileW;
+  const int ty = cx.local_block / tiles_x;
+  const int tx = cx.local_block - ty * tiles_x;
   const int row = ty * kTileH + (t >> 4);
-  const int col = tx * kTileW + 2 * (t & 15) + ((row + color) & 1);
+  const int col = tx * kTileW + 2 * (t & 15) + ((row + cx.color) & 1);
   if (row >= H || col >= W) return;
   const int plane = W * H;
   const int p = row * W + col;
@@ -1159,17 +1203,8 @@ void* pick_kernel(const LaunchCfg& cfg) {
   return pick_sr<Kn, -1, -1>(cfg.storage, cfg.rcp);
 }
 
-// One-dimensional grid over the tiles; tile_of_block() maps blocks to tiles.
-dim3 sweep_grid(int W, int H) {
-  return dim3((unsigned)(((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH)), 1, 1);
-}
-
 }  // namespace
 
-int sweep_slots(int W, int H) {
-  const dim3 g = sweep_grid(W, H);
-  return (int)g.x * kSweepThreads;
-}
 
 hipError_t launch_build_quads_u8(const uint8_t* img, int W, int H, int stride, uint32_t* out, hipStream_t s) {
   if (W <= 0 || H <= 0 || stride < W) return hipErrorInvalidValue;
@@ -1200,37 +1235,17 @@ hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, fl
   return hipGetLastError();
 }
 
-hipError_t launch_init(const ViewDev* v, const LaunchCfg& cfg, int W, int H, uint32_t pass, hipStream_t s) {
-  if (W <= 0 || H <= 0) return hipErrorInvalidValue;
-  void* fn = pick_kernel<InitK>(cfg);
-  for (int color = 0; color < 2; ++color) {
-    void* args[] = {(void*)&v, (void*)&color, (void*)&pass};
-    hipError_t e = hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
-    if (e != hipSuccess) return e;
+hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int blocks, hipStream_t s) {
+  if (blocks <= 0 || !table) return hipErrorInvalidValue;
+  void* fn = nullptr;
+  switch (kind) {
+    case kPhaseInit: fn = pick_kernel<InitK>(cfg); break;
+    case kPhaseSweep: fn = pick_kernel<SweepK>(cfg); break;
+    case kPhaseLocal: fn = pick_kernel<LocalK>(cfg); break;
+    case kPhaseFilter: fn = (void*)&k_filter; break;
   }
-  return hipGetLastError();
-}
-
-hipError_t launch_sweep(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, float th_mc, uint32_t pass,
-                        hipStream_t s) {
-  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
-  void* fn = pick_kernel<SweepK>(cfg);
-  void* args[] = {(void*)&v, (void*)&color, (void*)&th_mc, (void*)&pass};
-  return hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
-}
-
-hipError_t launch_local_opt(const ViewDev* v, const LaunchCfg& cfg, int W, int H, int color, uint32_t pass,
-                            hipStream_t s) {
-  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
-  void* fn = pick_kernel<LocalK>(cfg);
-  void* args[] = {(void*)&v, (void*)&color, (void*)&pass};
-  return hipLaunchKernel(fn, sweep_grid(W, H), dim3(kSweepThreads), args, 0, s);
-}
-
-hipError_t launch_filter(const ViewDev* v, int W, int H, int color, hipStream_t s) {
-  if (W <= 0 || H <= 0 || (color & ~1)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_filter, sweep_grid(W, H), dim3(kSweepThreads), 0, s, v, color);
-  return hipGetLastError();
+  void* args[] = {(void*)&table};
+  return hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(kSweepThreads), args, 0, s);
 }
 
 }  // namespace pm

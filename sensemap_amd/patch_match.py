@@ -274,3 +274,61 @@ class PatchMatchACMMCuda:
             self.close()
         except Exception:
             pass
+
+
+class _ProblemView:
+    """Read-only accessors on a problem handle owned by a batch."""
+
+    def __init__(self, lib, handle, W, H, N):
+        self.lib, self.handle, self.W, self.H, self.N = lib, handle, W, H, N
+
+    _get = PatchMatchACMMCuda._get
+    GetDepthMap = PatchMatchACMMCuda.GetDepthMap
+    GetNormalMap = PatchMatchACMMCuda.GetNormalMap
+    GetConfMap = PatchMatchACMMCuda.GetConfMap
+    GetCostMap = PatchMatchACMMCuda.GetCostMap
+    GetViewSelMap = PatchMatchACMMCuda.GetViewSelMap
+
+
+class PatchMatchBatch:
+    """Batched form (pm_batch_*): many views of uneven size, source count and
+    window shape advanced together, one launch per phase and kernel shape.
+    items: list of (PatchMatchOptions, Problem)."""
+
+    def __init__(self, items, engine: Optional[Engine] = None, graph: bool = False):
+        self.engine = engine or default_engine()
+        self.lib = self.engine.lib
+        self.descs = [ViewDesc(prob, opt) for opt, prob in items]
+        n = len(items)
+        self._copts = (abi.pm_options * n)(*[opt.to_c() for opt, _ in items])
+        self._cdescs = (abi.pm_view_desc * n)(*[d.desc for d in self.descs])
+        h = C.c_void_p()
+        abi.check(self.lib.pm_batch_create(self.engine.handle, n, self._copts, self._cdescs, C.byref(h)))
+        self.handle = h
+        if graph:
+            abi.check(self.lib.pm_batch_set_graph(self.handle, 1))
+        self.views = []
+        for i, d in enumerate(self.descs):
+            vh = C.c_void_p()
+            abi.check(self.lib.pm_batch_get_view(self.handle, i, C.byref(vh)))
+            self.views.append(_ProblemView(self.lib, vh, d.width, d.height, d.num_src))
+
+    def Run(self) -> bool:
+        abi.check(self.lib.pm_batch_run(self.handle))
+        return True
+
+    def GetStats(self) -> abi.pm_run_stats:
+        s = abi.pm_run_stats()
+        abi.check(self.lib.pm_batch_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.pm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -262,3 +262,68 @@ def to_problem(scene: Scene, with_gt_maps: bool = False):
         prob.depth_maps = [v.depth for v in views]
         prob.normal_maps = [v.normal for v in views]
     return prob
+
+
+@dataclass
+class Orbit:
+    views: List[RenderedView]
+    tris: np.ndarray
+    texture: Texture
+
+
+def orbit_scene(seed: int = 3, num_cams: int = 200, radius: float = 12.0, height: float = 12.0,
+                min_px: float = 640 * 480, max_px: float = 3000 * 2000, grid: int = 128,
+                extent: float = 45.0) -> Orbit:
+    """`num_cams` cameras on a circle of `radius` m at `height` m above a
+    heightfield (grid x grid mesh over [-extent, extent]^2, Gaussian bumps
+    sigma 0.5 m), all looking at the centre.  Each camera has its own size,
+    log-uniform in pixel count between min_px and max_px with aspect 4:3 or
+    3:2, and focal length 0.8 x width."""
+    rng = np.random.default_rng(seed)
+    tris = heightfield_tris(rng, (-extent, extent), (-extent, extent), 0.0, grid=grid,
+                            num_bumps=int(160 * (extent / 26.0) ** 2 / 4), sigma=0.5, amp=0.6)
+    dist = math.hypot(radius, height)
+    tex = Texture(spacing=8.0 * dist / 1000.0, amplitude=40.0, seed=seed * 7919 + 1)
+    views = []
+    for k in range(num_cams):
+        a = 2 * math.pi * k / num_cams
+        c = np.array([radius * math.cos(a), -height, radius * math.sin(a)])
+        R, T = look_at(c, [0.0, 0.0, 0.0], down=(0.0, 1.0, 0.0))
+        px = math.exp(rng.uniform(math.log(min_px), math.log(max_px)))
+        aspect = 4.0 / 3.0 if rng.uniform() < 0.5 else 1.5
+        w = int(round(math.sqrt(px * aspect)))
+        h = int(round(w / aspect))
+        f = 0.8 * w
+        cam = Camera((f, (w - 1) / 2.0, f, (h - 1) / 2.0), R, T, w, h)
+        views.append(render(tris, cam, tex))
+    return Orbit(views, tris, tex)
+
+
+def batch_problems(orbit: Orbit, num_views: int = 64, seed: int = 3, n_min: int = 5, n_max: int = 20,
+                   iterations: int = 6, **opt_overrides):
+    """Reference views drawn from the orbit, each with its N ~ U{n_min..n_max}
+    nearest orbit neighbours as sources and a window of radius 5 (step 2) or
+    7 (step 1) drawn per view.  Returns [(PatchMatchOptions, Problem)]."""
+    from .patch_match import Image, PatchMatchOptions, Problem
+    rng = np.random.default_rng(seed + 1000)
+    M = len(orbit.views)
+    images = [Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in orbit.views]
+    refs = rng.choice(M, size=num_views, replace=False)
+    items = []
+    for r in refs:
+        n = int(rng.integers(n_min, n_max + 1))
+        order = sorted((d for d in range(1, M // 2 + 1)), key=lambda d: d)
+        src = []
+        for d in order:
+            for s in ((r + d) % M, (r - d) % M):
+                if len(src) < n and s != r and s not in src:
+                    src.append(int(s))
+            if len(src) >= n:
+                break
+        wide = rng.uniform() < 0.5
+        dmin, dmax = _depth_range(orbit.views[r].depth)
+        opts = PatchMatchOptions(window_radius=7 if wide else 5, window_step=1 if wide else 2, sigma_spatial=3.0,
+                                 sigma_color=0.2, num_iterations=iterations, geom_consistency=False, thk=2,
+                                 seed=seed, **opt_overrides)
+        items.append((opts, Problem(int(r), src, images, dmin, dmax)))
+    return items
