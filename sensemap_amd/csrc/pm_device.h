@@ -74,39 +74,36 @@ inline int ws_floats_per_slot(int N) {
 }
 
 // One entry of a launch table: one view advanced by one phase.  A launch
-// covers the entries' blocks back to back; block_start is the first block
-// (after the XCD remap) of the entry.
+// covers the entries' tiles back to back; tile_start is the entry's first
+// tile within the launch.
 struct LaunchEntry {
   const ViewDev* v;
-  int32_t block_start;
-  int32_t tile_offset;  // first tile of the view this entry covers
+  int32_t tile_start;
   int32_t color;        // checkerboard colour of the pixels this entry updates
   uint32_t pass;        // random-stream pass id
   float th_mc;          // good-matching threshold of the iteration (sweep)
-  int32_t pad0;
 };
 
-// Every block of a launch owns 256 workspace slots (its threads), so no
-// two resident workgroups ever share workspace lines; the launch boundary
-// makes the workspace coherent across the XCDs' L2 caches.
+// A launch runs `grid` persistent workgroups over `total_tiles` tiles.  Each
+// workgroup owns 256 workspace slots (one per thread) for the whole launch,
+// so no two resident workgroups ever share workspace lines, and the launch
+// boundary makes the workspace coherent across the XCDs' L2 caches.
 struct LaunchTable {
   int32_t n;             // entries
-  int32_t total_blocks;  // blocks of the launch
-  int32_t ws_slots;      // total_blocks * 256
+  int32_t total_tiles;   // tiles of the launch
+  int32_t ws_slots;      // grid * 256
   int32_t pad0;
   float* ws;             // workspace of the launch
+  uint32_t* queue;       // 8 tile counters (one per XCD range), zero at launch start
   LaunchEntry e[1];      // n entries follow (allocated with the table)
 };
-
-constexpr int kMaxBlocksPerLaunch = 16384;
 
 inline size_t launch_table_bytes(int n) {
   return sizeof(LaunchTable) + (size_t)(n > 1 ? n - 1 : 0) * sizeof(LaunchEntry);
 }
 
-// Blocks (256-thread workgroups, one 32x16 tile of one colour each) a view
-// needs per colour pass.
-inline int view_blocks(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
+// 32x16 tiles of a view (one workgroup iteration each per colour pass).
+inline int view_tiles(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
 
 struct LaunchCfg {
   int storage;   // PM_STORAGE_*
@@ -127,9 +124,15 @@ hipError_t launch_build_depth_quads(const float* depth, int W, int H, float4* ou
 hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, float4* refm,
                               uint8_t* grad_ok, hipStream_t s);
 // Table-driven colour-pass launches: `table` is a device copy of a
-// LaunchTable whose total_blocks the caller passes as `blocks`.
+// LaunchTable whose queue counters are zero; `grid` comes from phase_grid
+// for the table's total_tiles (and sized its ws_slots).
 enum PhaseKind { kPhaseInit = 0, kPhaseSweep = 1, kPhaseLocal = 2, kPhaseFilter = 3 };
-hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int blocks,
-                        hipStream_t s);
+// Persistent grid for a launch of `tiles` tiles on the current device: the
+// number of CUs times the workgroups one CU holds (capped by blocks_per_cu
+// when > 0), and never more than the tiles.
+int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_cu);
+hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int grid, hipStream_t s);
+// Zero n uint32 words with a kernel (safe inside stream capture).
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 }  // namespace pm

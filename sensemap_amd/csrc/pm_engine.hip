@@ -562,16 +562,19 @@ struct pm_schedule {
   struct Launch {
     pm::PhaseKind kind;
     pm::LaunchCfg cfg;
-    size_t offset;
-    int blocks;
+    size_t offset;  // of the launch's table in `tables`
+    int tiles;
+    int grid;       // persistent workgroups
   };
   std::vector<uint8_t> tables;
   std::vector<Launch> launches;
-  void* dev = nullptr;
-  float* ws = nullptr;   // workspace sized for the largest launch
+  void* dev = nullptr;         // device copy of `tables`
+  uint32_t* queues = nullptr;  // 8 tile counters per launch
+  float* ws = nullptr;         // workspace sized for the largest launch
   size_t ws_floats = 0;
   ~pm_schedule() {
     if (dev) (void)hipFree(dev);
+    if (queues) (void)hipFree(queues);
     if (ws) (void)hipFree(ws);
   }
 };
@@ -601,13 +604,14 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
   }
   sch.tables.clear();
   sch.launches.clear();
+  sch.ws_floats = 0;
   for (size_t t = 0; t < T; ++t) {
     struct Group {
       pm::PhaseKind kind;
       pm::LaunchCfg cfg;
       std::vector<pm::LaunchEntry> e;
-      std::vector<int> blocks;
-      std::vector<int> nmax;   // sources of the entry's view (workspace size)
+      int tiles = 0;
+      int nmax = 1;  // most sources of a view in the launch (workspace size)
     };
     std::vector<Group> groups;
     for (size_t vi = 0; vi < views.size(); ++vi) {
@@ -619,61 +623,48 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
         for (Group& gg : groups)
           if (gg.kind == r.kind && gg.cfg == cfg) g = &gg;
         if (!g) {
-          groups.push_back(Group{r.kind, cfg, {}, {}, {}});
+          groups.push_back(Group{r.kind, cfg, {}, 0, 1});
           g = &groups.back();
         }
-        // a view larger than one launch is split into tile ranges
-        const int vb = pm::view_blocks(p->W, p->H);
-        for (int t0 = 0; t0 < vb; t0 += pm::kMaxBlocksPerLaunch) {
-          pm::LaunchEntry en{};
-          en.v = p->dev;
-          en.tile_offset = t0;
-          en.color = r.color;
-          en.pass = r.pass;
-          en.th_mc = r.th_mc;
-          g->e.push_back(en);
-          g->blocks.push_back(std::min(pm::kMaxBlocksPerLaunch, vb - t0));
-          g->nmax.push_back(p->N);
-        }
+        pm::LaunchEntry en{};
+        en.v = p->dev;
+        en.tile_start = g->tiles;
+        en.color = r.color;
+        en.pass = r.pass;
+        en.th_mc = r.th_mc;
+        g->e.push_back(en);
+        const long long tiles = (long long)g->tiles + pm::view_tiles(p->W, p->H);
+        if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "too many tiles in one launch");
+        g->tiles = (int)tiles;
+        g->nmax = std::max(g->nmax, p->N);
       }
     }
     for (Group& g : groups) {
-      // chunks of at most kMaxBlocksPerLaunch blocks, cut at entry boundaries
-      size_t k = 0;
-      while (k < g.e.size()) {
-        int total = 0, nmax = 1;
-        size_t k1 = k;
-        while (k1 < g.e.size() && (k1 == k || total + g.blocks[k1] <= pm::kMaxBlocksPerLaunch)) {
-          g.e[k1].block_start = total;
-          total += g.blocks[k1];
-          nmax = std::max(nmax, g.nmax[k1]);
-          ++k1;
-        }
-        const size_t off = (sch.tables.size() + 15) & ~(size_t)15;
-        const size_t bytes = pm::launch_table_bytes((int)(k1 - k));
-        sch.tables.resize(off + bytes);
-        pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + off);
-        lt->n = (int32_t)(k1 - k);
-        lt->total_blocks = total;
-        lt->ws_slots = total * pm::kSweepThreads;
-        lt->ws = nullptr;  // patched below, after the workspace is sized
-        std::memcpy(lt->e, g.e.data() + k, (k1 - k) * sizeof(pm::LaunchEntry));
-        sch.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, total});
-        const size_t need = (size_t)lt->ws_slots * pm::ws_floats_per_slot(nmax);
-        sch.ws_floats = std::max(sch.ws_floats, need);
-        k = k1;
-      }
+      const int grid = pm::phase_grid(g.kind, g.cfg, g.tiles, 0);
+      if (grid <= 0) return fail(PM_ERR_DEVICE, "no launch configuration for a phase kernel");
+      const size_t off = (sch.tables.size() + 15) & ~(size_t)15;
+      sch.tables.resize(off + pm::launch_table_bytes((int)g.e.size()));
+      pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + off);
+      lt->n = (int32_t)g.e.size();
+      lt->total_tiles = g.tiles;
+      lt->ws_slots = grid * pm::kSweepThreads;
+      std::memcpy(lt->e, g.e.data(), g.e.size() * sizeof(pm::LaunchEntry));
+      sch.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, g.tiles, grid});
+      sch.ws_floats = std::max(sch.ws_floats, (size_t)lt->ws_slots * pm::ws_floats_per_slot(g.nmax));
     }
   }
-  if (sch.ws) {
-    PM_HIP(hipFree(sch.ws));
-    sch.ws = nullptr;
-  }
+  if (sch.ws) PM_HIP(hipFree(sch.ws));
+  if (sch.queues) PM_HIP(hipFree(sch.queues));
+  if (sch.dev) PM_HIP(hipFree(sch.dev));
+  sch.ws = nullptr;
+  sch.queues = nullptr;
+  sch.dev = nullptr;
   PM_HIP(hipMalloc(&sch.ws, std::max<size_t>(sch.ws_floats, 1) * sizeof(float)));
-  for (const pm_schedule::Launch& l : sch.launches) ((pm::LaunchTable*)(sch.tables.data() + l.offset))->ws = sch.ws;
-  if (sch.dev) {
-    PM_HIP(hipFree(sch.dev));
-    sch.dev = nullptr;
+  PM_HIP(hipMalloc(&sch.queues, std::max<size_t>(8 * sch.launches.size(), 1) * sizeof(uint32_t)));
+  for (size_t k = 0; k < sch.launches.size(); ++k) {
+    pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + sch.launches[k].offset);
+    lt->ws = sch.ws;
+    lt->queue = sch.queues + 8 * k;
   }
   PM_HIP(hipMalloc(&sch.dev, std::max<size_t>(sch.tables.size(), 16)));
   PM_HIP(hipMemcpyAsync(sch.dev, sch.tables.data(), sch.tables.size(), hipMemcpyHostToDevice, s));
@@ -703,9 +694,10 @@ pm_status prepare_views(const std::vector<pm_problem*>& views, bool counting, hi
 // events when t is given.
 pm_status launch_all(const pm_schedule& sch, hipStream_t s, EventTimer* t, pm_run_stats* st) {
   pm_status rs;
+  PM_HIP(pm::launch_zero_u32(sch.queues, (int)(8 * sch.launches.size()), s));
   for (const pm_schedule::Launch& l : sch.launches) {
     if (t && (rs = t->rec(s)) != PM_OK) return rs;
-    PM_HIP(pm::launch_phase(l.kind, l.cfg, (const pm::LaunchTable*)((const uint8_t*)sch.dev + l.offset), l.blocks, s));
+    PM_HIP(pm::launch_phase(l.kind, l.cfg, (const pm::LaunchTable*)((const uint8_t*)sch.dev + l.offset), l.grid, s));
     if (t && (rs = t->rec(s)) != PM_OK) return rs;
     if (st) {
       if (l.kind == pm::kPhaseSweep) st->sweep_launches++;

@@ -30,6 +30,9 @@
 
 #include <type_traits>
 
+#include <map>
+#include <mutex>
+
 #include "pm_device.h"
 #include "pm_math.h"
 
@@ -472,33 +475,52 @@ struct TileSize {
 // Stage the colour table and the ref tile (zero outside the image) into
 // LDS, then map the thread to its pixel.  Returns false for threads whose
 // pixel lies outside the image (after the barrier).
-// The launch's entry for this workgroup.  Workgroups are dispatched
-// round-robin over the 8 XCDs (block b runs on XCD b % 8); the remap gives
-// each XCD a contiguous run of the launch's tiles, so the workgroups resident
-// on one XCD cover one compact band of a view and share source footprints in
-// that XCD's L2.
+// Persistent colour-pass kernels: a launch runs at most as many workgroups
+// as the chip holds at once, and each workgroup loops claiming 32x16 tiles
+// until the launch's tiles are exhausted.  The launch's tiles are split into
+// 8 contiguous ranges, one per XCD (workgroups are dispatched round-robin
+// over the XCDs, so block b starts on XCD b % 8); a workgroup takes tiles
+// from its own XCD's range first, so the workgroups of one XCD cover compact
+// bands of a view and share source footprints in that XCD's L2, and steals
+// from the other ranges once its own is empty.
+__device__ __forceinline__ int claim_tile(const LaunchTable* __restrict__ lt, int* next) {
+  __syncthreads();  // the previous tile's LDS and *next are no longer read
+  if (threadIdx.x == 0) {
+    const __attribute__((address_space(1))) LaunchTable* g = G(lt);
+    const int T = g->total_tiles;
+    uint32_t* q = g->queue;
+    const int xcd = blockIdx.x & 7;
+    int tile = -1;
+    for (int k = 0; k < 8 && tile < 0; ++k) {
+      const int x = (xcd + k) & 7;
+      const int lo = (int)(((long long)T * x) >> 3), hi = (int)(((long long)T * (x + 1)) >> 3);
+      if (lo >= hi) continue;
+      const uint32_t c = atomicAdd(q + x, 1u);
+      if (c < (uint32_t)(hi - lo)) tile = lo + (int)c;
+    }
+    *next = tile;
+  }
+  __syncthreads();
+  return *next;
+}
+
+// The launch entry (view, colour, pass) a tile belongs to.
 struct Ctx {
   const ViewDev* v;
   int color;
   uint32_t pass;
   float th_mc;
   int local_block;  // tile index within the view
-  int launch_block; // block index within the launch (after the XCD remap)
   float* ws;
   int ws_slots;
 };
 
-__device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt) {
-  const int B = gridDim.x;
-  const int b = blockIdx.x;
-  const int q = B >> 3, r = B & 7;
-  const int xcd = b & 7, j = b >> 3;
-  const int t = xcd * q + (xcd < r ? xcd : r) + j;
+__device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt, int t) {
   const __attribute__((address_space(1))) LaunchTable* g = G(lt);
   int lo = 0, hi = g->n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (g->e[mid].block_start <= t) lo = mid;
+    if (g->e[mid].tile_start <= t) lo = mid;
     else hi = mid - 1;
   }
   Ctx c;
@@ -506,8 +528,7 @@ __device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt) {
   c.color = g->e[lo].color;
   c.pass = g->e[lo].pass;
   c.th_mc = g->e[lo].th_mc;
-  c.local_block = t - g->e[lo].block_start + g->e[lo].tile_offset;
-  c.launch_block = t;
+  c.local_block = t - g->e[lo].tile_start;
   c.ws = g->ws;
   c.ws_slots = g->ws_slots;
   return c;
@@ -553,17 +574,15 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, 
   return true;
 }
 
-__device__ __forceinline__ int slot_index(const Ctx& cx) { return cx.launch_block * kSweepThreads + threadIdx.x; }
+// Workspace slot of this thread: slots belong to the resident workgroup, not
+// to the tile, so the workspace is sized by the grid.
+__device__ __forceinline__ int slot_index() { return blockIdx.x * kSweepThreads + threadIdx.x; }
 
 // ---------------------------------------------------------------------------
 // init: random (best of init_samples) or given depth/normal, then InitConf
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
-  __shared__ float ct[256];
-  __shared__ float spl[SpSize<R_, S_>::kFloats];
-  const Ctx cx = resolve(lt);
+__device__ __forceinline__ void init_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
   const ViewDev* __restrict__ v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
@@ -573,7 +592,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __res
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int N = v->N;
   const int plane = v->W * v->H;
-  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(), N};
   Counters cnt;
   pm_rng g;
   pm_rng_init(&g, v->seed, (uint32_t)v->view_id, (uint32_t)px.p, pass);
@@ -640,6 +659,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __res
   flush_counters(v, cnt);
 }
 
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __restrict__ lt) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next))
+    init_tile<R_, S_, STORE, RCP>(resolve(lt, t), tile, ct, spl);
+}
+
 // ---------------------------------------------------------------------------
 // checkerboard sweep
 
@@ -652,11 +681,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __res
 #endif
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
-  __shared__ float ct[256];
-  __shared__ float spl[SpSize<R_, S_>::kFloats];
-  const Ctx cx = resolve(lt);
+__device__ __forceinline__ void sweep_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
   const ViewDev* __restrict__ v = cx.v;
   const uint32_t pass = cx.pass;
   const float th_mc = cx.th_mc;
@@ -669,7 +694,7 @@ __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const Laun
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
   const float* iK = v->refInvK;
-  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(), N};
   Counters cnt;
 
   const float vd[3] = {px.np0, px.np1, 1.0f};
@@ -906,15 +931,21 @@ __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const Laun
   flush_counters(v, cnt, true);
 }
 
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const LaunchTable* __restrict__ lt) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next))
+    sweep_tile<R_, S_, STORE, RCP>(resolve(lt, t), tile, ct, spl);
+}
+
 // ---------------------------------------------------------------------------
 // local optimisation
 
 template <int R_, int S_, int STORE, int RCP>
-__global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
-  __shared__ float ct[256];
-  __shared__ float spl[SpSize<R_, S_>::kFloats];
-  const Ctx cx = resolve(lt);
+__device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
   const ViewDev* __restrict__ v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
@@ -925,7 +956,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* 
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
-  Ws ws{cx.ws, cx.ws_slots, slot_index(cx), N};
+  Ws ws{cx.ws, cx.ws_slots, slot_index(), N};
   Counters cnt;
 
   float conf = G(v->conf)[p];
@@ -1010,11 +1041,20 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* 
   flush_counters(v, cnt);
 }
 
+template <int R_, int S_, int STORE, int RCP>
+__global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* __restrict__ lt) {
+  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ float ct[256];
+  __shared__ float spl[SpSize<R_, S_>::kFloats];
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next))
+    local_opt_tile<R_, S_, STORE, RCP>(resolve(lt, t), tile, ct, spl);
+}
+
 // ---------------------------------------------------------------------------
 // FilterDepthMap (kernel_functions.cu:3282-3354)
 
-__global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __restrict__ lt) {
-  const Ctx cx = resolve(lt);
+__device__ __forceinline__ void filter_tile(const Ctx& cx) {
   const ViewDev* __restrict__ v = cx.v;
   const int t = threadIdx.x;
   const int W = v->W, H = v->H;
@@ -1056,6 +1096,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __r
     G(v->normal)[2 * plane + p] = 0.0f;
     G(v->conf)[p] = 0.0f;
   }
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __restrict__ lt) {
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) filter_tile(resolve(lt, t));
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,17 +1285,65 @@ hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, fl
   return hipGetLastError();
 }
 
-hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int blocks, hipStream_t s) {
-  if (blocks <= 0 || !table) return hipErrorInvalidValue;
-  void* fn = nullptr;
+namespace {
+
+void* phase_kernel(PhaseKind kind, const LaunchCfg& cfg) {
   switch (kind) {
-    case kPhaseInit: fn = pick_kernel<InitK>(cfg); break;
-    case kPhaseSweep: fn = pick_kernel<SweepK>(cfg); break;
-    case kPhaseLocal: fn = pick_kernel<LocalK>(cfg); break;
-    case kPhaseFilter: fn = (void*)&k_filter; break;
+    case kPhaseInit: return pick_kernel<InitK>(cfg);
+    case kPhaseSweep: return pick_kernel<SweepK>(cfg);
+    case kPhaseLocal: return pick_kernel<LocalK>(cfg);
+    case kPhaseFilter: return (void*)&k_filter;
   }
+  return nullptr;
+}
+
+}  // namespace
+
+int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_cu) {
+  if (tiles <= 0) return 0;
+  void* fn = phase_kernel(kind, cfg);
+  if (!fn) return 0;
+  static std::mutex mu;
+  static std::map<void*, int> occ;
+  static std::map<int, int> cus;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  int per_cu = 0, n_cu = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = occ.find(fn);
+    if (it == occ.end()) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSweepThreads, 0) != hipSuccess || n <= 0) n = 1;
+      it = occ.emplace(fn, n).first;
+    }
+    per_cu = it->second;
+    auto ic = cus.find(dev);
+    if (ic == cus.end()) {
+      int n = 0;
+      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 1;
+      ic = cus.emplace(dev, n).first;
+    }
+    n_cu = ic->second;
+  }
+  if (blocks_per_cu > 0 && blocks_per_cu < per_cu) per_cu = blocks_per_cu;
+  const long long g = (long long)n_cu * per_cu;
+  return (int)(g < tiles ? g : tiles);
+}
+
+hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int grid, hipStream_t s) {
+  if (grid <= 0 || !table) return hipErrorInvalidValue;
+  void* fn = phase_kernel(kind, cfg);
+  if (!fn) return hipErrorInvalidValue;
   void* args[] = {(void*)&table};
-  return hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(kSweepThreads), args, 0, s);
+  return hipLaunchKernel(fn, dim3((unsigned)grid), dim3(kSweepThreads), args, 0, s);
+}
+
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!p) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
 }
 
 }  // namespace pm
