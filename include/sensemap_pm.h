@@ -228,6 +228,54 @@ pm_status pm_batch_get_view(const pm_batch* batch, int32_t index, pm_problem** v
 pm_status pm_batch_get_stats(const pm_batch* batch, pm_run_stats* out);
 void pm_batch_destroy(pm_batch* batch);
 
+/* ---- launch shapes and the autotuner (no counterpart in the reference) ----
+ * Every colour-pass kernel runs persistent workgroups that claim tiles of
+ * 2^tile_lw columns x 512 / 2^tile_lw rows (tile_lw 4, 5, 6: 8, 16 or 32
+ * threads per tile row), with at most blocks_per_cu resident workgroups per
+ * CU (0 = as many as fit).  The shape changes only the speed: every shape
+ * gives the same maps, bit for bit.  An engine keeps one shape per class:
+ * phase kernel, window (radius, step), image storage, reciprocal mode and
+ * source-count bucket (N <= 8, N <= 16, N > 16); classes without one use
+ * tile_lw 5 and blocks_per_cu 0. */
+typedef enum pm_phase {
+  PM_PHASE_INIT = 0,
+  PM_PHASE_SWEEP = 1,
+  PM_PHASE_LOCAL = 2,
+  PM_PHASE_FILTER = 3
+} pm_phase;
+
+typedef struct pm_tuning {
+  int32_t tile_lw;        /* 4..6 */
+  int32_t blocks_per_cu;  /* >= 0 */
+} pm_tuning;
+
+typedef struct pm_tune_report {
+  pm_tuning sweep;           /* chosen shapes */
+  pm_tuning local_opt;
+  double sweep_ms_default;   /* one sweep launch of the view, default shape */
+  double sweep_ms_best;      /* ... chosen shape */
+  double local_opt_ms_default;
+  double local_opt_ms_best;
+  int32_t candidates;        /* shapes timed per kernel */
+  int32_t reserved0;
+} pm_tune_report;
+
+pm_status pm_engine_set_tuning(pm_engine* engine, int32_t phase, const pm_options* options, int32_t num_src,
+                               const pm_tuning* tuning);
+pm_status pm_engine_get_tuning(pm_engine* engine, int32_t phase, const pm_options* options, int32_t num_src,
+                               pm_tuning* out);
+/* Time every shape on this view's sweep and local-optimisation kernels and
+ * keep the fastest in the engine for the view's classes.  Runs the view's
+ * init and a few sweeps, so its maps are undefined until its next run. */
+pm_status pm_problem_autotune(pm_problem* problem, pm_tune_report* report);
+/* Keep the engine's shapes in a text file, one line per class, keyed by the
+ * device architecture name; loading skips lines of other architectures. */
+pm_status pm_engine_save_tuning(pm_engine* engine, const char* path);
+pm_status pm_engine_load_tuning(pm_engine* engine, const char* path);
+/* Before its first run, autotune every class of the batch that the engine
+ * holds no shape for, on the class's largest view. */
+pm_status pm_batch_set_autotune(pm_batch* batch, int32_t enable);
+
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,
                       const pm_view_desc* view, const pm_outputs* outputs);

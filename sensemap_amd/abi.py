@@ -120,6 +120,26 @@ class pm_run_stats(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
+PM_PHASE_INIT, PM_PHASE_SWEEP, PM_PHASE_LOCAL, PM_PHASE_FILTER = 0, 1, 2, 3
+
+
+class pm_tuning(C.Structure):
+    _fields_ = [("tile_lw", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+
+class pm_tune_report(C.Structure):
+    _fields_ = [
+        ("sweep", pm_tuning),
+        ("local_opt", pm_tuning),
+        ("sweep_ms_default", C.c_double),
+        ("sweep_ms_best", C.c_double),
+        ("local_opt_ms_default", C.c_double),
+        ("local_opt_ms_best", C.c_double),
+        ("candidates", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
 _SIGNATURES = {
     "pm_options_init": (C.c_int, [C.POINTER(pm_options)]),
     "pm_options_check": (C.c_int, [C.POINTER(pm_options)]),
@@ -150,6 +170,14 @@ _SIGNATURES = {
     "pm_batch_get_view": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
     "pm_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_run_stats)]),
     "pm_batch_destroy": (None, [C.c_void_p]),
+    "pm_batch_set_autotune": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pm_engine_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(pm_options), C.c_int32,
+                                       C.POINTER(pm_tuning)]),
+    "pm_engine_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(pm_options), C.c_int32,
+                                       C.POINTER(pm_tuning)]),
+    "pm_problem_autotune": (C.c_int, [C.c_void_p, C.POINTER(pm_tune_report)]),
+    "pm_engine_save_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pm_engine_load_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

@@ -13,8 +13,11 @@ namespace pm {
 constexpr int kMaxHyp = 11;          // 1 current + 2 neighbours + 8 directional
 constexpr int kNumRefine = 5;        // current + 4 refinement hypotheses
 constexpr int kQuadPad = 4;          // zero texels around every source image
-constexpr int kTileW = 32;           // sweep tile: 32 columns x 16 rows,
-constexpr int kTileH = 16;           //   256 threads = one colour of the tile
+// A tile is 2^lw columns x 512 / 2^lw rows of one view; its 256 threads
+// cover the pixels of one checkerboard colour (2^(lw-1) threads per row).
+// Shapes: lw = 4 (16x32), 5 (32x16, the default), 6 (64x8).
+constexpr int kTileLog2WMin = 4, kTileLog2WMax = 6, kTileLog2WDefault = 5;
+constexpr int kTilePixels = 512;
 constexpr int kSweepThreads = 256;
 constexpr float kMaxCost = 2.0f;
 constexpr float kVarThreshold = 0.65025f;  // 1e-5 * 255^2
@@ -92,7 +95,7 @@ struct LaunchTable {
   int32_t n;             // entries
   int32_t total_tiles;   // tiles of the launch
   int32_t ws_slots;      // grid * 256
-  int32_t pad0;
+  int32_t tile_lw;       // log2 of the tile width (kTileLog2WMin..kTileLog2WMax)
   float* ws;             // workspace of the launch
   uint32_t* queue;       // 8 tile counters (one per XCD range), zero at launch start
   LaunchEntry e[1];      // n entries follow (allocated with the table)
@@ -102,8 +105,12 @@ inline size_t launch_table_bytes(int n) {
   return sizeof(LaunchTable) + (size_t)(n > 1 ? n - 1 : 0) * sizeof(LaunchEntry);
 }
 
-// 32x16 tiles of a view (one workgroup iteration each per colour pass).
-inline int view_tiles(int W, int H) { return ((W + kTileW - 1) / kTileW) * ((H + kTileH - 1) / kTileH); }
+// Tiles of a view at tile width 2^lw (one workgroup iteration each per
+// colour pass).
+inline int view_tiles(int W, int H, int lw) {
+  const int tw = 1 << lw, th = kTilePixels >> lw;
+  return ((W + tw - 1) / tw) * ((H + th - 1) / th);
+}
 
 struct LaunchCfg {
   int storage;   // PM_STORAGE_*
@@ -131,6 +138,8 @@ enum PhaseKind { kPhaseInit = 0, kPhaseSweep = 1, kPhaseLocal = 2, kPhaseFilter 
 // number of CUs times the workgroups one CU holds (capped by blocks_per_cu
 // when > 0), and never more than the tiles.
 int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_cu);
+// Workgroups of the phase kernel one CU holds at once.
+int phase_occupancy(PhaseKind kind, const LaunchCfg& cfg);
 hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int grid, hipStream_t s);
 // Zero n uint32 words with a kernel (safe inside stream capture).
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -43,10 +45,25 @@ pm_status fail(pm_status st, const std::string& msg) {
 
 }  // namespace
 
+// Launch shape of one phase kernel class: tile width (log2) and the cap on
+// resident workgroups per CU (0 = as many as fit).
+struct TuneKey {
+  int kind, radius, step, storage, rcp, nbucket;
+  bool operator<(const TuneKey& o) const {
+    const int a[6] = {kind, radius, step, storage, rcp, nbucket};
+    const int b[6] = {o.kind, o.radius, o.step, o.storage, o.rcp, o.nbucket};
+    for (int k = 0; k < 6; ++k)
+      if (a[k] != b[k]) return a[k] < b[k];
+    return false;
+  }
+};
+
 struct pm_engine {
   int device = 0;
   hipStream_t stream = nullptr;
   int counting = 0;
+  std::mutex tune_mu;
+  std::map<TuneKey, pm_tuning> tuning;  // absent = default shape
 };
 
 struct pm_problem {
@@ -535,14 +552,18 @@ struct PhaseRec {
   float th_mc;
 };
 
+// Good-matching threshold of iteration `iter` (patch_match_cuda.cu:853).
+float th_mc_of(const pm_options& o, int iter) {
+  return (float)((double)o.init_ncc_matching_cost * std::exp((double)(-(float)(iter * iter) / o.alpha)));
+}
+
 std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o) {
   std::vector<std::vector<PhaseRec>> t;
   const uint32_t goff = o.geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
   // both colours of the init in one tick (init touches only its own pixel)
   t.push_back({{pm::kPhaseInit, 0, PM_PASS_INIT + goff, 0.0f}, {pm::kPhaseInit, 1, PM_PASS_INIT + goff, 0.0f}});
   for (int iter = 0; iter < o.num_iterations; ++iter) {
-    const float th_mc =
-        (float)((double)o.init_ncc_matching_cost * std::exp((double)(-(float)(iter * iter) / o.alpha)));
+    const float th_mc = th_mc_of(o, iter);
     for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseSweep, c, PM_PASS_SWEEP(iter, c) + goff, th_mc}});
     if (o.local_optimization && iter > 2)
       for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseLocal, c, PM_PASS_LOCAL(iter, c) + goff, 0.0f}});
@@ -595,13 +616,27 @@ namespace {
 
 pm::LaunchCfg cfg_of(const pm_options& o) { return pm::LaunchCfg{o.image_storage, o.rcp_mode, o.window_radius, o.window_step}; }
 
-pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch, hipStream_t s) {
-  std::vector<std::vector<std::vector<PhaseRec>>> ticks;
+int nbucket(int N) { return N <= 8 ? 0 : (N <= 16 ? 1 : 2); }
+
+TuneKey tune_key(int kind, const pm_options& o, int N) {
+  return TuneKey{kind, o.window_radius, o.window_step, o.image_storage, o.rcp_mode, nbucket(N)};
+}
+
+pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N) {
+  std::lock_guard<std::mutex> lk(e->tune_mu);
+  auto it = e->tuning.find(tune_key(kind, o, N));
+  if (it != e->tuning.end()) return it->second;
+  return pm_tuning{pm::kTileLog2WDefault, 0};
+}
+
+using Ticks = std::vector<std::vector<std::vector<PhaseRec>>>;  // [view][tick][phase]
+
+// Build the launches of `ticks` over `views`; `force` overrides the shapes
+// (autotuning).
+pm_status build_schedule(const std::vector<pm_problem*>& views, const Ticks& ticks, pm_schedule& sch, hipStream_t s,
+                         const pm_tuning* force = nullptr) {
   size_t T = 0;
-  for (pm_problem* p : views) {
-    ticks.push_back(view_ticks(p->opt));
-    T = std::max(T, ticks.back().size());
-  }
+  for (const auto& tv : ticks) T = std::max(T, tv.size());
   sch.tables.clear();
   sch.launches.clear();
   sch.ws_floats = 0;
@@ -609,6 +644,7 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
     struct Group {
       pm::PhaseKind kind;
       pm::LaunchCfg cfg;
+      pm_tuning shape;
       std::vector<pm::LaunchEntry> e;
       int tiles = 0;
       int nmax = 1;  // most sources of a view in the launch (workspace size)
@@ -619,11 +655,14 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
       pm_problem* p = views[vi];
       for (const PhaseRec& r : ticks[vi][t]) {
         pm::LaunchCfg cfg = r.kind == pm::kPhaseFilter ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
+        const pm_tuning shape = force ? *force : shape_of(p->engine, r.kind, p->opt, p->N);
         Group* g = nullptr;
         for (Group& gg : groups)
-          if (gg.kind == r.kind && gg.cfg == cfg) g = &gg;
+          if (gg.kind == r.kind && gg.cfg == cfg && gg.shape.tile_lw == shape.tile_lw &&
+              gg.shape.blocks_per_cu == shape.blocks_per_cu)
+            g = &gg;
         if (!g) {
-          groups.push_back(Group{r.kind, cfg, {}, 0, 1});
+          groups.push_back(Group{r.kind, cfg, shape, {}, 0, 1});
           g = &groups.back();
         }
         pm::LaunchEntry en{};
@@ -633,14 +672,14 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
         en.pass = r.pass;
         en.th_mc = r.th_mc;
         g->e.push_back(en);
-        const long long tiles = (long long)g->tiles + pm::view_tiles(p->W, p->H);
+        const long long tiles = (long long)g->tiles + pm::view_tiles(p->W, p->H, g->shape.tile_lw);
         if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "too many tiles in one launch");
         g->tiles = (int)tiles;
         g->nmax = std::max(g->nmax, p->N);
       }
     }
     for (Group& g : groups) {
-      const int grid = pm::phase_grid(g.kind, g.cfg, g.tiles, 0);
+      const int grid = pm::phase_grid(g.kind, g.cfg, g.tiles, g.shape.blocks_per_cu);
       if (grid <= 0) return fail(PM_ERR_DEVICE, "no launch configuration for a phase kernel");
       const size_t off = (sch.tables.size() + 15) & ~(size_t)15;
       sch.tables.resize(off + pm::launch_table_bytes((int)g.e.size()));
@@ -648,6 +687,7 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
       lt->n = (int32_t)g.e.size();
       lt->total_tiles = g.tiles;
       lt->ws_slots = grid * pm::kSweepThreads;
+      lt->tile_lw = g.shape.tile_lw;
       std::memcpy(lt->e, g.e.data(), g.e.size() * sizeof(pm::LaunchEntry));
       sch.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, g.tiles, grid});
       sch.ws_floats = std::max(sch.ws_floats, (size_t)lt->ws_slots * pm::ws_floats_per_slot(g.nmax));
@@ -669,6 +709,12 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, pm_schedule& sch
   PM_HIP(hipMalloc(&sch.dev, std::max<size_t>(sch.tables.size(), 16)));
   PM_HIP(hipMemcpyAsync(sch.dev, sch.tables.data(), sch.tables.size(), hipMemcpyHostToDevice, s));
   return PM_OK;
+}
+
+Ticks all_ticks(const std::vector<pm_problem*>& views) {
+  Ticks t;
+  for (pm_problem* p : views) t.push_back(view_ticks(p->opt));
+  return t;
 }
 
 // Per-view work before the launches: counting on/off and the initial state
@@ -738,7 +784,7 @@ pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_sche
   PM_HIP(hipSetDevice(e->device));
   hipStream_t s = e->stream;
   pm_status rs;
-  if (!sch.dev && (rs = build_schedule(views, sch, s)) != PM_OK) return rs;
+  if (!sch.dev && (rs = build_schedule(views, all_ticks(views), sch, s)) != PM_OK) return rs;
   if ((rs = prepare_views(views, e->counting != 0, s, true)) != PM_OK) return rs;
   EventTimer t;
   std::memset(st, 0, sizeof(*st));
@@ -754,6 +800,214 @@ pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_sche
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// launch shapes and the autotuner
+
+namespace {
+
+pm_status check_phase(int32_t phase) {
+  if (phase < PM_PHASE_INIT || phase > PM_PHASE_FILTER) return fail(PM_ERR_INVALID_ARG, "unknown phase");
+  return PM_OK;
+}
+
+pm_status check_tuning(const pm_tuning* t) {
+  if (!t) return fail(PM_ERR_INVALID_ARG, "null tuning");
+  if (t->tile_lw < pm::kTileLog2WMin || t->tile_lw > pm::kTileLog2WMax)
+    return fail(PM_ERR_INVALID_ARG, "tile_lw must be 4, 5 or 6");
+  if (t->blocks_per_cu < 0 || t->blocks_per_cu > 64) return fail(PM_ERR_INVALID_ARG, "blocks_per_cu must be 0..64");
+  return PM_OK;
+}
+
+// Device time of one run of a schedule (queue reset + its launches).
+pm_status time_schedule(const pm_schedule& sch, hipStream_t s, double* ms) {
+  EventTimer t;
+  pm_status rs;
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  if ((rs = launch_all(sch, s, nullptr, nullptr)) != PM_OK) return rs;
+  if ((rs = t.rec(s)) != PM_OK) return rs;
+  PM_HIP(hipStreamSynchronize(s));
+  *ms = t.between(0, 1);
+  return PM_OK;
+}
+
+std::vector<pm_tuning> tune_candidates(pm::PhaseKind kind, const pm_options& o) {
+  const int occ = pm::phase_occupancy(kind, cfg_of(o));
+  std::vector<pm_tuning> c;
+  for (int lw = pm::kTileLog2WMin; lw <= pm::kTileLog2WMax; ++lw) {
+    c.push_back(pm_tuning{lw, 0});
+    for (int b = occ - 1; b >= 1 && b >= occ - 2; --b) c.push_back(pm_tuning{lw, b});
+  }
+  return c;
+}
+
+// Time every candidate shape on one phase of view p (two rounds, the
+// minimum counts) and return the fastest.
+pm_status tune_phase(pm_problem* p, const PhaseRec& r, pm_tuning* best, double* ms_default, double* ms_best,
+                     int32_t* ncand) {
+  hipStream_t s = p->engine->stream;
+  const std::vector<pm_tuning> cands = tune_candidates(r.kind, p->opt);
+  std::vector<double> t(cands.size(), 1e30);
+  const std::vector<pm_problem*> one{p};
+  Ticks tk(1);
+  tk[0].push_back({r});
+  pm_status rs;
+  for (int round = 0; round < 2; ++round) {
+    for (size_t k = 0; k < cands.size(); ++k) {
+      pm_schedule sch;
+      if ((rs = build_schedule(one, tk, sch, s, &cands[k])) != PM_OK) return rs;
+      double ms = 0.0;
+      if ((rs = time_schedule(sch, s, &ms)) != PM_OK) return rs;
+      t[k] = std::min(t[k], ms);
+    }
+  }
+  size_t kb = 0;
+  *ms_default = 0.0;
+  for (size_t k = 0; k < cands.size(); ++k) {
+    if (t[k] < t[kb]) kb = k;
+    if (cands[k].tile_lw == pm::kTileLog2WDefault && cands[k].blocks_per_cu == 0) *ms_default = t[k];
+  }
+  *best = cands[kb];
+  *ms_best = t[kb];
+  *ncand = (int32_t)cands.size();
+  return PM_OK;
+}
+
+pm_status autotune(pm_problem* p, pm_tune_report* rep) {
+  pm_engine* e = p->engine;
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t s = e->stream;
+  const pm_options& o = p->opt;
+  const uint32_t goff = o.geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
+  pm_status rs;
+  // a realistic state: init and two iterations of sweeps, default shapes
+  {
+    Ticks warm(1);
+    warm[0] = {{{pm::kPhaseInit, 0, PM_PASS_INIT + goff, 0.0f}, {pm::kPhaseInit, 1, PM_PASS_INIT + goff, 0.0f}}};
+    for (int it = 0; it < 2; ++it)
+      for (int c = 0; c < 2; ++c) warm[0].push_back({{pm::kPhaseSweep, c, PM_PASS_SWEEP(it, c) + goff, th_mc_of(o, it)}});
+    const std::vector<pm_problem*> one{p};
+    if ((rs = prepare_views(one, false, s, true)) != PM_OK) return rs;
+    pm_schedule sch;
+    if ((rs = build_schedule(one, warm, sch, s, nullptr)) != PM_OK) return rs;
+    if ((rs = launch_all(sch, s, nullptr, nullptr)) != PM_OK) return rs;
+    PM_HIP(hipStreamSynchronize(s));
+  }
+  pm_tune_report r{};
+  const PhaseRec sw{pm::kPhaseSweep, 0, PM_PASS_SWEEP(2, 0) + goff, th_mc_of(o, 2)};
+  if ((rs = tune_phase(p, sw, &r.sweep, &r.sweep_ms_default, &r.sweep_ms_best, &r.candidates)) != PM_OK) return rs;
+  r.local_opt = pm_tuning{pm::kTileLog2WDefault, 0};
+  if (o.local_optimization) {
+    const PhaseRec lo{pm::kPhaseLocal, 0, PM_PASS_LOCAL(3, 0) + goff, 0.0f};
+    int32_t nc = 0;
+    if ((rs = tune_phase(p, lo, &r.local_opt, &r.local_opt_ms_default, &r.local_opt_ms_best, &nc)) != PM_OK) return rs;
+  }
+  {
+    std::lock_guard<std::mutex> lk(e->tune_mu);
+    e->tuning[tune_key(pm::kPhaseSweep, o, p->N)] = r.sweep;
+    if (o.local_optimization) e->tuning[tune_key(pm::kPhaseLocal, o, p->N)] = r.local_opt;
+  }
+  if (rep) *rep = r;
+  return PM_OK;
+}
+
+// Autotune, on its largest view, every sweep class of `views` the engine
+// holds no shape for.
+pm_status tune_batch(const std::vector<pm_problem*>& views) {
+  std::map<TuneKey, pm_problem*> largest;
+  for (pm_problem* p : views) {
+    const TuneKey k = tune_key(pm::kPhaseSweep, p->opt, p->N);
+    {
+      std::lock_guard<std::mutex> lk(p->engine->tune_mu);
+      if (p->engine->tuning.count(k)) continue;
+    }
+    auto it = largest.find(k);
+    if (it == largest.end() || (long long)p->W * p->H > (long long)it->second->W * it->second->H) largest[k] = p;
+  }
+  for (auto& kv : largest) {
+    pm_status rs = autotune(kv.second, nullptr);
+    if (rs != PM_OK) return rs;
+  }
+  return PM_OK;
+}
+
+std::string device_arch(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+  std::string a = prop.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+}  // namespace
+
+PM_API pm_status pm_engine_set_tuning(pm_engine* e, int32_t phase, const pm_options* o, int32_t num_src,
+                                      const pm_tuning* t) {
+  if (!e || !o) return fail(PM_ERR_INVALID_ARG, "null argument");
+  pm_status rs;
+  if ((rs = check_phase(phase)) != PM_OK || (rs = check_tuning(t)) != PM_OK) return rs;
+  std::lock_guard<std::mutex> lk(e->tune_mu);
+  e->tuning[tune_key(phase, *o, num_src)] = *t;
+  return PM_OK;
+}
+
+PM_API pm_status pm_engine_get_tuning(pm_engine* e, int32_t phase, const pm_options* o, int32_t num_src,
+                                      pm_tuning* out) {
+  if (!e || !o || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
+  pm_status rs;
+  if ((rs = check_phase(phase)) != PM_OK) return rs;
+  *out = shape_of(e, phase, *o, num_src);
+  return PM_OK;
+}
+
+PM_API pm_status pm_problem_autotune(pm_problem* p, pm_tune_report* report) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  return autotune(p, report);
+}
+
+PM_API pm_status pm_engine_save_tuning(pm_engine* e, const char* path) {
+  if (!e || !path) return fail(PM_ERR_INVALID_ARG, "null argument");
+  const std::string arch = device_arch(e->device);
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(PM_ERR_INVALID_ARG, std::string("cannot write ") + path);
+  std::lock_guard<std::mutex> lk(e->tune_mu);
+  std::fprintf(f, "# arch phase radius step storage rcp nbucket tile_lw blocks_per_cu\n");
+  for (const auto& kv : e->tuning) {
+    const TuneKey& k = kv.first;
+    std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", arch.c_str(), k.kind, k.radius, k.step, k.storage, k.rcp,
+                 k.nbucket, kv.second.tile_lw, kv.second.blocks_per_cu);
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? PM_OK : fail(PM_ERR_INVALID_ARG, std::string("cannot write ") + path);
+}
+
+PM_API pm_status pm_engine_load_tuning(pm_engine* e, const char* path) {
+  if (!e || !path) return fail(PM_ERR_INVALID_ARG, "null argument");
+  const std::string arch = device_arch(e->device);
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(PM_ERR_INVALID_ARG, std::string("cannot read ") + path);
+  char line[256];
+  int bad = 0;
+  std::map<TuneKey, pm_tuning> got;
+  while (std::fgets(line, sizeof(line), f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char a[64];
+    TuneKey k{};
+    pm_tuning t{};
+    if (std::sscanf(line, "%63s %d %d %d %d %d %d %d %d", a, &k.kind, &k.radius, &k.step, &k.storage, &k.rcp,
+                    &k.nbucket, &t.tile_lw, &t.blocks_per_cu) != 9 ||
+        check_phase(k.kind) != PM_OK || check_tuning(&t) != PM_OK) {
+      bad++;
+      continue;
+    }
+    if (arch == a) got[k] = t;
+  }
+  std::fclose(f);
+  if (bad) return fail(PM_ERR_INVALID_ARG, std::string("malformed lines in ") + path);
+  std::lock_guard<std::mutex> lk(e->tune_mu);
+  for (auto& kv : got) e->tuning[kv.first] = kv.second;
+  return PM_OK;
+}
+
+// ---------------------------------------------------------------------------
 // batch
 
 struct pm_batch {
@@ -762,6 +1016,8 @@ struct pm_batch {
   pm_schedule schedule;
   pm_run_stats stats{};
   bool use_graph = false;
+  bool autotune = false;
+  bool tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~pm_batch() {
@@ -798,15 +1054,35 @@ PM_API pm_status pm_batch_set_graph(pm_batch* b, int32_t enable) {
   return PM_OK;
 }
 
+PM_API pm_status pm_batch_set_autotune(pm_batch* b, int32_t enable) {
+  if (!b) return fail(PM_ERR_INVALID_ARG, "null batch");
+  b->autotune = enable != 0;
+  return PM_OK;
+}
+
 PM_API pm_status pm_batch_run(pm_batch* b) {
   if (!b) return fail(PM_ERR_INVALID_ARG, "null batch");
   pm_engine* e = b->engine;
+  if (b->autotune && !b->tuned) {
+    pm_status rs = tune_batch(b->views);
+    if (rs != PM_OK) return rs;
+    b->tuned = true;
+    // shapes may have changed: rebuild the launches and any captured graph
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (b->schedule.dev) {
+      PM_HIP(hipFree(b->schedule.dev));
+      b->schedule.dev = nullptr;
+    }
+  }
   if (!b->use_graph || e->counting) return run_views(e, b->views, b->schedule, &b->stats);
   PM_HIP(hipSetDevice(e->device));
   hipStream_t s = e->stream;
   pm_status rs;
   if (!b->exec) {
-    if (!b->schedule.dev && (rs = build_schedule(b->views, b->schedule, s)) != PM_OK) return rs;
+    if (!b->schedule.dev && (rs = build_schedule(b->views, all_ticks(b->views), b->schedule, s)) != PM_OK) return rs;
     // counters off before capture (their pointer lives in the view block)
     if ((rs = prepare_views(b->views, false, s, false)) != PM_OK) return rs;
     PM_HIP(hipStreamSynchronize(s));

@@ -468,8 +468,11 @@ struct SpSize {
 
 template <int R_>
 struct TileSize {
-  static constexpr int kBytes = R_ >= 0 ? (kTileH + 2 * R_) * (kTileW + 2 * R_)
-                                        : (kTileH + 2 * PM_MAX_WINDOW_RADIUS) * (kTileW + 2 * PM_MAX_WINDOW_RADIUS);
+  // largest (rows + 2R) x (cols + 2R) over the tile shapes
+  static constexpr int kR = R_ >= 0 ? R_ : PM_MAX_WINDOW_RADIUS;
+  static constexpr int shape(int lw) { return ((kTilePixels >> lw) + 2 * kR) * ((1 << lw) + 2 * kR); }
+  static constexpr int kBytes = shape(4) > shape(5) ? (shape(4) > shape(6) ? shape(4) : shape(6))
+                                                    : (shape(5) > shape(6) ? shape(5) : shape(6));
 };
 
 // Stage the colour table and the ref tile (zero outside the image) into
@@ -511,6 +514,7 @@ struct Ctx {
   uint32_t pass;
   float th_mc;
   int local_block;  // tile index within the view
+  int lw;           // log2 of the tile width
   float* ws;
   int ws_slots;
 };
@@ -529,6 +533,7 @@ __device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt, int t
   c.pass = g->e[lo].pass;
   c.th_mc = g->e[lo].th_mc;
   c.local_block = t - g->e[lo].tile_start;
+  c.lw = g->tile_lw;
   c.ws = g->ws;
   c.ws_slots = g->ws_slots;
   return c;
@@ -545,20 +550,21 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, 
   const int nsp = v->nwin * v->nwin;
   for (int e = t; e < nsp; e += kSweepThreads) spl[e] = G(v->sp)[e];
   const int W = v->W, H = v->H;
-  const int tiles_x = (W + kTileW - 1) / kTileW;
+  const int lw = cx.lw;
+  const int tiles_x = (W + (1 << lw) - 1) >> lw;
   const int ty = cx.local_block / tiles_x;
   const int tx = cx.local_block - ty * tiles_x;
-  const int row0 = ty * kTileH, col0 = tx * kTileW;
-  tw = kTileW + 2 * R;
-  const int th = kTileH + 2 * R;
+  const int row0 = ty * (kTilePixels >> lw), col0 = tx << lw;
+  tw = (1 << lw) + 2 * R;
+  const int th = (kTilePixels >> lw) + 2 * R;
   for (int e = t; e < tw * th; e += kSweepThreads) {
     const int rr = row0 - R + e / tw, cc = col0 - R + e % tw;
     tile[e] = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? (uint8_t)0 : G(v->ref)[rr * W + cc];
   }
   __syncthreads();
-  px.lrow = t >> 4;
+  px.lrow = t >> (lw - 1);
   px.row = row0 + px.lrow;
-  px.lcol = 2 * (t & 15) + ((px.row + cx.color) & 1);
+  px.lcol = 2 * (t & ((1 << (lw - 1)) - 1)) + ((px.row + cx.color) & 1);
   px.col = col0 + px.lcol;
   if (px.row >= H || px.col >= W) return false;
   px.p = px.row * W + px.col;
@@ -1058,11 +1064,12 @@ __device__ __forceinline__ void filter_tile(const Ctx& cx) {
   const ViewDev* __restrict__ v = cx.v;
   const int t = threadIdx.x;
   const int W = v->W, H = v->H;
-  const int tiles_x = (W + kTileW - 1) / kTileW;
+  const int lw = cx.lw;
+  const int tiles_x = (W + (1 << lw) - 1) >> lw;
   const int ty = cx.local_block / tiles_x;
   const int tx = cx.local_block - ty * tiles_x;
-  const int row = ty * kTileH + (t >> 4);
-  const int col = tx * kTileW + 2 * (t & 15) + ((row + cx.color) & 1);
+  const int row = ty * (kTilePixels >> lw) + (t >> (lw - 1));
+  const int col = (tx << lw) + 2 * (t & ((1 << (lw - 1)) - 1)) + ((row + cx.color) & 1);
   if (row >= H || col >= W) return;
   const int plane = W * H;
   const int p = row * W + col;
@@ -1299,25 +1306,33 @@ void* phase_kernel(PhaseKind kind, const LaunchCfg& cfg) {
 
 }  // namespace
 
+namespace {
+std::mutex g_occ_mu;
+std::map<void*, int> g_occ;
+std::map<int, int> g_cus;
+}  // namespace
+
+int phase_occupancy(PhaseKind kind, const LaunchCfg& cfg) {
+  void* fn = phase_kernel(kind, cfg);
+  if (!fn) return 1;
+  std::lock_guard<std::mutex> lk(g_occ_mu);
+  auto it = g_occ.find(fn);
+  if (it == g_occ.end()) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSweepThreads, 0) != hipSuccess || n <= 0) n = 1;
+    it = g_occ.emplace(fn, n).first;
+  }
+  return it->second;
+}
+
 int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_cu) {
   if (tiles <= 0) return 0;
-  void* fn = phase_kernel(kind, cfg);
-  if (!fn) return 0;
-  static std::mutex mu;
-  static std::map<void*, int> occ;
-  static std::map<int, int> cus;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
-  int per_cu = 0, n_cu = 0;
+  int per_cu = phase_occupancy(kind, cfg), n_cu = 0;
   {
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = occ.find(fn);
-    if (it == occ.end()) {
-      int n = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSweepThreads, 0) != hipSuccess || n <= 0) n = 1;
-      it = occ.emplace(fn, n).first;
-    }
-    per_cu = it->second;
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    std::map<int, int>& cus = g_cus;
     auto ic = cus.find(dev);
     if (ic == cus.end()) {
       int n = 0;

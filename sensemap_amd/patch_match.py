@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field, fields
 from typing import List, Optional, Sequence
 
@@ -180,6 +181,25 @@ class Engine:
     def set_counting(self, enable: bool) -> None:
         abi.check(self.lib.pm_engine_set_counting(self.handle, int(enable)))
 
+    def set_tuning(self, phase: int, options: PatchMatchOptions, num_src: int, tile_lw: int,
+                   blocks_per_cu: int = 0) -> None:
+        """Launch shape of one kernel class (pm_engine_set_tuning)."""
+        t = abi.pm_tuning(int(tile_lw), int(blocks_per_cu))
+        o = options.to_c()
+        abi.check(self.lib.pm_engine_set_tuning(self.handle, int(phase), C.byref(o), int(num_src), C.byref(t)))
+
+    def get_tuning(self, phase: int, options: PatchMatchOptions, num_src: int):
+        t = abi.pm_tuning()
+        o = options.to_c()
+        abi.check(self.lib.pm_engine_get_tuning(self.handle, int(phase), C.byref(o), int(num_src), C.byref(t)))
+        return t.tile_lw, t.blocks_per_cu
+
+    def save_tuning(self, path: str) -> None:
+        abi.check(self.lib.pm_engine_save_tuning(self.handle, os.fsencode(path)))
+
+    def load_tuning(self, path: str) -> None:
+        abi.check(self.lib.pm_engine_load_tuning(self.handle, os.fsencode(path)))
+
     def close(self) -> None:
         if self.handle:
             self.lib.pm_engine_destroy(self.handle)
@@ -264,6 +284,13 @@ class PatchMatchACMMCuda:
         """Source `src_index`'s depth map from another problem's current depth map (on device)."""
         abi.check(self.lib.pm_problem_set_src_depth(self.handle, int(src_index), other.handle))
 
+    def Autotune(self) -> abi.pm_tune_report:
+        """Time every launch shape on this view and keep the fastest in the
+        engine (pm_problem_autotune); the maps are undefined until the next Run()."""
+        r = abi.pm_tune_report()
+        abi.check(self.lib.pm_problem_autotune(self.handle, C.byref(r)))
+        return r
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.pm_problem_destroy(self.handle)
@@ -295,7 +322,7 @@ class PatchMatchBatch:
     window shape advanced together, one launch per phase and kernel shape.
     items: list of (PatchMatchOptions, Problem)."""
 
-    def __init__(self, items, engine: Optional[Engine] = None, graph: bool = False):
+    def __init__(self, items, engine: Optional[Engine] = None, graph: bool = False, autotune: bool = False):
         self.engine = engine or default_engine()
         self.lib = self.engine.lib
         self.descs = [ViewDesc(prob, opt) for opt, prob in items]
@@ -307,6 +334,8 @@ class PatchMatchBatch:
         self.handle = h
         if graph:
             abi.check(self.lib.pm_batch_set_graph(self.handle, 1))
+        if autotune:
+            abi.check(self.lib.pm_batch_set_autotune(self.handle, 1))
         self.views = []
         for i, d in enumerate(self.descs):
             vh = C.c_void_p()

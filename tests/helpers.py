@@ -21,7 +21,7 @@ def small_heightfield(seed=2, width=192, height=108, f=150.0, num_src=5):
 
 
 def c1_options(**kw) -> pmm.PatchMatchOptions:
-    """BASELINE config 0 matching parameters (11x11 step 2, sigma_s 3, sigma_c 0.2, 6 iterations)."""
+    """Matching parameters of the 640x480 three-plane workload (11x11 step 2, sigma_s 3, sigma_c 0.2, 6 iterations)."""
     base = dict(window_radius=5, window_step=2, sigma_spatial=3.0, sigma_color=0.2, num_iterations=6,
                 geom_consistency=False, thk=2, seed=1)
     base.update(kw)
