@@ -104,7 +104,10 @@ typedef struct pm_image {
   int32_t width;
   int32_t height;
   int32_t stride;  /* bytes between rows, >= width */
-  int32_t reserved0;
+  /* > 0: a serving loop may keep this image on the device under this key
+   * (images given the same key must be identical); <= 0: never cached.
+   * Ignored outside pm_server_*. */
+  int32_t cache_id;
 } pm_image;
 
 /* One reference view with its source views: the fields of Problem
@@ -275,6 +278,63 @@ pm_status pm_engine_load_tuning(pm_engine* engine, const char* path);
 /* Before its first run, autotune every class of the batch that the engine
  * holds no shape for, on the class's largest view. */
 pm_status pm_batch_set_autotune(pm_batch* batch, int32_t enable);
+
+/* ---- serving loop (no counterpart in the reference) ----
+ * A host thread owns the device side of a stream of depth-map jobs.  Jobs
+ * are admitted first-come first-served while the device memory they need
+ * fits in the HBM budget and fewer than max_active_views jobs are on the
+ * device; source images (pm_image.cache_id > 0) stay on the device in an
+ * LRU cache shared by all jobs and are evicted only when no job uses them.
+ * Every step advances each admitted job by one phase of its Run() schedule,
+ * and the phases of all jobs that share a kernel run in one launch
+ * (continuous batching: jobs join and leave between steps).  A job's maps
+ * do not depend on the other jobs: they equal, bit for bit, the maps of the
+ * same job run alone with pm_problem_run on an engine with the same launch
+ * shapes.  The server uses the engine's launch shapes (pm_engine_set_tuning,
+ * pm_problem_autotune, pm_engine_load_tuning) and never tunes by itself.
+ * The reference instead runs up to 3 engines per GPU from host threads, one
+ * view each (patch_match_controller.cc:964-995, mvs/utils.h:33). */
+typedef struct pm_server pm_server;
+
+typedef struct pm_server_config {
+  uint64_t hbm_budget_bytes;  /* device memory of the server: jobs, cache, workspace */
+  int32_t max_active_views;   /* jobs on the device at once (>= 1) */
+  int32_t reserved0;
+} pm_server_config;
+
+typedef struct pm_job_stats {
+  double queue_ms;    /* submit -> admitted */
+  double latency_ms;  /* submit -> outputs written */
+  int32_t status;     /* the job's pm_status */
+  int32_t reserved0;
+} pm_job_stats;
+
+typedef struct pm_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t cache_hits, cache_misses, cache_evictions;  /* per source image of a job */
+  uint64_t steps, launches;
+  uint64_t arena_capacity, arena_used, cache_bytes;    /* bytes */
+  int32_t active, pending;
+  int32_t max_active_seen;
+  int32_t reserved0;
+} pm_server_stats;
+
+/* The server reserves hbm_budget_bytes of device memory at creation. */
+pm_status pm_server_create(pm_engine* engine, const pm_server_config* config, pm_server** out);
+/* Queue one job.  options, view and outputs are copied; the memory they
+ * point to (images, cameras, depth maps, output buffers) is borrowed until
+ * pm_server_wait returns for the ticket.  Outputs go to the non-null
+ * pm_outputs buffers.  PM_ERR_OVER_BUDGET when the job alone needs more
+ * than the budget. */
+pm_status pm_server_submit(pm_server* server, const pm_options* options, const pm_view_desc* view,
+                           const pm_outputs* outputs, uint64_t* ticket);
+/* *done = 1 once the job's outputs are written (or it failed). */
+pm_status pm_server_poll(pm_server* server, uint64_t ticket, int32_t* done);
+/* Block until the job is done; returns its status and releases the ticket. */
+pm_status pm_server_wait(pm_server* server, uint64_t ticket, pm_job_stats* stats);
+pm_status pm_server_get_stats(pm_server* server, pm_server_stats* out);
+/* Finishes every submitted job, then frees the server. */
+void pm_server_destroy(pm_server* server);
 
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,

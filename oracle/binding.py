@@ -43,6 +43,9 @@ def load():
         lib.pmo_ncc_cost_f64.restype = C.c_double
         lib.pmo_ncc_cost_f64.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, P(C.c_double),
                                          C.c_int]
+        lib.pmo_cost_maps.restype = C.c_int
+        lib.pmo_cost_maps.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(C.c_float), P(C.c_float), C.c_int,
+                                      P(C.c_float), P(C.c_double)]
         lib.pmo_geom_cost.restype = C.c_float
         lib.pmo_geom_cost.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, C.c_float, C.c_int]
         for name in ("pmo_expf", "pmo_sinf", "pmo_cosf", "pmo_acosf"):
@@ -85,3 +88,22 @@ def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):
     if count:
         out["counters"] = (cnt.ncc_evaluations, cnt.ncc_samples, cnt.geo_evaluations)
     return out
+
+
+def cost_maps(options, view, depth, normal, threads=0):
+    """FP32 canonical and FP64 NCC costs of every pixel's hypothesis
+    (depth HxW, normal 3xHxW) against every source: (cost32, cost64), each
+    N x H x W, NaN where depth <= 0."""
+    lib = load()
+    W, H, N = view.width, view.height, view.num_src
+    d = np.ascontiguousarray(depth, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    c32 = np.empty((N, H, W), np.float32)
+    c64 = np.empty((N, H, W), np.float64)
+    opt = options.to_c()
+    rc = lib.pmo_cost_maps(C.byref(opt), C.byref(view.desc), d.ctypes.data_as(C.POINTER(C.c_float)),
+                           n.ctypes.data_as(C.POINTER(C.c_float)), int(threads),
+                           c32.ctypes.data_as(C.POINTER(C.c_float)), c64.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError("pmo_cost_maps failed")
+    return c32, c64

@@ -1160,18 +1160,14 @@ float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, in
  * weights, zero-border bilinear source, centre check, variance cut) with
  * every quantity in double and the homography applied per sample without
  * incremental accumulation.  The precision reference for the FP32 paths. */
-double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row, int col, const double hyp[4],
-                        int src) {
-  float k8[8];
-  float* poses = (float*)malloc(sizeof(float) * PM_POSE_STRIDE * view->num_src);
-  pmo_pack_poses(view, poses, k8);
+static double ncc_f64(const pm_options* opt, const pm_view_desc* view, const float* poses, const float k8[8],
+                      int row, int col, const double hyp[4], int src) {
   const float* pose = poses + (size_t)src * PM_POSE_STRIDE;
   const double K[4] = {pose[0], pose[1], pose[2], pose[3]};
   double Rr[9], Tt[3];
   for (int i = 0; i < 9; ++i) Rr[i] = pose[4 + i];
   for (int i = 0; i < 3; ++i) Tt[i] = pose[13 + i];
   const double iK[4] = {k8[4], k8[5], k8[6], k8[7]};
-  free(poses);
   const double n0 = hyp[0], n1 = hyp[1], n2 = hyp[2], dep = hyp[3];
   const double dist = dep * (n0 * (iK[0] * col + iK[1]) + n1 * (iK[2] * row + iK[3]) + n2);
   double A[9];
@@ -1240,6 +1236,61 @@ double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row
   if (val < 0) val = 0;
   if (val > 2) val = 2;
   return val;
+}
+
+double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row, int col, const double hyp[4],
+                        int src) {
+  float k8[8];
+  float* poses = (float*)malloc(sizeof(float) * PM_POSE_STRIDE * view->num_src);
+  pmo_pack_poses(view, poses, k8);
+  const double c = ncc_f64(opt, view, poses, k8, row, col, hyp, src);
+  free(poses);
+  return c;
+}
+
+/* Per-pixel, per-source NCC costs of the hypothesis maps (depth W*H,
+ * normal 3 planar W*H): the FP32 canonical cost under opt->accum_mode into
+ * out32 and its FP64 restatement into out64 (both N planar W*H; NaN where
+ * depth <= 0).  The precision study's reference. */
+int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, const float* normal,
+                  int num_threads, float* out32, double* out64) {
+#ifdef _OPENMP
+  if (num_threads > 0) omp_set_num_threads(num_threads);
+#else
+  (void)num_threads;
+#endif
+  pmo_view v;
+  uint8_t* rc;
+  uint8_t* sc[PM_MAX_NUM_SRC];
+  if (prepare_view(opt, view, &v, &rc, sc) != 0) return -1;
+  float k8[8];
+  float* poses = (float*)malloc(sizeof(float) * PM_POSE_STRIDE * view->num_src);
+  pmo_pack_poses(view, poses, k8);
+  const int W = view->ref.width, H = view->ref.height, N = view->num_src;
+  const long plane = (long)W * H;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long p = 0; p < plane; ++p) {
+    const int row = (int)(p / W), col = (int)(p % W);
+    const float d = depth[p];
+    for (int i = 0; i < N; ++i) {
+      if (!(d > 0.0f)) {
+        out32[i * plane + p] = NAN;
+        out64[i * plane + p] = NAN;
+        continue;
+      }
+      hyp4 h;
+      h.n[0] = normal[p];
+      h.n[1] = normal[plane + p];
+      h.n[2] = normal[2 * plane + p];
+      h.d = d;
+      const double hd[4] = {h.n[0], h.n[1], h.n[2], h.d};
+      out32[i * plane + p] = ncc_cost(opt, &v, row, col, &h, i, NULL);
+      out64[i * plane + p] = ncc_f64(opt, view, poses, k8, row, col, hd, i);
+    }
+  }
+  free(poses);
+  free_view(&v, rc, sc);
+  return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -94,6 +94,11 @@ float pmo_ncc_cost(const pm_options* opt, const pm_view_desc* view, int row, int
                    const float hyp[4], int src);
 double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row, int col,
                         const double hyp[4], int src);
+/* Both costs for every pixel and source of the hypothesis maps (depth
+ * W*H, normal 3 planar W*H) into out32 / out64 (N planar W*H, NaN where
+ * depth <= 0). */
+int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, const float* normal,
+                  int num_threads, float* out32, double* out64);
 /* Geometric-consistency term (FP32 canonical), K = ref {fx, cx, fy, cy}. */
 float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, int col,
                     float depth, int src);

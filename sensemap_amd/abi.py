@@ -72,7 +72,7 @@ class pm_image(C.Structure):
         ("width", C.c_int32),
         ("height", C.c_int32),
         ("stride", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("cache_id", C.c_int32),
     ]
 
 
@@ -140,6 +140,25 @@ class pm_tune_report(C.Structure):
     ]
 
 
+class pm_server_config(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_views", C.c_int32), ("reserved0", C.c_int32)]
+
+
+class pm_job_stats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("latency_ms", C.c_double), ("status", C.c_int32),
+                ("reserved0", C.c_int32)]
+
+
+class pm_server_stats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+        ("cache_hits", C.c_uint64), ("cache_misses", C.c_uint64), ("cache_evictions", C.c_uint64),
+        ("steps", C.c_uint64), ("launches", C.c_uint64),
+        ("arena_capacity", C.c_uint64), ("arena_used", C.c_uint64), ("cache_bytes", C.c_uint64),
+        ("active", C.c_int32), ("pending", C.c_int32), ("max_active_seen", C.c_int32), ("reserved0", C.c_int32),
+    ]
+
+
 _SIGNATURES = {
     "pm_options_init": (C.c_int, [C.POINTER(pm_options)]),
     "pm_options_check": (C.c_int, [C.POINTER(pm_options)]),
@@ -178,6 +197,13 @@ _SIGNATURES = {
     "pm_problem_autotune": (C.c_int, [C.c_void_p, C.POINTER(pm_tune_report)]),
     "pm_engine_save_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "pm_engine_load_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pm_server_create": (C.c_int, [C.c_void_p, C.POINTER(pm_server_config), C.POINTER(C.c_void_p)]),
+    "pm_server_submit": (C.c_int, [C.c_void_p, C.POINTER(pm_options), C.POINTER(pm_view_desc),
+                                   C.POINTER(pm_outputs), C.POINTER(C.c_uint64)]),
+    "pm_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "pm_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(pm_job_stats)]),
+    "pm_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_server_stats)]),
+    "pm_server_destroy": (None, [C.c_void_p]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

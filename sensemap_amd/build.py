@@ -29,7 +29,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", "-Wall", "-Wno-unused-function",
 ]
 
-HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip"]
+HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip", "pm_server.hip"]
 # extra compile flags (tuning knobs such as -DPM_SWEEP_WAVES_PER_EU=6)
 EXTRA = os.environ.get("PM_EXTRA_HIPFLAGS", "").split()
 
@@ -57,7 +57,7 @@ def build_engine(out_path: str | None = None) -> str:
             f.result()
     out = out_path or os.path.join(LIB, "libsensemap_pm.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, out)
     shutil.rmtree(objdir, ignore_errors=True)
     return out

@@ -18,6 +18,7 @@ constexpr int kQuadPad = 4;          // zero texels around every source image
 // Shapes: lw = 4 (16x32), 5 (32x16, the default), 6 (64x8).
 constexpr int kTileLog2WMin = 4, kTileLog2WMax = 6, kTileLog2WDefault = 5;
 constexpr int kTilePixels = 512;
+constexpr int kMaxBlocksPerCU = 8;   // resident workgroups of a persistent launch per CU, at most
 constexpr int kSweepThreads = 256;
 constexpr float kMaxCost = 2.0f;
 constexpr float kVarThreshold = 0.65025f;  // 1e-5 * 255^2

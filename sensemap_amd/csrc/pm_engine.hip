@@ -8,98 +8,89 @@
 // Get*Map() getters become pm_problem_get_*.  Errors are returned as
 // pm_status with a per-thread message, never by exit() as the reference's
 // CUDA_SAFE_CALL does (src/util/cudacc.cc:30-38).
-#include <hip/hip_runtime.h>
+#include "pm_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <mutex>
 #include <new>
-#include <string>
-#include <vector>
 
-#include "../../include/sensemap_pm.h"
-#include "pm_device.h"
 #include "pm_math.h"
 
-#define PM_API extern "C" __attribute__((visibility("default")))
+using namespace pmh;
+
+namespace pmh {
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
 pm_status fail(pm_status st, const std::string& msg) {
   g_last_error = msg;
   return st;
 }
 
-#define PM_HIP(call)                                                                           \
-  do {                                                                                         \
-    hipError_t e_ = (call);                                                                    \
-    if (e_ != hipSuccess) {                                                                    \
-      return fail(e_ == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,            \
-                  std::string(#call) + ": " + hipGetErrorString(e_));                          \
-    }                                                                                          \
-  } while (0)
+const std::string& last_error() { return g_last_error; }
 
-}  // namespace
+// ---------------------------------------------------------------------------
+// arena
 
-// Launch shape of one phase kernel class: tile width (log2) and the cap on
-// resident workgroups per CU (0 = as many as fit).
-struct TuneKey {
-  int kind, radius, step, storage, rcp, nbucket;
-  bool operator<(const TuneKey& o) const {
-    const int a[6] = {kind, radius, step, storage, rcp, nbucket};
-    const int b[6] = {o.kind, o.radius, o.step, o.storage, o.rcp, o.nbucket};
-    for (int k = 0; k < 6; ++k)
-      if (a[k] != b[k]) return a[k] < b[k];
-    return false;
+Arena::~Arena() {
+  if (base_) (void)hipFree(base_);
+}
+
+pm_status Arena::init(size_t bytes) {
+  bytes = round(bytes);
+  PM_HIP(hipMalloc(&base_, bytes));
+  cap_ = bytes;
+  used_ = 0;
+  holes_.clear();
+  holes_[0] = bytes;
+  return PM_OK;
+}
+
+bool Arena::alloc(size_t bytes, size_t* off) {
+  bytes = round(bytes > 0 ? bytes : 1);
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto it = holes_.begin(); it != holes_.end(); ++it) {
+    if (it->second < bytes) continue;
+    *off = it->first;
+    const size_t rest = it->second - bytes;
+    const size_t roff = it->first + bytes;
+    holes_.erase(it);
+    if (rest) holes_[roff] = rest;
+    used_ += bytes;
+    return true;
   }
-};
+  return false;
+}
 
-struct pm_engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int counting = 0;
-  std::mutex tune_mu;
-  std::map<TuneKey, pm_tuning> tuning;  // absent = default shape
-};
-
-struct pm_problem {
-  pm_engine* engine = nullptr;
-  pm_options opt{};
-  int W = 0, H = 0, N = 0;
-  uint32_t view_id = 0;
-  float depth_min = 0.f, depth_max = 0.f;
-  std::vector<void*> allocs;
-  pm::ViewDev host{};
-  pm::ViewDev* dev = nullptr;
-  float* init_depth = nullptr;   // device copies of the given initial state
-  float* init_normal = nullptr;
-  unsigned long long* counters = nullptr;
-  std::vector<float> poses;      // N * PM_POSE_STRIDE
-  float refk8[8] = {};
-  pm_run_stats stats{};
-  size_t device_bytes = 0;
-  struct pm_schedule* schedule = nullptr;
-
-  template <typename T>
-  pm_status alloc(T** p, size_t count) {
-    void* q = nullptr;
-    const size_t bytes = count * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 1);
-    if (e != hipSuccess) {
-      return fail(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
-                  "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+void Arena::release(size_t off, size_t bytes) {
+  bytes = round(bytes > 0 ? bytes : 1);
+  std::lock_guard<std::mutex> lk(mu_);
+  used_ -= bytes;
+  auto it = holes_.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != holes_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    holes_.erase(nx);
+  }
+  if (it != holes_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      holes_.erase(it);
     }
-    allocs.push_back(q);
-    device_bytes += bytes;
-    *p = (T*)q;
-    return PM_OK;
   }
-  ~pm_problem();
-};
+}
+
+size_t Arena::used() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return used_;
+}
+
+}  // namespace pmh
 
 // ---------------------------------------------------------------------------
 // options
@@ -169,7 +160,7 @@ PM_API pm_status pm_options_check(const pm_options* o) {
   return PM_OK;
 }
 
-PM_API const char* pm_last_error(void) { return g_last_error.c_str(); }
+PM_API const char* pm_last_error(void) { return pmh::last_error().c_str(); }
 PM_API int32_t pm_abi_version(void) { return PM_ABI_VERSION; }
 
 // ---------------------------------------------------------------------------
@@ -211,7 +202,7 @@ PM_API pm_status pm_engine_set_counting(pm_engine* e, int32_t enable) {
 // ---------------------------------------------------------------------------
 // problem
 
-namespace {
+namespace pmh {
 
 // InitTransforms (patch_match_cuda.cu:493-580) with ComputeRelativePose,
 // ComposeProjectionMatrix and ComposeInverseProjectionMatrix
@@ -340,9 +331,58 @@ pm_status check_view(const pm_options* o, const pm_view_desc* v) {
   return PM_OK;
 }
 
-}  // namespace
+}  // namespace pmh
 
-PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_view_desc* v, pm_problem** out) {
+namespace pmh {
+
+size_t src_quad_bytes(const pm_image& im, int storage) {
+  const size_t n = (size_t)(im.width + 2 * pm::kQuadPad) * (im.height + 2 * pm::kQuadPad);
+  return n * (storage == PM_STORAGE_U8 ? sizeof(uint32_t) : sizeof(uint2));
+}
+
+size_t job_block_bytes(const pm_options* o, const pm_view_desc* v, const bool* src_cached) {
+  const size_t plane = (size_t)v->ref.width * v->ref.height;
+  const int N = v->num_src;
+  int nwin = 0;
+  for (int d = -o->window_radius; d <= o->window_radius; d += o->window_step) ++nwin;
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b += Arena::round(bytes > 0 ? bytes : 1); };
+  add(plane);                                         // ref
+  add((size_t)nwin * nwin * sizeof(float));           // spatial weights
+  add(256 * sizeof(float));                           // colour weights
+  add((size_t)N * PM_POSE_STRIDE * sizeof(float));    // poses
+  for (int i = 0; i < N; ++i)
+    if (!src_cached || !src_cached[i]) add(src_quad_bytes(v->src[i], o->image_storage));
+  if (o->geom_consistency)
+    for (int i = 0; i < N; ++i)
+      add((size_t)(v->src[i].width + 2 * pm::kQuadPad) * (v->src[i].height + 2 * pm::kQuadPad) * sizeof(float4));
+  add(plane * sizeof(float4));                        // moments
+  add(plane);                                         // gradient gate
+  add(plane * sizeof(float));                         // depth
+  add(3 * plane * sizeof(float));                     // normal
+  add(plane * sizeof(float));                         // conf
+  add((size_t)N * plane * sizeof(float));             // cost
+  add(plane * sizeof(uint32_t));                      // view_sel
+  add(6 * sizeof(unsigned long long));                // counters
+  if (!o->init_depth_random) {
+    add(plane * sizeof(float));
+    add(3 * plane * sizeof(float));
+  }
+  add(sizeof(pm::ViewDev));
+  return b;
+}
+
+pm_status upload_quads(const pm_image& im, int storage, void* dst, uint8_t* staging, hipStream_t s) {
+  PM_HIP(hipMemcpy2DAsync(staging, im.width, im.data, im.stride, im.width, im.height, hipMemcpyHostToDevice, s));
+  if (storage == PM_STORAGE_U8)
+    PM_HIP(pm::launch_build_quads_u8(staging, im.width, im.height, im.width, (uint32_t*)dst, s));
+  else
+    PM_HIP(pm::launch_build_quads_f16(staging, im.width, im.height, im.width, (uint2*)dst, s));
+  return PM_OK;
+}
+
+pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* v, const CreateEnv& env,
+                         pm_problem** out) {
   if (!e || !o || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
   *out = nullptr;
   pm_status st = pm_options_check(o);
@@ -354,8 +394,10 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   if (!p) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
   struct Guard {
     pm_problem*& p;
+    uint8_t* staging = nullptr;  // temporary staging (engine problems)
     bool keep = false;
     ~Guard() {
+      if (staging) (void)hipFree(staging);
       if (!keep) delete p;
     }
   } guard{p};
@@ -367,9 +409,14 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   p->view_id = v->view_id;
   p->depth_min = v->depth_min;
   p->depth_max = v->depth_max;
+  if (env.arena) {
+    p->arena = env.arena;
+    p->block_off = env.block_off;
+    p->block_bytes = env.block_bytes;
+  }
   const int W = p->W, H = p->H, N = p->N;
   const size_t plane = (size_t)W * H;
-  hipStream_t s = e->stream;
+  hipStream_t s = env.stream ? env.stream : e->stream;
   pm::ViewDev& h = p->host;
   std::memset(&h, 0, sizeof(h));
 
@@ -391,70 +438,61 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   if ((st = p->alloc(&d_poses, p->poses.size())) != PM_OK) return st;
   PM_HIP(hipMemcpyAsync(d_poses, p->poses.data(), p->poses.size() * sizeof(float), hipMemcpyHostToDevice, s));
 
-  // source images: plain upload, then the padded quad layout
+  // staging for the uploads below: the caller's, or a temporary one
   size_t max_src = 0;
-  for (int i = 0; i < N; ++i) max_src = std::max(max_src, (size_t)v->src[i].width * v->src[i].height);
-  uint8_t* staging = nullptr;
-  PM_HIP(hipMalloc(&staging, max_src));
+  for (int i = 0; i < N; ++i) {
+    const size_t n = (size_t)v->src[i].width * v->src[i].height;
+    max_src = std::max(max_src, o->geom_consistency ? n * sizeof(float) : n);
+  }
+  uint8_t* staging = env.staging;
+  if (!staging) {
+    PM_HIP(hipMalloc(&guard.staging, max_src));
+    staging = guard.staging;
+  } else if (env.staging_bytes < max_src) {
+    return fail(PM_ERR_INTERNAL, "staging buffer too small");
+  }
+
+  // source images in the padded quad layout (stream order keeps the staging
+  // buffer's reuse safe: each copy follows the previous quad kernel)
   for (int i = 0; i < N; ++i) {
     const pm_image& si = v->src[i];
-    const int pw = si.width + 2 * pm::kQuadPad, ph = si.height + 2 * pm::kQuadPad;
     pm::SrcDev& sd = h.src[i];
     sd.width = si.width;
     sd.height = si.height;
-    sd.pitch = pw;
-    hipError_t he = hipMemcpy2DAsync(staging, si.width, si.data, si.stride, si.width, si.height,
-                                     hipMemcpyHostToDevice, s);
-    if (he == hipSuccess) {
-      if (o->image_storage == PM_STORAGE_U8) {
-        uint32_t* q = nullptr;
-        if ((st = p->alloc(&q, (size_t)pw * ph)) != PM_OK) {
-          (void)hipFree(staging);
-          return st;
-        }
-        he = pm::launch_build_quads_u8(staging, si.width, si.height, si.width, q, s);
-        sd.tex = q;
-      } else {
-        uint2* q = nullptr;
-        if ((st = p->alloc(&q, (size_t)pw * ph)) != PM_OK) {
-          (void)hipFree(staging);
-          return st;
-        }
-        he = pm::launch_build_quads_f16(staging, si.width, si.height, si.width, q, s);
-        sd.tex = q;
-      }
+    sd.pitch = si.width + 2 * pm::kQuadPad;
+    if (env.src_tex && env.src_tex[i]) {
+      sd.tex = env.src_tex[i];
+      continue;
     }
-    if (he == hipSuccess) he = hipStreamSynchronize(s);  // staging is reused by the next source
-    if (he != hipSuccess) {
-      (void)hipFree(staging);
-      return fail(PM_ERR_DEVICE, std::string("source upload: ") + hipGetErrorString(he));
+    void* q = nullptr;
+    if (o->image_storage == PM_STORAGE_U8) {
+      uint32_t* t = nullptr;
+      if ((st = p->alloc(&t, src_quad_bytes(si, o->image_storage) / sizeof(uint32_t))) != PM_OK) return st;
+      q = t;
+    } else {
+      uint2* t = nullptr;
+      if ((st = p->alloc(&t, src_quad_bytes(si, o->image_storage) / sizeof(uint2))) != PM_OK) return st;
+      q = t;
     }
+    if ((st = upload_quads(si, o->image_storage, q, staging, s)) != PM_OK) return st;
+    sd.tex = q;
   }
-  (void)hipFree(staging);
 
   // source depth maps (geometric pass)
   if (o->geom_consistency) {
     for (int i = 0; i < N; ++i) {
       const pm_image& si = v->src[i];
       const size_t n = (size_t)si.width * si.height;
-      float* tmp = nullptr;
-      PM_HIP(hipMalloc(&tmp, n * sizeof(float)));
       float4* dq = nullptr;
       const size_t pn = (size_t)(si.width + 2 * pm::kQuadPad) * (si.height + 2 * pm::kQuadPad);
-      if ((st = p->alloc(&dq, pn)) != PM_OK) {
-        (void)hipFree(tmp);
-        return st;
-      }
-      hipError_t he = hipMemcpyAsync(tmp, v->src_depth_maps[i], n * sizeof(float), hipMemcpyHostToDevice, s);
-      if (he == hipSuccess) he = pm::launch_build_depth_quads(tmp, si.width, si.height, dq, s);
-      if (he == hipSuccess) he = hipStreamSynchronize(s);
-      (void)hipFree(tmp);
-      if (he != hipSuccess) return fail(PM_ERR_DEVICE, std::string("depth upload: ") + hipGetErrorString(he));
+      if ((st = p->alloc(&dq, pn)) != PM_OK) return st;
+      PM_HIP(hipMemcpyAsync(staging, v->src_depth_maps[i], n * sizeof(float), hipMemcpyHostToDevice, s));
+      PM_HIP(pm::launch_build_depth_quads((const float*)staging, si.width, si.height, dq, s));
       h.src[i].depth_q = dq;
     }
   }
 
-  // state, workspace, moments
+  // state, moments
   float4* refm = nullptr;
   uint8_t* grad = nullptr;
   if ((st = p->alloc(&refm, plane)) != PM_OK) return st;
@@ -512,45 +550,41 @@ PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_v
   h.ct = d_ct;
   h.counters = nullptr;
   if ((st = p->alloc(&p->dev, 1)) != PM_OK) return st;
+  if (p->arena && p->block_used != p->block_bytes) return fail(PM_ERR_INTERNAL, "job memory block size mismatch");
   PM_HIP(hipMemcpyAsync(p->dev, &h, sizeof(h), hipMemcpyHostToDevice, s));
   PM_HIP(pm::launch_ref_moments(p->dev, W, H, o->accum_mode, refm, grad, s));
+  // host sources of the copies above are pageable locals and caller memory
   PM_HIP(hipStreamSynchronize(s));
   guard.keep = true;
   *out = p;
   return PM_OK;
 }
 
-namespace {
+}  // namespace pmh
 
-struct EventTimer {
-  std::vector<hipEvent_t> ev;
-  pm_status rec(hipStream_t s) {
-    hipEvent_t e;
-    PM_HIP(hipEventCreate(&e));
-    ev.push_back(e);
-    PM_HIP(hipEventRecord(e, s));
-    return PM_OK;
-  }
-  float between(size_t a, size_t b) const {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) return 0.0f;
-    return ms;
-  }
-  ~EventTimer() {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  }
-};
+PM_API pm_status pm_problem_create(pm_engine* e, const pm_options* o, const pm_view_desc* v, pm_problem** out) {
+  return create_problem(e, o, v, CreateEnv{}, out);
+}
 
-// One phase of one view: the iteration schedule of Run()
-// (patch_match_cuda.cu:774-909): init, then per iteration the black and red
-// sweeps and, from iteration 3, the black and red local optimisation; the
-// optional FilterDepthMap pair at the end (:1007-1033).
-struct PhaseRec {
-  pm::PhaseKind kind;
-  int color;
-  uint32_t pass;
-  float th_mc;
-};
+namespace pmh {
+
+pm_status EventTimer::rec(hipStream_t s) {
+  hipEvent_t e;
+  PM_HIP(hipEventCreate(&e));
+  ev.push_back(e);
+  PM_HIP(hipEventRecord(e, s));
+  return PM_OK;
+}
+
+float EventTimer::between(size_t a, size_t b) const {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) return 0.0f;
+  return ms;
+}
+
+EventTimer::~EventTimer() {
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
 
 // Good-matching threshold of iteration `iter` (patch_match_cuda.cu:853).
 float th_mc_of(const pm_options& o, int iter) {
@@ -573,32 +607,7 @@ std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o) {
   return t;
 }
 
-}  // namespace
-
-// All launches of one run over a set of views: at tick t every view runs its
-// phase t, and views whose phase shares the kernel (kind and window shape)
-// share one launch, described by a LaunchTable.  The tables of all launches
-// are built on the host once and uploaded in one copy.
-struct pm_schedule {
-  struct Launch {
-    pm::PhaseKind kind;
-    pm::LaunchCfg cfg;
-    size_t offset;  // of the launch's table in `tables`
-    int tiles;
-    int grid;       // persistent workgroups
-  };
-  std::vector<uint8_t> tables;
-  std::vector<Launch> launches;
-  void* dev = nullptr;         // device copy of `tables`
-  uint32_t* queues = nullptr;  // 8 tile counters per launch
-  float* ws = nullptr;         // workspace sized for the largest launch
-  size_t ws_floats = 0;
-  ~pm_schedule() {
-    if (dev) (void)hipFree(dev);
-    if (queues) (void)hipFree(queues);
-    if (ws) (void)hipFree(ws);
-  }
-};
+}  // namespace pmh
 
 pm_problem::~pm_problem() {
   delete schedule;
@@ -612,7 +621,7 @@ PM_API void pm_problem_destroy(pm_problem* p) {
   delete p;
 }
 
-namespace {
+namespace pmh {
 
 pm::LaunchCfg cfg_of(const pm_options& o) { return pm::LaunchCfg{o.image_storage, o.rcp_mode, o.window_radius, o.window_step}; }
 
@@ -629,70 +638,80 @@ pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N) {
   return pm_tuning{pm::kTileLog2WDefault, 0};
 }
 
-using Ticks = std::vector<std::vector<std::vector<PhaseRec>>>;  // [view][tick][phase]
+pm_status plan_launches(const std::vector<std::pair<pm_problem*, const std::vector<PhaseRec>*>>& phases,
+                        TickPlan& plan, const pm_tuning* force) {
+  struct Group {
+    pm::PhaseKind kind;
+    pm::LaunchCfg cfg;
+    pm_tuning shape;
+    std::vector<pm::LaunchEntry> e;
+    int tiles = 0;
+    int nmax = 1;  // most sources of a view in the launch (workspace size)
+  };
+  std::vector<Group> groups;
+  for (const auto& ph : phases) {
+    pm_problem* p = ph.first;
+    for (const PhaseRec& r : *ph.second) {
+      const pm::LaunchCfg cfg = r.kind == pm::kPhaseFilter ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
+      const pm_tuning shape = force ? *force : shape_of(p->engine, r.kind, p->opt, p->N);
+      Group* g = nullptr;
+      for (Group& gg : groups)
+        if (gg.kind == r.kind && gg.cfg == cfg && gg.shape.tile_lw == shape.tile_lw &&
+            gg.shape.blocks_per_cu == shape.blocks_per_cu)
+          g = &gg;
+      if (!g) {
+        groups.push_back(Group{r.kind, cfg, shape, {}, 0, 1});
+        g = &groups.back();
+      }
+      pm::LaunchEntry en{};
+      en.v = p->dev;
+      en.tile_start = g->tiles;
+      en.color = r.color;
+      en.pass = r.pass;
+      en.th_mc = r.th_mc;
+      g->e.push_back(en);
+      const long long tiles = (long long)g->tiles + pm::view_tiles(p->W, p->H, g->shape.tile_lw);
+      if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "too many tiles in one launch");
+      g->tiles = (int)tiles;
+      g->nmax = std::max(g->nmax, p->N);
+    }
+  }
+  for (Group& g : groups) {
+    const int grid = pm::phase_grid(g.kind, g.cfg, g.tiles, g.shape.blocks_per_cu);
+    if (grid <= 0) return fail(PM_ERR_DEVICE, "no launch configuration for a phase kernel");
+    const size_t off = (plan.tables.size() + 15) & ~(size_t)15;
+    plan.tables.resize(off + pm::launch_table_bytes((int)g.e.size()));
+    pm::LaunchTable* lt = (pm::LaunchTable*)(plan.tables.data() + off);
+    lt->n = (int32_t)g.e.size();
+    lt->total_tiles = g.tiles;
+    lt->ws_slots = grid * pm::kSweepThreads;
+    lt->tile_lw = g.shape.tile_lw;
+    lt->ws = nullptr;
+    lt->queue = nullptr;
+    std::memcpy(lt->e, g.e.data(), g.e.size() * sizeof(pm::LaunchEntry));
+    plan.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, g.tiles, grid});
+    plan.ws_floats = std::max(plan.ws_floats, (size_t)lt->ws_slots * pm::ws_floats_per_slot(g.nmax));
+  }
+  return PM_OK;
+}
 
 // Build the launches of `ticks` over `views`; `force` overrides the shapes
 // (autotuning).
 pm_status build_schedule(const std::vector<pm_problem*>& views, const Ticks& ticks, pm_schedule& sch, hipStream_t s,
-                         const pm_tuning* force = nullptr) {
+                         const pm_tuning* force) {
   size_t T = 0;
   for (const auto& tv : ticks) T = std::max(T, tv.size());
-  sch.tables.clear();
-  sch.launches.clear();
-  sch.ws_floats = 0;
+  TickPlan plan;
   for (size_t t = 0; t < T; ++t) {
-    struct Group {
-      pm::PhaseKind kind;
-      pm::LaunchCfg cfg;
-      pm_tuning shape;
-      std::vector<pm::LaunchEntry> e;
-      int tiles = 0;
-      int nmax = 1;  // most sources of a view in the launch (workspace size)
-    };
-    std::vector<Group> groups;
-    for (size_t vi = 0; vi < views.size(); ++vi) {
-      if (t >= ticks[vi].size()) continue;
-      pm_problem* p = views[vi];
-      for (const PhaseRec& r : ticks[vi][t]) {
-        pm::LaunchCfg cfg = r.kind == pm::kPhaseFilter ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
-        const pm_tuning shape = force ? *force : shape_of(p->engine, r.kind, p->opt, p->N);
-        Group* g = nullptr;
-        for (Group& gg : groups)
-          if (gg.kind == r.kind && gg.cfg == cfg && gg.shape.tile_lw == shape.tile_lw &&
-              gg.shape.blocks_per_cu == shape.blocks_per_cu)
-            g = &gg;
-        if (!g) {
-          groups.push_back(Group{r.kind, cfg, shape, {}, 0, 1});
-          g = &groups.back();
-        }
-        pm::LaunchEntry en{};
-        en.v = p->dev;
-        en.tile_start = g->tiles;
-        en.color = r.color;
-        en.pass = r.pass;
-        en.th_mc = r.th_mc;
-        g->e.push_back(en);
-        const long long tiles = (long long)g->tiles + pm::view_tiles(p->W, p->H, g->shape.tile_lw);
-        if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "too many tiles in one launch");
-        g->tiles = (int)tiles;
-        g->nmax = std::max(g->nmax, p->N);
-      }
-    }
-    for (Group& g : groups) {
-      const int grid = pm::phase_grid(g.kind, g.cfg, g.tiles, g.shape.blocks_per_cu);
-      if (grid <= 0) return fail(PM_ERR_DEVICE, "no launch configuration for a phase kernel");
-      const size_t off = (sch.tables.size() + 15) & ~(size_t)15;
-      sch.tables.resize(off + pm::launch_table_bytes((int)g.e.size()));
-      pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + off);
-      lt->n = (int32_t)g.e.size();
-      lt->total_tiles = g.tiles;
-      lt->ws_slots = grid * pm::kSweepThreads;
-      lt->tile_lw = g.shape.tile_lw;
-      std::memcpy(lt->e, g.e.data(), g.e.size() * sizeof(pm::LaunchEntry));
-      sch.launches.push_back(pm_schedule::Launch{g.kind, g.cfg, off, g.tiles, grid});
-      sch.ws_floats = std::max(sch.ws_floats, (size_t)lt->ws_slots * pm::ws_floats_per_slot(g.nmax));
-    }
+    std::vector<std::pair<pm_problem*, const std::vector<PhaseRec>*>> phases;
+    for (size_t vi = 0; vi < views.size(); ++vi)
+      if (t < ticks[vi].size()) phases.emplace_back(views[vi], &ticks[vi][t]);
+    pm_status rs = plan_launches(phases, plan, force);
+    if (rs != PM_OK) return rs;
   }
+  sch.tables = std::move(plan.tables);
+  sch.launches = std::move(plan.launches);
+  sch.ws_floats = plan.ws_floats;
   if (sch.ws) PM_HIP(hipFree(sch.ws));
   if (sch.queues) PM_HIP(hipFree(sch.queues));
   if (sch.dev) PM_HIP(hipFree(sch.dev));
@@ -797,12 +816,12 @@ pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_sche
   return read_counters(views, st);
 }
 
-}  // namespace
+}  // namespace pmh
 
 // ---------------------------------------------------------------------------
 // launch shapes and the autotuner
 
-namespace {
+namespace pmh {
 
 pm_status check_phase(int32_t phase) {
   if (phase < PM_PHASE_INIT || phase > PM_PHASE_FILTER) return fail(PM_ERR_INVALID_ARG, "unknown phase");
@@ -937,7 +956,7 @@ std::string device_arch(int device) {
   return c == std::string::npos ? a : a.substr(0, c);
 }
 
-}  // namespace
+}  // namespace pmh
 
 PM_API pm_status pm_engine_set_tuning(pm_engine* e, int32_t phase, const pm_options* o, int32_t num_src,
                                       const pm_tuning* t) {

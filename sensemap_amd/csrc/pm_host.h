@@ -1,0 +1,234 @@
+// pm_host.h -- host-side internals shared by the engine (pm_engine.hip) and
+// the serving loop (pm_server.hip).  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sensemap_pm.h"
+#include "pm_device.h"
+
+#define PM_API extern "C" __attribute__((visibility("default")))
+
+namespace pmh {
+
+// Record `msg` as this thread's pm_last_error() text and return `st`.
+pm_status fail(pm_status st, const std::string& msg);
+const std::string& last_error();
+
+}  // namespace pmh
+
+#define PM_HIP(call)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      return ::pmh::fail(e_ == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,     \
+                         std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    }                                                                                          \
+  } while (0)
+
+namespace pmh {
+
+// Launch-shape class: phase kernel, window, storage, reciprocal mode and
+// source-count bucket.
+struct TuneKey {
+  int kind, radius, step, storage, rcp, nbucket;
+  bool operator<(const TuneKey& o) const {
+    const int a[6] = {kind, radius, step, storage, rcp, nbucket};
+    const int b[6] = {o.kind, o.radius, o.step, o.storage, o.rcp, o.nbucket};
+    for (int k = 0; k < 6; ++k)
+      if (a[k] != b[k]) return a[k] < b[k];
+    return false;
+  }
+};
+
+// First-fit allocator over one device allocation: the HBM a serving loop
+// may hold.  Offsets are 256-byte aligned.  Thread-safe.
+class Arena {
+ public:
+  static constexpr size_t kAlign = 256;
+  Arena() = default;
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  ~Arena();
+  pm_status init(size_t bytes);
+  // false when no hole is large enough
+  bool alloc(size_t bytes, size_t* off);
+  void release(size_t off, size_t bytes);
+  uint8_t* base() const { return base_; }
+  size_t capacity() const { return cap_; }
+  size_t used() const;
+  static size_t round(size_t bytes) { return (bytes + kAlign - 1) & ~(kAlign - 1); }
+
+ private:
+  mutable std::mutex mu_;
+  std::map<size_t, size_t> holes_;  // offset -> length
+  uint8_t* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+};
+
+// Where a problem's device memory and uploads come from.
+struct CreateEnv {
+  hipStream_t stream = nullptr;          // uploads and set-up kernels (null: the engine's stream)
+  Arena* arena = nullptr;                // non-null: all job memory in one arena range, reserved by the caller
+  size_t block_off = 0, block_bytes = 0;  //   that range
+  const void* const* src_tex = nullptr;  // non-null: per-source quads already on the device (null entries: upload)
+  uint8_t* staging = nullptr;            // non-null: upload staging buffer of staging_bytes
+  size_t staging_bytes = 0;
+};
+
+}  // namespace pmh
+
+struct pm_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int counting = 0;
+  std::mutex tune_mu;
+  std::map<pmh::TuneKey, pm_tuning> tuning;  // absent = default shape
+};
+
+struct pm_problem {
+  pm_engine* engine = nullptr;
+  pm_options opt{};
+  int W = 0, H = 0, N = 0;
+  uint32_t view_id = 0;
+  float depth_min = 0.f, depth_max = 0.f;
+  std::vector<void*> allocs;           // hipMalloc'd (engine problems)
+  pmh::Arena* arena = nullptr;         // arena-backed (server jobs): one range, carved in order
+  size_t block_off = 0, block_bytes = 0, block_used = 0;
+  pm::ViewDev host{};
+  pm::ViewDev* dev = nullptr;
+  float* init_depth = nullptr;   // device copies of the given initial state
+  float* init_normal = nullptr;
+  unsigned long long* counters = nullptr;
+  std::vector<float> poses;      // N * PM_POSE_STRIDE
+  float refk8[8] = {};
+  pm_run_stats stats{};
+  size_t device_bytes = 0;
+  struct pm_schedule* schedule = nullptr;
+
+  template <typename T>
+  pm_status alloc(T** p, size_t count) {
+    const size_t bytes = count * sizeof(T);
+    if (arena) {
+      const size_t off = block_used;
+      const size_t need = pmh::Arena::round(bytes > 0 ? bytes : 1);
+      if (off + need > block_bytes) return pmh::fail(PM_ERR_INTERNAL, "job memory block too small");
+      block_used += need;
+      device_bytes += need;
+      *p = (T*)(arena->base() + block_off + off);
+      return PM_OK;
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 1);
+    if (e != hipSuccess) {
+      return pmh::fail(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
+                       "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    allocs.push_back(q);
+    device_bytes += bytes;
+    *p = (T*)q;
+    return PM_OK;
+  }
+  ~pm_problem();
+};
+
+namespace pmh {
+
+struct EventTimer {
+  std::vector<hipEvent_t> ev;
+  pm_status rec(hipStream_t s);
+  float between(size_t a, size_t b) const;
+  ~EventTimer();
+};
+
+// One phase of one view: the iteration schedule of Run()
+// (patch_match_cuda.cu:774-909): init, then per iteration the black and red
+// sweeps and, from iteration 3, the black and red local optimisation; the
+// optional FilterDepthMap pair at the end (:1007-1033).
+struct PhaseRec {
+  pm::PhaseKind kind;
+  int color;
+  uint32_t pass;
+  float th_mc;
+};
+
+using Ticks = std::vector<std::vector<std::vector<PhaseRec>>>;  // [view][tick][phase]
+
+float th_mc_of(const pm_options& o, int iter);
+std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o);
+Ticks all_ticks(const std::vector<pm_problem*>& views);
+
+pm::LaunchCfg cfg_of(const pm_options& o);
+TuneKey tune_key(int kind, const pm_options& o, int N);
+pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N);
+
+// Validation shared by every entry point that takes a view.
+pm_status check_view(const pm_options* o, const pm_view_desc* v);
+
+// Device bytes of one job's memory block (state, tables, reference and the
+// sources not given in `src_cached`), in Arena::round units.
+size_t job_block_bytes(const pm_options* o, const pm_view_desc* v, const bool* src_cached);
+// Device bytes of one source's quad texels at the given storage.
+size_t src_quad_bytes(const pm_image& im, int storage);
+// Upload one 8-bit image into quads at `dst` (staging: a device buffer of at
+// least width*height bytes; stream-ordered, no wait).
+pm_status upload_quads(const pm_image& im, int storage, void* dst, uint8_t* staging, hipStream_t s);
+
+pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* v, const CreateEnv& env,
+                         pm_problem** out);
+
+}  // namespace pmh
+
+// All launches of one run over a set of views: at tick t every view runs its
+// phase t, and views whose phase shares the kernel (kind, window shape and
+// launch shape) share one launch, described by a LaunchTable.  The tables of
+// all launches are built on the host once and uploaded in one copy.
+struct pm_schedule {
+  struct Launch {
+    pm::PhaseKind kind;
+    pm::LaunchCfg cfg;
+    size_t offset;  // of the launch's table in `tables`
+    int tiles;
+    int grid;       // persistent workgroups
+  };
+  std::vector<uint8_t> tables;
+  std::vector<Launch> launches;
+  void* dev = nullptr;         // device copy of `tables`
+  uint32_t* queues = nullptr;  // 8 tile counters per launch
+  float* ws = nullptr;         // workspace sized for the largest launch
+  size_t ws_floats = 0;
+  ~pm_schedule() {
+    if (dev) (void)hipFree(dev);
+    if (queues) (void)hipFree(queues);
+    if (ws) (void)hipFree(ws);
+  }
+};
+
+namespace pmh {
+
+// Host image of the launches of one tick (or of a whole run): groups the
+// phases by kernel class and writes LaunchTables.  `ws` and `queue` pointers
+// are patched by the caller once the device buffers are known.
+struct TickPlan {
+  std::vector<uint8_t> tables;
+  std::vector<pm_schedule::Launch> launches;
+  size_t ws_floats = 0;  // workspace the launches need
+};
+pm_status plan_launches(const std::vector<std::pair<pm_problem*, const std::vector<PhaseRec>*>>& phases,
+                        TickPlan& plan, const pm_tuning* force);
+
+pm_status build_schedule(const std::vector<pm_problem*>& views, const Ticks& ticks, pm_schedule& sch, hipStream_t s,
+                         const pm_tuning* force = nullptr);
+pm_status prepare_views(const std::vector<pm_problem*>& views, bool counting, hipStream_t s, bool allow_memset);
+pm_status launch_all(const pm_schedule& sch, hipStream_t s, EventTimer* t, pm_run_stats* st);
+pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_schedule& sch, pm_run_stats* st);
+pm_status tune_batch(const std::vector<pm_problem*>& views);
+
+}  // namespace pmh

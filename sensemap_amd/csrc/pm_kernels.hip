@@ -42,6 +42,14 @@ namespace {
 // Pointers stored in ViewDev are generic; cast them to the global address
 // space so loads and stores compile to global_* (or s_load for uniform
 // addresses) instead of flat_*.
+// A view descriptor as the colour-pass kernels read it: through the
+// constant address space, so that the compiler may use scalar loads (the
+// descriptor is written only by host copies between launches) and keeps
+// its fields, the source texture bases among them, in scalar registers.
+using VPtr = const __attribute__((address_space(4))) ViewDev*;
+template <typename T>
+using K4 = const __attribute__((address_space(4))) T;
+
 template <typename T>
 __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
   return (__attribute__((address_space(1))) T*)p;
@@ -100,7 +108,7 @@ struct Counters {
 };
 
 // Counter slots: [0..2] all kernels, [3..5] the sweep kernel alone.
-__device__ __forceinline__ void flush_counters(const ViewDev* v, const Counters& c, bool sweep = false) {
+__device__ __forceinline__ void flush_counters(VPtr v, const Counters& c, bool sweep = false) {
   if (v->counters) {
     atomicAdd(&v->counters[0], c.ncc_eval);
     atomicAdd(&v->counters[1], c.ncc_samples);
@@ -174,7 +182,7 @@ __device__ __forceinline__ void neumaier_add(float& s, float& c, float x) {
 // the lane's hypothesis.  tile points at the LDS ref tile entry of the
 // window's top-left sample; tw is the tile row length.
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, const Pix& px,
+__device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
                                           const uint8_t* tile, int tw, const float* ct, const float* sp,
                                           float nx, float ny, float nz, float d, Counters& cnt) {
   const int R = R_ >= 0 ? R_ : v->radius;
@@ -185,7 +193,7 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
   const float K0 = pose[0], K1 = pose[1], K2 = pose[2], K3 = pose[3];
   const __attribute__((address_space(1))) float* Rm = pose + 4;
   const __attribute__((address_space(1))) float* T = pose + 13;
-  const float* iK = v->refInvK;
+  K4<float>* iK = v->refInvK;
   const float fcol = (float)px.col, frow = (float)px.row;
   const float ray0 = iK[0] * fcol + iK[1];
   const float ray1 = iK[2] * frow + iK[3];
@@ -217,7 +225,7 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
   const float izc = 1.0f / zc;
   const float xf = (H[0] * fcol + H[1] * frow + H[2]) * izc;
   const float yf = (H[3] * fcol + H[4] * frow + H[5]) * izc;
-  const SrcDev& s = v->src[i];
+  K4<SrcDev>& s = v->src[i];
   const float Wsf = (float)s.width, Hsf = (float)s.height;
   if (!(xf > -1.0f && xf < Wsf && yf > -1.0f && yf < Hsf)) return kMaxCost;
   if (px.var_r < kVarThreshold) return kMaxCost;
@@ -316,8 +324,8 @@ __device__ __forceinline__ float ncc_eval(const ViewDev* __restrict__ v, int i, 
 }
 
 // ComputeGeomConsistencyCostShared (kernel_functions.cu:505-568).
-__device__ __forceinline__ float geom_eval(const ViewDev* __restrict__ v, int i, const Pix& px, float depth,
-                                           const float* K, Counters& cnt) {
+__device__ __forceinline__ float geom_eval(VPtr v, int i, const Pix& px, float depth,
+                                           K4<float>* K, Counters& cnt) {
   cnt.geo_eval++;
   const __attribute__((address_space(1))) float* P = G(v->poses) + i * PM_POSE_STRIDE + 16;
   const __attribute__((address_space(1))) float* iP = G(v->poses) + i * PM_POSE_STRIDE + 28;
@@ -325,7 +333,7 @@ __device__ __forceinline__ float geom_eval(const ViewDev* __restrict__ v, int i,
   const float inv_fz = 1.0f / (P[8] * fx + P[9] * fy + P[10] * fz + P[11]);
   float sc = inv_fz * (P[0] * fx + P[1] * fy + P[2] * fz + P[3]);
   float sr = inv_fz * (P[4] * fx + P[5] * fy + P[6] * fz + P[7]);
-  const SrcDev& s = v->src[i];
+  K4<SrcDev>& s = v->src[i];
   float u = fminf(fmaxf(sc, -3.0f), (float)s.width + 2.0f);
   float w_ = fminf(fmaxf(sr, -3.0f), (float)s.height + 2.0f);
   const float x0f = floorf(u), y0f = floorf(w_);
@@ -350,9 +358,9 @@ __device__ __forceinline__ float geom_eval(const ViewDev* __restrict__ v, int i,
 
 // ComputeMultiViewCostVector (kernel_functions.cu:1447-1495), priors off.
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ float mv_cost(const ViewDev* __restrict__ v, int i, const Pix& px, const uint8_t* tile,
+__device__ __forceinline__ float mv_cost(VPtr v, int i, const Pix& px, const uint8_t* tile,
                                          int tw, const float* ct, const float* sp, float nx, float ny, float nz,
-                                         float d, const float* Kgeo, Counters& cnt) {
+                                         float d, K4<float>* Kgeo, Counters& cnt) {
   float cost = ncc_eval<R_, S_, STORE, RCP>(v, i, px, tile, tw, ct, sp, nx, ny, nz, d, cnt);
   if (v->geom) cost = cost + v->geom_regularizer * geom_eval(v, i, px, d, Kgeo, cnt);
   return cost;
@@ -446,7 +454,7 @@ __device__ __forceinline__ void correct_normal(float n[3], const float vd[3], fl
   n[2] = r2;
 }
 
-__device__ __forceinline__ float propagate_depth(float d1, const float n1[3], const float* iK, int row1, int col1,
+__device__ __forceinline__ float propagate_depth(float d1, const float n1[3], K4<float>* iK, int row1, int col1,
                                                  float np0, float np1) {
   const float x1 = d1 * (iK[0] * (float)col1 + iK[1]);
   const float y1 = d1 * (iK[2] * (float)row1 + iK[3]);
@@ -468,11 +476,12 @@ struct SpSize {
 
 template <int R_>
 struct TileSize {
-  // largest (rows + 2R) x (cols + 2R) over the tile shapes
+  // The staged ref tile has one row pitch for every tile shape (the widest
+  // tile plus the halo), so window offsets in LDS are compile-time
+  // constants; rows: the tallest tile plus the halo.
   static constexpr int kR = R_ >= 0 ? R_ : PM_MAX_WINDOW_RADIUS;
-  static constexpr int shape(int lw) { return ((kTilePixels >> lw) + 2 * kR) * ((1 << lw) + 2 * kR); }
-  static constexpr int kBytes = shape(4) > shape(5) ? (shape(4) > shape(6) ? shape(4) : shape(6))
-                                                    : (shape(5) > shape(6) ? shape(5) : shape(6));
+  static constexpr int kPitch = (1 << kTileLog2WMax) + 2 * kR;
+  static constexpr int kBytes = ((kTilePixels >> kTileLog2WMin) + 2 * kR) * kPitch;
 };
 
 // Stage the colour table and the ref tile (zero outside the image) into
@@ -504,12 +513,14 @@ __device__ __forceinline__ int claim_tile(const LaunchTable* __restrict__ lt, in
     *next = tile;
   }
   __syncthreads();
-  return *next;
+  // the same value in every lane: keep it (and everything resolved from it,
+  // the view's parameters and pointers) in scalar registers
+  return __builtin_amdgcn_readfirstlane(*next);
 }
 
 // The launch entry (view, colour, pass) a tile belongs to.
 struct Ctx {
-  const ViewDev* v;
+  VPtr v;
   int color;
   uint32_t pass;
   float th_mc;
@@ -528,7 +539,7 @@ __device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt, int t
     else hi = mid - 1;
   }
   Ctx c;
-  c.v = g->e[lo].v;
+  c.v = (VPtr)g->e[lo].v;
   c.color = g->e[lo].color;
   c.pass = g->e[lo].pass;
   c.th_mc = g->e[lo].th_mc;
@@ -542,9 +553,9 @@ __device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt, int t
 // Stage the colour table, the spatial weights and the ref tile (zero outside
 // the image) into LDS, then map the thread to its pixel.  Returns false for
 // threads whose pixel lies outside the image (after the barrier).
-__device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, float* ct, float* spl, Pix& px,
-                                           int& tw) {
-  const ViewDev* __restrict__ v = cx.v;
+__device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, int pitch, uint8_t* tile, float* ct, float* spl,
+                                           Pix& px) {
+  const VPtr v = cx.v;
   const int t = threadIdx.x;
   ct[t] = G(v->ct)[t];
   const int nsp = v->nwin * v->nwin;
@@ -555,11 +566,12 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, 
   const int ty = cx.local_block / tiles_x;
   const int tx = cx.local_block - ty * tiles_x;
   const int row0 = ty * (kTilePixels >> lw), col0 = tx << lw;
-  tw = (1 << lw) + 2 * R;
+  const int tw = (1 << lw) + 2 * R;
   const int th = (kTilePixels >> lw) + 2 * R;
   for (int e = t; e < tw * th; e += kSweepThreads) {
-    const int rr = row0 - R + e / tw, cc = col0 - R + e % tw;
-    tile[e] = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? (uint8_t)0 : G(v->ref)[rr * W + cc];
+    const int er = e / tw, ec = e - er * tw;
+    const int rr = row0 - R + er, cc = col0 - R + ec;
+    tile[er * pitch + ec] = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? (uint8_t)0 : G(v->ref)[rr * W + cc];
   }
   __syncthreads();
   px.lrow = t >> (lw - 1);
@@ -573,7 +585,7 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, uint8_t* tile, 
   px.var_r = m.y;
   px.inv_sw = m.z;
   px.err_r = m.w;
-  px.c = tile[(px.lrow + R) * tw + px.lcol + R];
+  px.c = tile[(px.lrow + R) * pitch + px.lcol + R];
   px.shifted = (v->accum_mode == PM_ACCUM_COMPENSATED) && (px.var_r < kLowTexVar);
   px.np0 = v->refInvK[0] * (float)px.col + v->refInvK[1];
   px.np1 = v->refInvK[2] * (float)px.row + v->refInvK[3];
@@ -589,12 +601,12 @@ __device__ __forceinline__ int slot_index() { return blockIdx.x * kSweepThreads 
 
 template <int R_, int S_, int STORE, int RCP>
 __device__ __forceinline__ void init_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
-  const ViewDev* __restrict__ v = cx.v;
+  const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
-  int tw;
-  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
+  const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
+  if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int N = v->N;
   const int plane = v->W * v->H;
@@ -688,18 +700,18 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __res
 
 template <int R_, int S_, int STORE, int RCP>
 __device__ __forceinline__ void sweep_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
-  const ViewDev* __restrict__ v = cx.v;
+  const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const float th_mc = cx.th_mc;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
-  int tw;
-  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
+  const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
+  if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
-  const float* iK = v->refInvK;
+  K4<float>* iK = v->refInvK;
   Ws ws{cx.ws, cx.ws_slots, slot_index(), N};
   Counters cnt;
 
@@ -952,12 +964,12 @@ __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const Laun
 
 template <int R_, int S_, int STORE, int RCP>
 __device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
-  const ViewDev* __restrict__ v = cx.v;
+  const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
-  int tw;
-  if (!setup_pass(cx, R, tile, ct, spl, px, tw)) return;
+  const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
+  if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
   const uint8_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
@@ -1034,7 +1046,7 @@ __device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint8_t* tile, flo
   G(v->normal)[p] = bn[0];
   G(v->normal)[plane + p] = bn[1];
   G(v->normal)[2 * plane + p] = bn[2];
-  const float* Kgeo = v->compat_lo_geo_bug ? v->refInvK : v->refK;
+  K4<float>* Kgeo = v->compat_lo_geo_bug ? v->refInvK : v->refK;
   for (int i = 0; i < N; ++i) {
     float cv;
     if ((sel >> i) & 1u) {
@@ -1061,7 +1073,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* 
 // FilterDepthMap (kernel_functions.cu:3282-3354)
 
 __device__ __forceinline__ void filter_tile(const Ctx& cx) {
-  const ViewDev* __restrict__ v = cx.v;
+  const VPtr v = cx.v;
   const int t = threadIdx.x;
   const int W = v->W, H = v->H;
   const int lw = cx.lw;
@@ -1073,7 +1085,7 @@ __device__ __forceinline__ void filter_tile(const Ctx& cx) {
   if (row >= H || col >= W) return;
   const int plane = W * H;
   const int p = row * W + col;
-  const float* iK = v->refInvK;
+  K4<float>* iK = v->refInvK;
   const float depth = G(v->depth)[p];
   if (depth <= 0.0f) return;
   const float n[3] = {G(v->normal)[p], G(v->normal)[plane + p], G(v->normal)[2 * plane + p]};
@@ -1341,6 +1353,7 @@ int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_c
     }
     n_cu = ic->second;
   }
+  if (per_cu > kMaxBlocksPerCU) per_cu = kMaxBlocksPerCU;
   if (blocks_per_cu > 0 && blocks_per_cu < per_cu) per_cu = blocks_per_cu;
   const long long g = (long long)n_cu * per_cu;
   return (int)(g < tiles ? g : tiles);

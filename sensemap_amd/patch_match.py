@@ -131,7 +131,7 @@ class ViewDesc:
         for k, idx in enumerate(problem.src_image_idxs):
             b = np.ascontiguousarray(imgs[idx].bitmap, dtype=np.uint8)
             self._keep.append(b)
-            srcs[k] = self._image(b)
+            srcs[k] = self._image(b, cache_id=int(idx) + 1)
             cams[k] = _camera(imgs[idx])
         self._keep += [srcs, cams]
         d.src = C.cast(srcs, C.POINTER(abi.pm_image))
@@ -161,8 +161,11 @@ class ViewDesc:
         self.num_src = n
 
     @staticmethod
-    def _image(b: np.ndarray) -> abi.pm_image:
+    def _image(b: np.ndarray, cache_id: int = 0) -> abi.pm_image:
+        """cache_id > 0: the image's key in a serving loop's source cache
+        (here the Problem's image index + 1)."""
         im = abi.pm_image()
+        im.cache_id = int(cache_id)
         im.data = b.ctypes.data_as(C.POINTER(C.c_uint8))
         im.height, im.width = int(b.shape[0]), int(b.shape[1])
         im.stride = int(b.strides[0])
@@ -354,6 +357,86 @@ class PatchMatchBatch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.pm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServerJob:
+    """One depth-map job of a PatchMatchServer: owns the view's input buffers
+    and the output maps the server writes."""
+
+    def __init__(self, server: "PatchMatchServer", options: PatchMatchOptions, problem: Problem):
+        self.server = server
+        self.options = options
+        self.desc = ViewDesc(problem, options)
+        W, H, N = self.desc.width, self.desc.height, self.desc.num_src
+        self.depth = np.empty((H, W), np.float32)
+        self.normal = np.empty((3, H, W), np.float32)
+        self.conf = np.empty((H, W), np.float32)
+        self.cost = np.empty((N, H, W), np.float32)
+        self.view_sel = np.empty((H, W), np.uint32)
+        f = C.POINTER(C.c_float)
+        self._out = abi.pm_outputs(self.depth.ctypes.data_as(f), self.normal.ctypes.data_as(f),
+                                   self.conf.ctypes.data_as(f), self.cost.ctypes.data_as(f),
+                                   self.view_sel.ctypes.data_as(C.POINTER(C.c_uint32)))
+        self._copt = options.to_c()
+        t = C.c_uint64()
+        abi.check(server.lib.pm_server_submit(server.handle, C.byref(self._copt), C.byref(self.desc.desc),
+                                              C.byref(self._out), C.byref(t)))
+        self.ticket = t.value
+        self.stats: Optional[abi.pm_job_stats] = None
+
+    def done(self) -> bool:
+        d = C.c_int32()
+        abi.check(self.server.lib.pm_server_poll(self.server.handle, self.ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self) -> abi.pm_job_stats:
+        """Block until the maps are written; raises if the job failed."""
+        if self.stats is None:
+            st = abi.pm_job_stats()
+            rc = self.server.lib.pm_server_wait(self.server.handle, self.ticket, C.byref(st))
+            self.stats = st
+            abi.check(rc)
+        return self.stats
+
+    def maps(self) -> dict:
+        self.wait()
+        return {"depth": self.depth, "normal": self.normal, "conf": self.conf, "cost": self.cost,
+                "view_sel": self.view_sel}
+
+
+class PatchMatchServer:
+    """Serving loop (pm_server_*): jobs admitted under an HBM budget, source
+    images cached on the device, phases of all admitted jobs batched into
+    shared launches.  Uses the engine's launch shapes."""
+
+    def __init__(self, engine: Optional[Engine] = None, hbm_budget_bytes: int = 16 << 30,
+                 max_active_views: int = 64):
+        self.engine = engine or default_engine()
+        self.lib = self.engine.lib
+        cfg = abi.pm_server_config(int(hbm_budget_bytes), int(max_active_views), 0)
+        h = C.c_void_p()
+        abi.check(self.lib.pm_server_create(self.engine.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+
+    def submit(self, options: PatchMatchOptions, problem: Problem) -> ServerJob:
+        return ServerJob(self, options, problem)
+
+    def stats(self) -> abi.pm_server_stats:
+        s = abi.pm_server_stats()
+        abi.check(self.lib.pm_server_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def close(self) -> None:
+        """Finish every submitted job, then free the server."""
+        if getattr(self, "handle", None):
+            self.lib.pm_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -37,7 +37,7 @@ def test_exports_every_declared_symbol():
 def test_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     structs = ["pm_options", "pm_camera", "pm_image", "pm_view_desc", "pm_outputs", "pm_run_stats", "pm_tuning",
-               "pm_tune_report"]
+               "pm_tune_report", "pm_server_config", "pm_job_stats", "pm_server_stats"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for s in structs:
         lines.append(f'  printf("{s} %zu\\n", sizeof({s}));')
