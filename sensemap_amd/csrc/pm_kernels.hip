@@ -141,21 +141,38 @@ __device__ __forceinline__ float rcp_z(float z) {
   return r;
 }
 
-// Fetch the 2x2 footprint of source texel (x0, y0) from the quad layout.
-// tex is uniform across the wave (the source loop is uniform), off is the
-// 32-bit texel index, so the load uses the scalar-base + 32-bit-offset form.
+// The 2x2 footprint of source texel (x0, y0) in the quad layout: one
+// 4-byte (u8) or 8-byte (f16) load.  tex is uniform across the wave (the
+// source loop is uniform) and off is the 32-bit texel index, so the load
+// uses the scalar-base + 32-bit-offset form.
 template <int STORE>
-__device__ __forceinline__ void fetch_quad(const void* tex, uint32_t off, float& t0, float& t1, float& t2,
-                                           float& t3) {
+struct Quad {
+  using type = uint32_t;
+};
+template <>
+struct Quad<PM_STORAGE_F16> {
+  using type = u2v;
+};
+
+template <int STORE>
+__device__ __forceinline__ typename Quad<STORE>::type load_quad(const void* tex, uint32_t off) {
   const __attribute__((address_space(1))) char* b = G((const char*)tex);
   if constexpr (STORE == PM_STORAGE_U8) {
-    const uint32_t q = *(const __attribute__((address_space(1))) uint32_t*)(b + (off << 2));
+    return *(const __attribute__((address_space(1))) uint32_t*)(b + (off << 2));
+  } else {
+    return *(const __attribute__((address_space(1))) u2v*)(b + (off << 3));
+  }
+}
+
+template <int STORE>
+__device__ __forceinline__ void unpack_quad(typename Quad<STORE>::type q, float& t0, float& t1, float& t2,
+                                            float& t3) {
+  if constexpr (STORE == PM_STORAGE_U8) {
     t0 = (float)(q & 0xffu);
     t1 = (float)((q >> 8) & 0xffu);
     t2 = (float)((q >> 16) & 0xffu);
     t3 = (float)(q >> 24);
   } else {
-    const u2v q = *(const __attribute__((address_space(1))) u2v*)(b + (off << 3));
     t0 = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
     t1 = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
     t2 = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
@@ -244,27 +261,41 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   float e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
   const float mu_r = px.mu_r;
   const uint32_t upitch = (uint32_t)pitch;
-  // One window walk; SHIFTED selects the compensated moment update.
+  // One window walk; SHIFTED selects the compensated moment update.  A
+  // sample is issued (projection, fraction, texel load) ahead of its use:
+  // with compile-time window shapes each row keeps kDepth loads in flight,
+  // so a wave waits for memory once per row instead of once per sample.
+  // Issue order, arithmetic and accumulation order are those of a plain
+  // sample-by-sample walk.
+  struct Smp {
+    float fx, fy;
+    typename Quad<STORE>::type q;
+  };
+  auto issue = [&](float x, float y, float z) __attribute__((always_inline)) {
+    const float rz = rcp_z<RCP>(z);
+    float u = x * rz, w_ = y * rz;
+    u = fminf(fmaxf(u, -3.0f), umax);
+    w_ = fminf(fmaxf(w_, -3.0f), vmax);
+    Smp sm;
+    // fraction = min(u - floor(u), 1 - 2^-24): v_fract_f32
+    sm.fx = __builtin_amdgcn_fractf(u);
+    sm.fy = __builtin_amdgcn_fractf(w_);
+    const int x0 = cvt_flr(u), y0 = cvt_flr(w_);
+    const uint32_t off = __umul24((uint32_t)(y0 + kQuadPad), upitch) + (uint32_t)(x0 + kQuadPad);
+    sm.q = load_quad<STORE>(tex, off);
+    return sm;
+  };
   auto walk = [&](auto shifted_tag) __attribute__((always_inline)) {
     constexpr bool kShifted = decltype(shifted_tag)::value;
     for (int ir = 0; ir < n; ++ir) {
-      float x = bx, y = by, z = bz;
       const uint8_t* trow = tile + ir * S * tw;
       const float* sprow = sp + ir * n;
-      auto sample = [&](int j) __attribute__((always_inline)) {
-        const float rz = rcp_z<RCP>(z);
-        float u = x * rz, w_ = y * rz;
-        u = fminf(fmaxf(u, -3.0f), umax);
-        w_ = fminf(fmaxf(w_, -3.0f), vmax);
-        // fraction = min(u - floor(u), 1 - 2^-24): v_fract_f32
-        const float fx = __builtin_amdgcn_fractf(u), fy = __builtin_amdgcn_fractf(w_);
-        const int x0 = cvt_flr(u), y0 = cvt_flr(w_);
-        const uint32_t off = __umul24((uint32_t)(y0 + kQuadPad), upitch) + (uint32_t)(x0 + kQuadPad);
+      auto consume = [&](int j, const Smp& sm) __attribute__((always_inline)) {
         float t0, t1, t2, t3;
-        fetch_quad<STORE>(tex, off, t0, t1, t2, t3);
-        const float top = fmaf(fx, t1 - t0, t0);
-        const float bot = fmaf(fx, t3 - t2, t2);
-        const float sv = fmaf(fy, bot - top, top);
+        unpack_quad<STORE>(sm.q, t0, t1, t2, t3);
+        const float top = fmaf(sm.fx, t1 - t0, t0);
+        const float bot = fmaf(sm.fx, t3 - t2, t2);
+        const float sv = fmaf(sm.fy, bot - top, top);
         const uint32_t ri = trow[j * S];
         const float r = (float)ri;
         const float w = sprow[j] * ct[__builtin_amdgcn_sad_u8((uint32_t)c, ri, 0u)];
@@ -281,16 +312,38 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
           neumaier_add(s2, e2, wd * dd);
           neumaier_add(s3, e3, wd * rd);
         }
-        x = x + hxc;
-        y = y + hyc;
-        z = z + hzc;
       };
+      float x = bx, y = by, z = bz;  // the next sample to issue
       if constexpr (R_ >= 0) {
         constexpr int kN = R_ >= 0 ? (2 * R_) / S_ + 1 : 1;
+        constexpr int kDepth = kN < 3 ? kN : 3;
+        Smp ring[kDepth];
 #pragma unroll
-        for (int j = 0; j < kN; ++j) sample(j);
+        for (int j = 0; j < kDepth; ++j) {
+          ring[j] = issue(x, y, z);
+          x = x + hxc;
+          y = y + hyc;
+          z = z + hzc;
+        }
+#pragma unroll
+        for (int j = 0; j < kN; ++j) {
+          const Smp cur = ring[j % kDepth];
+          if (j + kDepth < kN) {
+            ring[j % kDepth] = issue(x, y, z);
+            x = x + hxc;
+            y = y + hyc;
+            z = z + hzc;
+          }
+          consume(j, cur);
+        }
       } else {
-        for (int j = 0; j < n; ++j) sample(j);
+        for (int j = 0; j < n; ++j) {
+          const Smp cur = issue(x, y, z);
+          x = x + hxc;
+          y = y + hyc;
+          z = z + hzc;
+          consume(j, cur);
+        }
       }
       bx = bx + hxr;
       by = by + hyr;
