@@ -336,6 +336,29 @@ pm_status pm_server_get_stats(pm_server* server, pm_server_stats* out);
 /* Finishes every submitted job, then frees the server. */
 void pm_server_destroy(pm_server* server);
 
+/* ---- on-disk map formats ----
+ * Mat<T>::Write / Read (util/mat.h:178-215): a text header "W&H&D&", then
+ * width*height*depth values of elem_size bytes, little-endian, slice-major
+ * (DepthMap, conf and cost maps use it as is). */
+pm_status pm_mat_write(const char* path, int32_t width, int32_t height, int32_t depth, int32_t elem_size,
+                       const void* data);
+pm_status pm_mat_read_header(const char* path, int32_t* width, int32_t* height, int32_t* depth);
+/* The header must match width, height and depth. */
+pm_status pm_mat_read(const char* path, int32_t width, int32_t height, int32_t depth, int32_t elem_size,
+                      void* data);
+/* NormalMap::Encode / Decode (normal_map.cc:133-176): 3 planar float
+ * slices of `pixels` values <-> one uint32 per pixel.  NormalMap::Write is
+ * pm_normal_encode + pm_mat_write(..., 1, 4, codes). */
+pm_status pm_normal_encode(int64_t pixels, const float* normal, uint32_t* codes);
+pm_status pm_normal_decode(int64_t pixels, const uint32_t* codes, float* normal);
+/* The problem's normal map encoded on the device (same bits as
+ * pm_normal_encode of pm_problem_get_normal_map). */
+pm_status pm_problem_get_normal_codes(const pm_problem* problem, uint32_t* out);
+/* DepthMap::Write and NormalMap::Write of the problem's current maps, as
+ * PatchMatchController::ProcessProblem stores them
+ * (patch_match_controller.cc:4706, 4719); either path may be null. */
+pm_status pm_problem_write_maps(const pm_problem* problem, const char* depth_path, const char* normal_path);
+
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,
                       const pm_view_desc* view, const pm_outputs* outputs);

@@ -204,6 +204,14 @@ _SIGNATURES = {
     "pm_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(pm_job_stats)]),
     "pm_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_server_stats)]),
     "pm_server_destroy": (None, [C.c_void_p]),
+    "pm_mat_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "pm_mat_read_header": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
+    "pm_mat_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "pm_normal_encode": (C.c_int, [C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
+    "pm_normal_decode": (C.c_int, [C.c_int64, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "pm_problem_get_normal_codes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pm_problem_write_maps": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

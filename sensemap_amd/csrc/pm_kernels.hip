@@ -195,6 +195,11 @@ __device__ __forceinline__ void neumaier_add(float& s, float& c, float x) {
   s = t;
 }
 
+// Texel loads a window row keeps in flight (build knob).
+#ifndef PM_PIPE_DEPTH
+#define PM_PIPE_DEPTH 3
+#endif
+
 // ComputeNCCCost (kernel_functions.cu:1245-1340) for source i (uniform) and
 // the lane's hypothesis.  tile points at the LDS ref tile entry of the
 // window's top-left sample; tw is the tile row length.
@@ -316,7 +321,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
       float x = bx, y = by, z = bz;  // the next sample to issue
       if constexpr (R_ >= 0) {
         constexpr int kN = R_ >= 0 ? (2 * R_) / S_ + 1 : 1;
-        constexpr int kDepth = kN < 3 ? kN : 3;
+        constexpr int kDepth = kN < PM_PIPE_DEPTH ? kN : PM_PIPE_DEPTH;
         Smp ring[kDepth];
 #pragma unroll
         for (int j = 0; j < kDepth; ++j) {

@@ -287,6 +287,15 @@ class PatchMatchACMMCuda:
         """Source `src_index`'s depth map from another problem's current depth map (on device)."""
         abi.check(self.lib.pm_problem_set_src_depth(self.handle, int(src_index), other.handle))
 
+    def GetNormalCodes(self) -> np.ndarray:
+        """The normal map in NormalMap's uint32 file encoding, encoded on the device."""
+        return self._get(self.lib.pm_problem_get_normal_codes, (self.H, self.W), np.uint32)
+
+    def WriteMaps(self, depth_path: Optional[str] = None, normal_path: Optional[str] = None) -> None:
+        """DepthMap::Write / NormalMap::Write of the current maps."""
+        abi.check(self.lib.pm_problem_write_maps(self.handle, os.fsencode(depth_path) if depth_path else None,
+                                                 os.fsencode(normal_path) if normal_path else None))
+
     def Autotune(self) -> abi.pm_tune_report:
         """Time every launch shape on this view and keep the fastest in the
         engine (pm_problem_autotune); the maps are undefined until the next Run()."""
