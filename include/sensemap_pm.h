@@ -359,6 +359,42 @@ pm_status pm_problem_get_normal_codes(const pm_problem* problem, uint32_t* out);
  * (patch_match_controller.cc:4706, 4719); either path may be null. */
 pm_status pm_problem_write_maps(const pm_problem* problem, const char* depth_path, const char* normal_path);
 
+/* ---- cross-view depth filter ----
+ * CrossFilterCuda::Run (patch_match_cuda.cu:1493-1551) with
+ * CrossFilterKernel (kernel_functions.cu:3561-3778): every view's depth map
+ * is checked against the depth and normal maps of its extended sources
+ * (Problem::src_image_extend_idxs).  A pixel keeps a depth when at least
+ * min(filter_min_num_consistent, num_src) sources reproject it within
+ * filter_geom_consistency_max_cost pixels, with a relative depth difference
+ * below depth_diff_threshold and normals within max_normal_error degrees;
+ * the kept depth is the reference's median of the consistent depths.  All
+ * views are filtered in one device pass. */
+typedef struct pm_cross_options {
+  float filter_geom_consistency_max_cost;  /* 1.0 (patch_match_options.h:139) */
+  float max_normal_error;                  /* degrees, 15 (:141) */
+  float depth_diff_threshold;              /* 0.02 (:231) */
+  int32_t filter_min_num_consistent;       /* 1 (:135) */
+} pm_cross_options;
+
+typedef struct pm_cross_image {
+  pm_camera cam;
+  int32_t width;
+  int32_t height;
+  const float* depth;            /* width*height, 0 = no depth */
+  const uint32_t* normal_codes;  /* width*height, NormalMap encoding */
+} pm_cross_image;
+
+typedef struct pm_cross_view {
+  int32_t ref;          /* index into the images */
+  int32_t num_src;      /* 1..31 */
+  const int32_t* src;   /* indices into the images */
+  float* refined;       /* out: the ref image's width*height refined depth map */
+} pm_cross_view;
+
+pm_status pm_cross_options_init(pm_cross_options* options);
+pm_status pm_cross_filter(pm_engine* engine, const pm_cross_options* options, int32_t num_images,
+                          const pm_cross_image* images, int32_t num_views, const pm_cross_view* views);
+
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,
                       const pm_view_desc* view, const pm_outputs* outputs);

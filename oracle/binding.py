@@ -46,6 +46,9 @@ def load():
         lib.pmo_cost_maps.restype = C.c_int
         lib.pmo_cost_maps.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(C.c_float), P(C.c_float), C.c_int,
                                       P(C.c_float), P(C.c_double)]
+        lib.pmo_cross_filter.restype = C.c_int
+        lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
+                                         P(abi.pm_cross_view), C.c_int]
         lib.pmo_geom_cost.restype = C.c_float
         lib.pmo_geom_cost.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, C.c_float, C.c_int]
         for name in ("pmo_expf", "pmo_sinf", "pmo_cosf", "pmo_acosf"):
@@ -107,3 +110,15 @@ def cost_maps(options, view, depth, normal, threads=0):
     if rc != 0:
         raise RuntimeError("pmo_cost_maps failed")
     return c32, c64
+
+
+def cross_filter(images, views, options=None, threads=0):
+    """The transcription of the cross-view filter, same inputs as
+    sensemap_amd.cross_filter.cross_filter."""
+    from sensemap_amd.cross_filter import CrossFilterOptions, CrossInputs
+    lib = load()
+    opt = (options or CrossFilterOptions()).to_c()
+    inp = CrossInputs(images, views)
+    if lib.pmo_cross_filter(C.byref(opt), len(images), inp.images, len(views), inp.views, int(threads)) != 0:
+        raise RuntimeError("pmo_cross_filter failed")
+    return inp.outputs

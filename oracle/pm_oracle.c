@@ -1294,6 +1294,138 @@ int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* CrossFilterKernel (kernel_functions.cu:3561-3778) for one pixel of one
+ * view, and CrossFilterCuda::Run (patch_match_cuda.cu:1493-1551) over the
+ * views.  Sampling of the layered depth/normal textures at (x + 0.5, y + 0.5)
+ * with point filtering and a zero border (patch_match_cuda.cu:1186-1196) is
+ * texel (floor(x + 0.5), floor(y + 0.5)) inside the image, 0 outside. */
+static void cross_decode(uint32_t code, float n[3]) {
+  /* float q1 = iq1 / decode_factor - 1 (kernel_functions.cu:3597) */
+  const float q1 = (float)(code >> 16) / 32767.0f - 1.0f;
+  const float q2 = (float)(code & 0xffffu) / 32767.0f - 1.0f;
+  const float s = q1 * q1 + q2 * q2;
+  n[0] = (float)(1.0 - 2.0 * (double)s);
+  const float r = sqrtf(1.0f - s);
+  n[1] = (2.0f * q1) * r;
+  n[2] = (2.0f * q2) * r;
+}
+
+static float cross_cos_deg(double deg) { return (float)cos((double)(float)(deg * 0.0174532925199432)); }
+
+static float cross_pixel(const pm_cross_options* o, const pm_cross_image* images, const pm_cross_view* cv,
+                         const float* poses, const float k8[8], int min_consistent, int row, int col) {
+  const pm_cross_image* ref = &images[cv->ref];
+  const int W = ref->width, H = ref->height;
+  const float depth = ref->depth[(size_t)row * W + col];
+  if ((double)depth < 1e-6 || col < 3 || col > W - 3 || row < 3 || row > H - 3) return 0.0f;
+  float nr[3];
+  cross_decode(ref->normal_codes[(size_t)row * W + col], nr);
+  const float* iK = k8 + 4;
+  const float fcol = (float)col, frow = (float)row;
+  float vr[3] = {iK[0] * fcol + iK[1], iK[2] * frow + iK[3], 1.0f};
+  const float nv = sqrtf(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+  vr[0] = vr[0] / nv;
+  vr[1] = vr[1] / nv;
+  vr[2] = vr[2] / nv;
+  if (nr[0] * vr[0] + nr[1] * vr[1] + nr[2] * vr[2] >= cross_cos_deg(100.0)) return 0.0f;
+  const float fp[3] = {depth * (iK[0] * fcol + iK[1]), depth * (iK[2] * frow + iK[3]), depth};
+  const float sqr_thr = o->depth_diff_threshold * o->depth_diff_threshold;
+  const float max_cos = cross_cos_deg((double)o->max_normal_error);
+  float depths[32];
+  depths[0] = depth;
+  int ng = 0;
+  for (int i = 0; i < cv->num_src; ++i) {
+    const float* pose = poses + (size_t)i * PM_POSE_STRIDE;
+    const float* P = pose + 16;
+    const float* iP = pose + 28;
+    const float* R = pose + 4;
+    const float inv_fz = 1.0f / (P[8] * fp[0] + P[9] * fp[1] + P[10] * fp[2] + P[11]);
+    float sc = inv_fz * (P[0] * fp[0] + P[1] * fp[1] + P[2] * fp[2] + P[3]);
+    float sr = inv_fz * (P[4] * fp[0] + P[5] * fp[1] + P[6] * fp[2] + P[7]);
+    const pm_cross_image* si = &images[cv->src[i]];
+    const float xf = sc + 0.5f, yf = sr + 0.5f;
+    float sd = 0.0f;
+    uint32_t scode = 0u;
+    if (xf >= 0.0f && xf < (float)si->width && yf >= 0.0f && yf < (float)si->height) {
+      const size_t q = (size_t)(int)floorf(yf) * si->width + (size_t)(int)floorf(xf);
+      sd = si->depth[q];
+      scode = si->normal_codes[q];
+    }
+    if ((double)sd < 1e-4) continue;
+    sc = sc * sd;
+    sr = sr * sd;
+    const float bx = iP[0] * sc + iP[1] * sr + iP[2] * sd + iP[3];
+    const float by = iP[4] * sc + iP[5] * sr + iP[6] * sd + iP[7];
+    const float bz = iP[8] * sc + iP[9] * sr + iP[10] * sd + iP[11];
+    const float inv_bz = 1.0f / bz;
+    const float bcol = inv_bz * (k8[0] * bx + k8[1] * bz);
+    const float brow = inv_bz * (k8[2] * by + k8[3] * bz);
+    const float dc = fcol - bcol, dr = frow - brow;
+    const float reproj = sqrtf(dc * dc + dr * dr);
+    if (reproj > o->filter_geom_consistency_max_cost || (depth - bz) * (depth - bz) > depth * depth * sqr_thr)
+      continue;
+    float sn[3];
+    cross_decode(scode, sn);
+    const float r0 = R[0] * sn[0] + R[3] * sn[1] + R[6] * sn[2];
+    const float r1 = R[1] * sn[0] + R[4] * sn[1] + R[7] * sn[2];
+    const float r2 = R[2] * sn[0] + R[5] * sn[1] + R[8] * sn[2];
+    if (nr[0] * r0 + nr[1] * r1 + nr[2] * r2 < max_cos) continue;
+    ng++;
+    depths[ng] = bz;
+  }
+  if (ng < min_consistent) return 0.0f;
+  /* BubbleSort(depths, num_good_src_depth) (utils.h:206-220), then the
+   * element at int(num_good_src_depth * 0.5) (kernel_functions.cu:3767-3771) */
+  for (int a = 0; a < ng - 1; ++a)
+    for (int b = 0; b < ng - 1 - a; ++b)
+      if (depths[b] > depths[b + 1]) {
+        const float t = depths[b];
+        depths[b] = depths[b + 1];
+        depths[b + 1] = t;
+      }
+  return depths[(int)(ng * 0.5)];
+}
+
+int pmo_cross_filter(const pm_cross_options* o, int num_images, const pm_cross_image* images, int num_views,
+                     const pm_cross_view* views, int num_threads) {
+#ifdef _OPENMP
+  if (num_threads > 0) omp_set_num_threads(num_threads);
+#else
+  (void)num_threads;
+#endif
+  for (int k = 0; k < num_views; ++k) {
+    const pm_cross_view* cv = &views[k];
+    if (cv->ref < 0 || cv->ref >= num_images || cv->num_src < 1 || cv->num_src > 31) return -1;
+    const pm_cross_image* ref = &images[cv->ref];
+    pm_image srcs[31];
+    pm_camera cams[31];
+    memset(srcs, 0, sizeof(srcs));
+    for (int s = 0; s < cv->num_src; ++s) {
+      srcs[s].width = images[cv->src[s]].width;
+      srcs[s].height = images[cv->src[s]].height;
+      cams[s] = images[cv->src[s]].cam;
+    }
+    pm_view_desc vd;
+    memset(&vd, 0, sizeof(vd));
+    vd.num_src = cv->num_src;
+    vd.ref_cam = ref->cam;
+    vd.src = srcs;
+    vd.src_cams = cams;
+    float poses[31 * PM_POSE_STRIDE];
+    float k8[8];
+    pmo_pack_poses(&vd, poses, k8);
+    const int minc = o->filter_min_num_consistent < cv->num_src ? o->filter_min_num_consistent : cv->num_src;
+    const long plane = (long)ref->width * ref->height;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long p = 0; p < plane; ++p) {
+      const int row = (int)(p / ref->width), col = (int)(p % ref->width);
+      cv->refined[p] = cross_pixel(o, images, cv, poses, k8, minc, row, col);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Run(): patch_match_cuda.cu:748-1057 */
 int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, int row_begin,
                  int row_end, int num_threads, pmo_counters* counters) {

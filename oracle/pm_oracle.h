@@ -99,6 +99,9 @@ double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row
  * depth <= 0). */
 int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, const float* normal,
                   int num_threads, float* out32, double* out64);
+/* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
+int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
+                     int num_views, const pm_cross_view* views, int num_threads);
 /* Geometric-consistency term (FP32 canonical), K = ref {fx, cx, fy, cy}. */
 float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, int col,
                     float depth, int src);

@@ -159,6 +159,21 @@ class pm_server_stats(C.Structure):
     ]
 
 
+class pm_cross_options(C.Structure):
+    _fields_ = [("filter_geom_consistency_max_cost", C.c_float), ("max_normal_error", C.c_float),
+                ("depth_diff_threshold", C.c_float), ("filter_min_num_consistent", C.c_int32)]
+
+
+class pm_cross_image(C.Structure):
+    _fields_ = [("cam", pm_camera), ("width", C.c_int32), ("height", C.c_int32),
+                ("depth", C.POINTER(C.c_float)), ("normal_codes", C.POINTER(C.c_uint32))]
+
+
+class pm_cross_view(C.Structure):
+    _fields_ = [("ref", C.c_int32), ("num_src", C.c_int32), ("src", C.POINTER(C.c_int32)),
+                ("refined", C.POINTER(C.c_float))]
+
+
 _SIGNATURES = {
     "pm_options_init": (C.c_int, [C.POINTER(pm_options)]),
     "pm_options_check": (C.c_int, [C.POINTER(pm_options)]),
@@ -212,6 +227,9 @@ _SIGNATURES = {
     "pm_normal_decode": (C.c_int, [C.c_int64, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "pm_problem_get_normal_codes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pm_problem_write_maps": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "pm_cross_options_init": (C.c_int, [C.POINTER(pm_cross_options)]),
+    "pm_cross_filter": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_cross_image),
+                                  C.c_int32, C.POINTER(pm_cross_view)]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

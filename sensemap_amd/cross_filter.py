@@ -1,0 +1,90 @@
+"""Cross-view depth filter (pm_cross_filter): CrossFilterCuda::Run
+(src/mvs/patch_match_cuda.cu:1493-1551) over a set of views, each checked
+against its extended source images' depth and normal maps."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class CrossFilterOptions:
+    """The PatchMatchOptions fields the filter reads (patch_match_options.h:135-141, 231)."""
+    filter_geom_consistency_max_cost: float = 1.0
+    max_normal_error: float = 15.0
+    depth_diff_threshold: float = 0.02
+    filter_min_num_consistent: int = 1
+
+    def to_c(self) -> abi.pm_cross_options:
+        return abi.pm_cross_options(float(self.filter_geom_consistency_max_cost), float(self.max_normal_error),
+                                    float(self.depth_diff_threshold), int(self.filter_min_num_consistent))
+
+
+@dataclass
+class CrossImage:
+    """One image: camera (fx, cx, fy, cy), world->camera R, T; its depth map
+    (H, W) and its normal map, either (3, H, W) floats or (H, W) uint32
+    NormalMap codes."""
+    K: Sequence[float]
+    R: np.ndarray
+    T: np.ndarray
+    depth: np.ndarray
+    normal: np.ndarray
+
+
+class CrossInputs:
+    """Owns the numpy buffers and ctypes arrays behind one call."""
+
+    def __init__(self, images: List[CrossImage], views: List[Tuple[int, Sequence[int]]]):
+        from . import io as pmio
+        self._keep = []
+        n = len(images)
+        self.images = (abi.pm_cross_image * n)()
+        self.shapes = []
+        for i, im in enumerate(images):
+            d = np.ascontiguousarray(im.depth, np.float32)
+            nm = np.asarray(im.normal)
+            codes = np.ascontiguousarray(nm if nm.dtype == np.uint32 and nm.ndim == 2 else pmio.encode_normals(nm))
+            self._keep += [d, codes]
+            c = abi.pm_camera()
+            for k in range(4):
+                c.K[k] = float(im.K[k])
+            R = np.asarray(im.R, np.float64).reshape(9)
+            T = np.asarray(im.T, np.float64).reshape(3)
+            for k in range(9):
+                c.R[k] = float(R[k])
+            for k in range(3):
+                c.T[k] = float(T[k])
+            self.images[i].cam = c
+            self.images[i].height, self.images[i].width = d.shape
+            self.images[i].depth = d.ctypes.data_as(C.POINTER(C.c_float))
+            self.images[i].normal_codes = codes.ctypes.data_as(C.POINTER(C.c_uint32))
+            self.shapes.append(d.shape)
+        self.views = (abi.pm_cross_view * len(views))()
+        self.outputs = []
+        for k, (ref, srcs) in enumerate(views):
+            s = (C.c_int32 * len(srcs))(*[int(x) for x in srcs])
+            out = np.zeros(self.shapes[ref], np.float32)
+            self._keep += [s]
+            self.outputs.append(out)
+            self.views[k].ref = int(ref)
+            self.views[k].num_src = len(srcs)
+            self.views[k].src = C.cast(s, C.POINTER(C.c_int32))
+            self.views[k].refined = out.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def cross_filter(images: List[CrossImage], views: List[Tuple[int, Sequence[int]]],
+                 options: Optional[CrossFilterOptions] = None, engine=None) -> List[np.ndarray]:
+    """Refined depth map of every view (ref index, source indices), in one
+    device pass."""
+    from .patch_match import default_engine
+    eng = engine or default_engine()
+    opt = (options or CrossFilterOptions()).to_c()
+    inp = CrossInputs(images, views)
+    abi.check(eng.lib.pm_cross_filter(eng.handle, C.byref(opt), len(images), inp.images, len(views), inp.views))
+    return inp.outputs

@@ -1,0 +1,306 @@
+// pm_cross.hip -- the cross-view depth filter: CrossFilterCuda::Run
+// (src/mvs/patch_match_cuda.cu:1493-1551) and CrossFilterKernel
+// (src/mvs/kernel_functions.cu:3561-3778), all views in one launch.
+//
+// Layout.  Every image used by a view has its depth map and normal codes in
+// HBM once (the reference copies all of them into two layered textures of
+// the largest size).  A table of views (ref maps, output, poses, sources)
+// drives one launch of 16x16-pixel workgroups; each workgroup finds its view
+// by binary search over the views' first tiles.  The pass reads 8 bytes
+// (depth, code) per source per pixel and does about 150 FP32 operations per
+// source: it is bound by memory latency, not by the ALUs.
+#include "pm_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace pmh;
+
+namespace {
+
+constexpr int kCrossMaxSrc = 31;  // MAX_NUM_CROSS_SRC (utils.h:39)
+constexpr int kCrossTile = 16;    // a workgroup covers 16x16 pixels
+
+struct CrossSrc {
+  const float* depth;
+  const uint32_t* codes;
+  int32_t width, height;
+};
+
+struct CrossViewDev {
+  int32_t W, H, N, tile_start;
+  int32_t min_consistent;  // min(filter_min_num_consistent, N)
+  int32_t pad0;
+  float K[4], iK[4];
+  const float* depth;
+  const uint32_t* codes;
+  float* out;
+  const float* poses;  // N * PM_POSE_STRIDE
+  CrossSrc src[kCrossMaxSrc];
+};
+
+struct CrossParams {
+  float max_reproj;      // filter_geom_consistency_max_cost
+  float sqr_depth_thr;   // depth_diff_threshold^2
+  float max_cos_normal;  // cosf(DEG2RAD(max_normal_error))
+  float max_view_cos;    // cosf(DEG2RAD(100))
+  int32_t n_views;
+};
+
+using CV = const __attribute__((address_space(4))) CrossViewDev;
+
+// The decode inside CrossFilterKernel (kernel_functions.cu:3593-3603).
+__device__ __forceinline__ void decode_code(uint32_t code, float n[3]) {
+  const float q1 = (float)(code >> 16) / 32767.0f - 1.0f;
+  const float q2 = (float)(code & 0xffffu) / 32767.0f - 1.0f;
+  const float s = q1 * q1 + q2 * q2;
+  n[0] = (float)(1.0 - 2.0 * (double)s);
+  const float r = sqrtf(1.0f - s);
+  n[1] = (2.0f * q1) * r;
+  n[2] = (2.0f * q2) * r;
+}
+
+__device__ float cross_pixel(CV& v, const CrossParams& prm, int row, int col) {
+  const __attribute__((address_space(1))) float* dmap = (const __attribute__((address_space(1))) float*)v.depth;
+  const int p = row * v.W + col;
+  const float depth = dmap[p];
+  if ((double)depth < 1e-6 || col < 3 || col > v.W - 3 || row < 3 || row > v.H - 3) return 0.0f;
+  float nr[3];
+  decode_code(((const __attribute__((address_space(1))) uint32_t*)v.codes)[p], nr);
+  const float fcol = (float)col, frow = (float)row;
+  float vr0 = v.iK[0] * fcol + v.iK[1];
+  float vr1 = v.iK[2] * frow + v.iK[3];
+  float vr2 = 1.0f;
+  const float nv = sqrtf(vr0 * vr0 + vr1 * vr1 + vr2 * vr2);
+  vr0 = vr0 / nv;
+  vr1 = vr1 / nv;
+  vr2 = vr2 / nv;
+  if (nr[0] * vr0 + nr[1] * vr1 + nr[2] * vr2 >= prm.max_view_cos) return 0.0f;
+  const float fx = depth * (v.iK[0] * fcol + v.iK[1]);
+  const float fy = depth * (v.iK[2] * frow + v.iK[3]);
+  const float fz = depth;
+  float depths[kCrossMaxSrc + 1];
+  depths[0] = depth;
+  int ng = 0;
+  for (int i = 0; i < v.N; ++i) {
+    const __attribute__((address_space(1))) float* pose =
+        (const __attribute__((address_space(1))) float*)v.poses + i * PM_POSE_STRIDE;
+    const __attribute__((address_space(1))) float* P = pose + 16;
+    const __attribute__((address_space(1))) float* iP = pose + 28;
+    const __attribute__((address_space(1))) float* R = pose + 4;
+    const float inv_fz = 1.0f / (P[8] * fx + P[9] * fy + P[10] * fz + P[11]);
+    float sc = inv_fz * (P[0] * fx + P[1] * fy + P[2] * fz + P[3]);
+    float sr = inv_fz * (P[4] * fx + P[5] * fy + P[6] * fz + P[7]);
+    // point sampling at (sc + 0.5, sr + 0.5) with a zero border
+    const float xf = sc + 0.5f, yf = sr + 0.5f;
+    const int sw = v.src[i].width, sh = v.src[i].height;
+    float sd = 0.0f;
+    uint32_t scode = 0u;
+    if (xf >= 0.0f && xf < (float)sw && yf >= 0.0f && yf < (float)sh) {
+      const int q = (int)floorf(yf) * sw + (int)floorf(xf);
+      sd = ((const __attribute__((address_space(1))) float*)v.src[i].depth)[q];
+      scode = ((const __attribute__((address_space(1))) uint32_t*)v.src[i].codes)[q];
+    }
+    if ((double)sd < 1e-4) continue;
+    sc = sc * sd;
+    sr = sr * sd;
+    const float bx = iP[0] * sc + iP[1] * sr + iP[2] * sd + iP[3];
+    const float by = iP[4] * sc + iP[5] * sr + iP[6] * sd + iP[7];
+    const float bz = iP[8] * sc + iP[9] * sr + iP[10] * sd + iP[11];
+    const float inv_bz = 1.0f / bz;
+    const float bcol = inv_bz * (v.K[0] * bx + v.K[1] * bz);
+    const float brow = inv_bz * (v.K[2] * by + v.K[3] * bz);
+    const float dc = fcol - bcol, dr = frow - brow;
+    const float reproj = sqrtf(dc * dc + dr * dr);
+    if (reproj > prm.max_reproj || (depth - bz) * (depth - bz) > depth * depth * prm.sqr_depth_thr) continue;
+    float sn[3];
+    decode_code(scode, sn);
+    const float r0 = R[0] * sn[0] + R[3] * sn[1] + R[6] * sn[2];
+    const float r1 = R[1] * sn[0] + R[4] * sn[1] + R[7] * sn[2];
+    const float r2 = R[2] * sn[0] + R[5] * sn[1] + R[8] * sn[2];
+    if (nr[0] * r0 + nr[1] * r1 + nr[2] * r2 < prm.max_cos_normal) continue;
+    ng++;
+    depths[ng] = bz;
+  }
+  if (ng < v.min_consistent) return 0.0f;
+  // BubbleSort(depths, num_good_src_depth) (utils.h:206-220): the first ng
+  // of the ng + 1 collected depths, as the reference calls it
+  for (int a = 0; a < ng - 1; ++a)
+    for (int b = 0; b < ng - 1 - a; ++b)
+      if (depths[b] > depths[b + 1]) {
+        const float t = depths[b];
+        depths[b] = depths[b + 1];
+        depths[b + 1] = t;
+      }
+  return depths[ng / 2];
+}
+
+__global__ __launch_bounds__(256) void k_cross_filter(const CrossViewDev* __restrict__ views, CrossParams prm) {
+  const int t = blockIdx.x;
+  const __attribute__((address_space(4))) CrossViewDev* tab = (const __attribute__((address_space(4))) CrossViewDev*)views;
+  int lo = 0, hi = prm.n_views - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].tile_start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  CV& v = tab[lo];
+  const int lt = t - v.tile_start;
+  const int tiles_x = (v.W + kCrossTile - 1) / kCrossTile;
+  const int ty = lt / tiles_x, tx = lt - ty * tiles_x;
+  const int col = tx * kCrossTile + (int)(threadIdx.x & (kCrossTile - 1));
+  const int row = ty * kCrossTile + (int)(threadIdx.x / kCrossTile);
+  if (col >= v.W || row >= v.H) return;
+  ((__attribute__((address_space(1))) float*)v.out)[row * v.W + col] = cross_pixel(v, prm, row, col);
+}
+
+// cosf(DEG2RAD(deg)) as the reference forms it: the double product rounded
+// to the float argument of cosf (kernel_functions.h:39).
+float cos_deg(double deg) { return (float)std::cos((double)(float)(deg * 0.0174532925199432)); }
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+PM_API pm_status pm_cross_options_init(pm_cross_options* o) {
+  if (!o) return fail(PM_ERR_INVALID_ARG, "null options");
+  o->filter_geom_consistency_max_cost = 1.0f;
+  o->max_normal_error = 15.0f;
+  o->depth_diff_threshold = 0.02f;
+  o->filter_min_num_consistent = 1;
+  return PM_OK;
+}
+
+PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_t num_images,
+                                 const pm_cross_image* images, int32_t num_views, const pm_cross_view* views) {
+  if (!e || !o || !images || !views || num_images <= 0 || num_views <= 0)
+    return fail(PM_ERR_INVALID_ARG, "invalid cross-filter arguments");
+  if (!(o->filter_geom_consistency_max_cost >= 0.0f) || !(o->depth_diff_threshold >= 0.0f) ||
+      !(o->max_normal_error >= 0.0f) || o->filter_min_num_consistent < 0)
+    return fail(PM_ERR_INVALID_ARG, "invalid cross-filter options");
+  for (int32_t i = 0; i < num_images; ++i) {
+    const pm_cross_image& im = images[i];
+    if (im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > ((int64_t)1 << 28) || !im.depth ||
+        !im.normal_codes || !finite_camera(im.cam))
+      return fail(PM_ERR_INVALID_ARG, "invalid cross-filter image " + std::to_string(i));
+  }
+  for (int32_t k = 0; k < num_views; ++k) {
+    const pm_cross_view& cv = views[k];
+    if (cv.ref < 0 || cv.ref >= num_images || !cv.refined || !cv.src)
+      return fail(PM_ERR_INVALID_ARG, "invalid cross-filter view " + std::to_string(k));
+    if (cv.num_src < 1 || cv.num_src > kCrossMaxSrc)
+      return fail(PM_ERR_TOO_MANY_SOURCES, "cross-filter views take 1..31 sources");
+    for (int32_t s = 0; s < cv.num_src; ++s)
+      if (cv.src[s] < 0 || cv.src[s] >= num_images)
+        return fail(PM_ERR_INVALID_ARG, "cross-filter source index out of range in view " + std::to_string(k));
+  }
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+
+  // images used by some view: depth and codes once each
+  std::vector<int64_t> img_off(num_images, -1);
+  int64_t img_total = 0;
+  auto use = [&](int32_t i) {
+    if (img_off[i] < 0) {
+      img_off[i] = img_total;
+      img_total += (int64_t)images[i].width * images[i].height;
+    }
+  };
+  int64_t out_total = 0;
+  std::vector<int64_t> out_off(num_views);
+  for (int32_t k = 0; k < num_views; ++k) {
+    use(views[k].ref);
+    for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
+    out_off[k] = out_total;
+    out_total += (int64_t)images[views[k].ref].width * images[views[k].ref].height;
+  }
+  DevBuf d_depth, d_codes, d_out, d_poses, d_tab;
+  PM_HIP(hipMalloc(&d_depth.p, (size_t)img_total * sizeof(float)));
+  PM_HIP(hipMalloc(&d_codes.p, (size_t)img_total * sizeof(uint32_t)));
+  PM_HIP(hipMalloc(&d_out.p, (size_t)out_total * sizeof(float)));
+  for (int32_t i = 0; i < num_images; ++i) {
+    if (img_off[i] < 0) continue;
+    const size_t n = (size_t)images[i].width * images[i].height;
+    PM_HIP(hipMemcpyAsync((float*)d_depth.p + img_off[i], images[i].depth, n * sizeof(float), hipMemcpyHostToDevice,
+                          st));
+    PM_HIP(hipMemcpyAsync((uint32_t*)d_codes.p + img_off[i], images[i].normal_codes, n * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, st));
+  }
+
+  // poses and the view table
+  std::vector<float> poses;
+  std::vector<CrossViewDev> tab(num_views);
+  std::vector<size_t> pose_off(num_views);
+  int64_t tiles = 0;
+  for (int32_t k = 0; k < num_views; ++k) {
+    const pm_cross_view& cv = views[k];
+    const pm_cross_image& ref = images[cv.ref];
+    pm_view_desc vd{};
+    std::vector<pm_image> srcs(cv.num_src);
+    std::vector<pm_camera> cams(cv.num_src);
+    for (int32_t s = 0; s < cv.num_src; ++s) {
+      const pm_cross_image& si = images[cv.src[s]];
+      srcs[s].width = si.width;
+      srcs[s].height = si.height;
+      cams[s] = si.cam;
+    }
+    vd.num_src = cv.num_src;
+    vd.ref_cam = ref.cam;
+    vd.src = srcs.data();
+    vd.src_cams = cams.data();
+    float k8[8];
+    pose_off[k] = poses.size();
+    poses.resize(poses.size() + (size_t)cv.num_src * PM_POSE_STRIDE);
+    pack_poses(&vd, poses.data() + pose_off[k], k8);
+    CrossViewDev& t = tab[k];
+    std::memset(&t, 0, sizeof(t));
+    t.W = ref.width;
+    t.H = ref.height;
+    t.N = cv.num_src;
+    t.min_consistent = std::min(o->filter_min_num_consistent, cv.num_src);
+    std::memcpy(t.K, k8, sizeof(t.K));
+    std::memcpy(t.iK, k8 + 4, sizeof(t.iK));
+    t.depth = (const float*)d_depth.p + img_off[cv.ref];
+    t.codes = (const uint32_t*)d_codes.p + img_off[cv.ref];
+    t.out = (float*)d_out.p + out_off[k];
+    for (int32_t s = 0; s < cv.num_src; ++s) {
+      const int32_t i = cv.src[s];
+      t.src[s].depth = (const float*)d_depth.p + img_off[i];
+      t.src[s].codes = (const uint32_t*)d_codes.p + img_off[i];
+      t.src[s].width = images[i].width;
+      t.src[s].height = images[i].height;
+    }
+    t.tile_start = (int32_t)tiles;
+    tiles += (int64_t)((ref.width + kCrossTile - 1) / kCrossTile) * ((ref.height + kCrossTile - 1) / kCrossTile);
+    if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "cross filter: too many pixels in one call");
+  }
+  PM_HIP(hipMalloc(&d_poses.p, std::max<size_t>(poses.size(), 1) * sizeof(float)));
+  PM_HIP(hipMemcpyAsync(d_poses.p, poses.data(), poses.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  for (int32_t k = 0; k < num_views; ++k) tab[k].poses = (const float*)d_poses.p + pose_off[k];
+  PM_HIP(hipMalloc(&d_tab.p, tab.size() * sizeof(CrossViewDev)));
+  PM_HIP(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(CrossViewDev), hipMemcpyHostToDevice, st));
+
+  CrossParams prm;
+  prm.max_reproj = o->filter_geom_consistency_max_cost;
+  prm.sqr_depth_thr = o->depth_diff_threshold * o->depth_diff_threshold;
+  prm.max_cos_normal = cos_deg((double)o->max_normal_error);
+  prm.max_view_cos = cos_deg(100.0);
+  prm.n_views = num_views;
+  hipLaunchKernelGGL(k_cross_filter, dim3((unsigned)tiles), dim3(kCrossTile * kCrossTile), 0, st,
+                     (const CrossViewDev*)d_tab.p, prm);
+  PM_HIP(hipGetLastError());
+  for (int32_t k = 0; k < num_views; ++k) {
+    const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
+    PM_HIP(hipMemcpyAsync(views[k].refined, (const float*)d_out.p + out_off[k], n * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+  }
+  PM_HIP(hipStreamSynchronize(st));
+  return PM_OK;
+}

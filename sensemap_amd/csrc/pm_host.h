@@ -171,6 +171,10 @@ pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N);
 
 // Validation shared by every entry point that takes a view.
 pm_status check_view(const pm_options* o, const pm_view_desc* v);
+bool finite_camera(const pm_camera& c);
+// InitTransforms (patch_match_cuda.cu:493-580): PM_POSE_STRIDE floats per
+// source, FP64 algebra rounded once, and the ref {fx,cx,fy,cy, 1/fx,...}.
+void pack_poses(const pm_view_desc* view, float* poses, float k8[8]);
 
 // Device bytes of one job's memory block (state, tables, reference and the
 // sources not given in `src_cached`), in Arena::round units.
