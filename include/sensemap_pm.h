@@ -80,6 +80,8 @@ typedef struct pm_options {
   int32_t conf_filter;                /* 0: FilterDepthMap runs when filter && conf_filter */
   float conf_threshold;               /* 0.5 */
   float depth_diff_threshold;         /* 0.02 */
+  float filter_geom_consistency_max_cost;  /* 1.0: RunFilter's per-source threshold (patch_match_options.h:139) */
+  int32_t filter_min_num_consistent;  /* 1: RunFilter keeps a depth with this many consistent sources (:135) */
   /* ---- extensions (no counterpart in the reference) ---- */
   int32_t init_samples;               /* 1 = reference semantics; K > 1 keeps the best of K random inits */
   int32_t compat_lo_geo_bug;          /* 1 = reproduce kernel_functions.cu:2924 (ref_invK passed as K) */
@@ -87,6 +89,7 @@ typedef struct pm_options {
   int32_t accum_mode;                 /* PM_ACCUM_PLAIN or PM_ACCUM_COMPENSATED (NCC moments) */
   int32_t rcp_mode;                   /* PM_RCP_EXACT (bitwise with the CPU transcription) or PM_RCP_FAST */
   int32_t reserved0;
+  int32_t reserved1;
   uint64_t seed;                      /* Philox key; streams are keyed by (seed, view_id, pixel, pass) */
 } pm_options;
 
@@ -198,6 +201,17 @@ pm_status pm_problem_get_stats(const pm_problem* problem, pm_run_stats* out);
  * patch_match_cuda.cu:493-580). */
 pm_status pm_problem_get_poses(const pm_problem* problem, float* poses, float ref_k8[8]);
 void pm_problem_destroy(pm_problem* problem);
+
+/* RunFilter (patch_match_cuda.cu:1074-1108): the view's given depth map
+ * (init_depth, loaded as InitWorkspaceFilter does, :701-712) is checked
+ * against the source depth maps with the geometric-consistency cost.
+ * FilterDepthMapsKernel (kernel_functions.cu:3356-3450) zeroes a depth with
+ * fewer than min(filter_min_num_consistent, num_src) sources within
+ * filter_geom_consistency_max_cost; with deduplication,
+ * DeduplicDepthMapsKernel (:3452-3490) zeroes a depth that any source
+ * explains.  Needs a problem created with geom_consistency = 1 and
+ * init_depth_random = 0; the result is its depth map. */
+pm_status pm_problem_run_filter(pm_problem* problem, int32_t deduplication);
 
 /* Device-to-device hand-off between passes (the reference goes through
  * host DepthMap/NormalMap caches, patch_match_controller.cc:4571-4595,

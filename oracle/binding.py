@@ -46,6 +46,8 @@ def load():
         lib.pmo_cost_maps.restype = C.c_int
         lib.pmo_cost_maps.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(C.c_float), P(C.c_float), C.c_int,
                                       P(C.c_float), P(C.c_double)]
+        lib.pmo_run_filter.restype = C.c_int
+        lib.pmo_run_filter.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, P(C.c_float), C.c_int]
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
@@ -122,3 +124,14 @@ def cross_filter(images, views, options=None, threads=0):
     if lib.pmo_cross_filter(C.byref(opt), len(images), inp.images, len(views), inp.views, int(threads)) != 0:
         raise RuntimeError("pmo_cross_filter failed")
     return inp.outputs
+
+
+def run_filter(options, view, dedup=False, threads=0):
+    """RunFilter's transcription on a ViewDesc (its init_depth is the given map)."""
+    lib = load()
+    out = np.zeros((view.height, view.width), np.float32)
+    opt = options.to_c()
+    if lib.pmo_run_filter(C.byref(opt), C.byref(view.desc), int(bool(dedup)),
+                          out.ctypes.data_as(C.POINTER(C.c_float)), int(threads)) != 0:
+        raise RuntimeError("pmo_run_filter failed")
+    return out

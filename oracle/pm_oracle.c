@@ -1294,6 +1294,49 @@ int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* RunFilter (patch_match_cuda.cu:1074-1108): the given depth map
+ * (view->init_depth, InitWorkspaceFilter :701-712) through
+ * FilterDepthMapsKernel (kernel_functions.cu:3356-3450) or, with dedup,
+ * DeduplicDepthMapsKernel (:3452-3490). */
+int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, float* depth_out, int num_threads) {
+#ifdef _OPENMP
+  if (num_threads > 0) omp_set_num_threads(num_threads);
+#else
+  (void)num_threads;
+#endif
+  if (!opt->geom_consistency || !view->init_depth || !view->src_depth_maps) return -1;
+  pmo_view v;
+  uint8_t* rc;
+  uint8_t* sc[PM_MAX_NUM_SRC];
+  if (prepare_view(opt, view, &v, &rc, sc) != 0) return -1;
+  const int W = v.W, N = v.N;
+  const long plane = (long)W * v.H;
+  const int minc = opt->filter_min_num_consistent < N ? opt->filter_min_num_consistent : N;
+  memcpy(depth_out, view->init_depth, sizeof(float) * (size_t)plane);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long p = 0; p < plane; ++p) {
+    const int row = (int)(p / W), col = (int)(p % W);
+    const float d = depth_out[p];
+    if ((double)d < 1e-6) {
+      depth_out[p] = 0.0f;
+      continue;
+    }
+    const float np[3] = {v.refInvK[0] * (float)col + v.refInvK[1], v.refInvK[2] * (float)row + v.refInvK[3], 1.0f};
+    int good = 0, explained = 0;
+    for (int i = 0; i < N; ++i) {
+      if (geom_cost(opt, &v, row, col, d, np, i, v.refK, NULL) <= opt->filter_geom_consistency_max_cost) {
+        explained = 1;
+        if (dedup) break;
+        good++;
+      }
+    }
+    if (dedup ? explained : good < minc) depth_out[p] = 0.0f;
+  }
+  free_view(&v, rc, sc);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* CrossFilterKernel (kernel_functions.cu:3561-3778) for one pixel of one
  * view, and CrossFilterCuda::Run (patch_match_cuda.cu:1493-1551) over the
  * views.  Sampling of the layered depth/normal textures at (x + 0.5, y + 0.5)

@@ -99,6 +99,8 @@ double pmo_ncc_cost_f64(const pm_options* opt, const pm_view_desc* view, int row
  * depth <= 0). */
 int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, const float* normal,
                   int num_threads, float* out32, double* out64);
+/* RunFilter on view->init_depth into depth_out (W*H). */
+int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, float* depth_out, int num_threads);
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);

@@ -52,12 +52,15 @@ class pm_options(C.Structure):
         ("conf_filter", C.c_int32),
         ("conf_threshold", C.c_float),
         ("depth_diff_threshold", C.c_float),
+        ("filter_geom_consistency_max_cost", C.c_float),
+        ("filter_min_num_consistent", C.c_int32),
         ("init_samples", C.c_int32),
         ("compat_lo_geo_bug", C.c_int32),
         ("image_storage", C.c_int32),
         ("accum_mode", C.c_int32),
         ("rcp_mode", C.c_int32),
         ("reserved0", C.c_int32),
+        ("reserved1", C.c_int32),
         ("seed", C.c_uint64),
     ]
 
@@ -227,6 +230,7 @@ _SIGNATURES = {
     "pm_normal_decode": (C.c_int, [C.c_int64, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "pm_problem_get_normal_codes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pm_problem_write_maps": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "pm_problem_run_filter": (C.c_int, [C.c_void_p, C.c_int32]),
     "pm_cross_options_init": (C.c_int, [C.POINTER(pm_cross_options)]),
     "pm_cross_filter": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_cross_image),
                                   C.c_int32, C.POINTER(pm_cross_view)]),

@@ -142,6 +142,9 @@ int phase_grid(PhaseKind kind, const LaunchCfg& cfg, int tiles, int blocks_per_c
 // Workgroups of the phase kernel one CU holds at once.
 int phase_occupancy(PhaseKind kind, const LaunchCfg& cfg);
 hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable* table, int grid, hipStream_t s);
+// RunFilter over the view's current depth map (a 2-D grid of 16x16 pixels).
+hipError_t launch_run_filter(const ViewDev* v, int W, int H, int dedup, float filter_max_cost, int min_consistent,
+                             hipStream_t s);
 // Zero n uint32 words with a kernel (safe inside stream capture).
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 

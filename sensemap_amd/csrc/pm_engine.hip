@@ -124,6 +124,8 @@ PM_API pm_status pm_options_init(pm_options* o) {
   o->conf_filter = 0;
   o->conf_threshold = 0.5f;
   o->depth_diff_threshold = 0.02f;
+  o->filter_geom_consistency_max_cost = 1.0f;
+  o->filter_min_num_consistent = 1;
   o->init_samples = 1;
   o->compat_lo_geo_bug = 1;
   o->image_storage = PM_STORAGE_U8;
@@ -1200,6 +1202,21 @@ PM_API pm_status pm_problem_get_poses(const pm_problem* p, float* poses, float r
   if (!p || !poses || !ref_k8) return fail(PM_ERR_INVALID_ARG, "null argument");
   std::memcpy(poses, p->poses.data(), p->poses.size() * sizeof(float));
   std::memcpy(ref_k8, p->refk8, sizeof(p->refk8));
+  return PM_OK;
+}
+
+PM_API pm_status pm_problem_run_filter(pm_problem* p, int32_t deduplication) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  if (!p->opt.geom_consistency || !p->init_depth)
+    return fail(PM_ERR_INVALID_ARG, "RunFilter needs geom_consistency = 1 and a given depth map (init_depth_random = 0)");
+  PM_HIP(hipSetDevice(p->engine->device));
+  hipStream_t s = p->engine->stream;
+  const size_t plane = (size_t)p->W * p->H;
+  PM_HIP(hipMemcpyAsync(p->host.depth, p->init_depth, plane * sizeof(float), hipMemcpyDeviceToDevice, s));
+  const int min_c = std::min(p->opt.filter_min_num_consistent, p->N);
+  PM_HIP(pm::launch_run_filter(p->dev, p->W, p->H, deduplication ? 1 : 0, p->opt.filter_geom_consistency_max_cost,
+                               min_c, s));
+  PM_HIP(hipStreamSynchronize(s));
   return PM_OK;
 }
 

@@ -1180,6 +1180,43 @@ __global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __r
   for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) filter_tile(resolve(lt, t));
 }
 
+// RunFilter (patch_match_cuda.cu:1074-1108): FilterDepthMapsKernel
+// (kernel_functions.cu:3356-3450) or, with dedup, DeduplicDepthMapsKernel
+// (:3452-3490).  The geometric-consistency cost is the sweep's geometric
+// term (same bilinear source depth and back-projection).  One thread per
+// pixel; each pixel reads and writes only its own depth.
+__global__ __launch_bounds__(256) void k_run_filter(const ViewDev* vd, int dedup, float filter_max_cost,
+                                                    int min_consistent) {
+  const VPtr v = (VPtr)vd;
+  const int col = blockIdx.x * 16 + (int)(threadIdx.x & 15u);
+  const int row = blockIdx.y * 16 + (int)(threadIdx.x >> 4);
+  if (col >= v->W || row >= v->H) return;
+  const int p = row * v->W + col;
+  __attribute__((address_space(1))) float* depth = G(v->depth);
+  const float d = depth[p];
+  if ((double)d < 1e-6) {
+    depth[p] = 0.0f;
+    return;
+  }
+  Pix px;
+  px.row = row;
+  px.col = col;
+  px.np0 = v->refInvK[0] * (float)col + v->refInvK[1];
+  px.np1 = v->refInvK[2] * (float)row + v->refInvK[3];
+  Counters cnt;
+  int good = 0;
+  for (int i = 0; i < v->N; ++i) {
+    if (geom_eval(v, i, px, d, v->refK, cnt) <= filter_max_cost) {
+      if (dedup) {
+        depth[p] = 0.0f;
+        return;
+      }
+      good++;
+    }
+  }
+  if (!dedup && good < min_consistent) depth[p] = 0.0f;
+}
+
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1423,6 +1460,14 @@ hipError_t launch_phase(PhaseKind kind, const LaunchCfg& cfg, const LaunchTable*
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {(void*)&table};
   return hipLaunchKernel(fn, dim3((unsigned)grid), dim3(kSweepThreads), args, 0, s);
+}
+
+hipError_t launch_run_filter(const ViewDev* v, int W, int H, int dedup, float filter_max_cost, int min_consistent,
+                             hipStream_t s) {
+  if (W <= 0 || H <= 0 || !v) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_run_filter, dim3((unsigned)((W + 15) / 16), (unsigned)((H + 15) / 16)), dim3(256), 0, s, v, dedup,
+                     filter_max_cost, min_consistent);
+  return hipGetLastError();
 }
 
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s) {

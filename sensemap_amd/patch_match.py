@@ -48,6 +48,8 @@ class PatchMatchOptions:
     conf_filter: bool = False
     conf_threshold: float = 0.5
     depth_diff_threshold: float = 0.02
+    filter_geom_consistency_max_cost: float = 1.0
+    filter_min_num_consistent: int = 1
     # extensions
     init_samples: int = 1
     compat_lo_geo_bug: bool = True
@@ -282,6 +284,11 @@ class PatchMatchACMMCuda:
     def InitFrom(self, other: "PatchMatchACMMCuda") -> None:
         """Initial depth/normal from another problem's current maps (on device)."""
         abi.check(self.lib.pm_problem_init_from(self.handle, other.handle))
+
+    def RunFilter(self, deduplication_flag: bool = False) -> bool:
+        """RunFilter (patch_match_cuda.cu:1074-1108) on the view's given depth map."""
+        abi.check(self.lib.pm_problem_run_filter(self.handle, int(bool(deduplication_flag))))
+        return True
 
     def SetSrcDepthFrom(self, src_index: int, other: "PatchMatchACMMCuda") -> None:
         """Source `src_index`'s depth map from another problem's current depth map (on device)."""
