@@ -87,6 +87,23 @@ def c2_options(pmm, geom: bool, rcp: int):
                                  init_depth_random=not geom, thk=2, seed=2, rcp_mode=rcp)
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_1920x1080_pmc.md")
+
+
+def pmc_traffic():
+    """HBM bytes per photometric k_sweep launch of this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
+    calibration; the file gives the commands), or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        return None
+    return None
+
+
 def sweep_flops(stats) -> float:
     return (stats.sweep_ncc_samples * FLOP_PER_SAMPLE + stats.sweep_ncc_evaluations * FLOP_PER_EVAL
             + stats.sweep_geo_evaluations * FLOP_PER_GEO)
@@ -446,7 +463,7 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic(),
             "avg_launch_ms": round(avg_launch_ms, 3),
             "flop_per_launch": int(flops_sweep / launches_per_step),
         },

@@ -103,20 +103,23 @@ struct Ws {
   }
 };
 
+// Per-thread counts of one pixel's phase, 32 bits: even the largest window
+// (65x65) x 32 sources x 16 evaluations is 2.2e6 samples per pixel and
+// phase (init: init_samples <= 64, checked by pm_options_check).
 struct Counters {
-  unsigned long long ncc_eval = 0, ncc_samples = 0, geo_eval = 0;
+  uint32_t ncc_eval = 0, ncc_samples = 0, geo_eval = 0;
 };
 
 // Counter slots: [0..2] all kernels, [3..5] the sweep kernel alone.
 __device__ __forceinline__ void flush_counters(VPtr v, const Counters& c, bool sweep = false) {
   if (v->counters) {
-    atomicAdd(&v->counters[0], c.ncc_eval);
-    atomicAdd(&v->counters[1], c.ncc_samples);
-    atomicAdd(&v->counters[2], c.geo_eval);
+    atomicAdd(&v->counters[0], (unsigned long long)c.ncc_eval);
+    atomicAdd(&v->counters[1], (unsigned long long)c.ncc_samples);
+    atomicAdd(&v->counters[2], (unsigned long long)c.geo_eval);
     if (sweep) {
-      atomicAdd(&v->counters[3], c.ncc_eval);
-      atomicAdd(&v->counters[4], c.ncc_samples);
-      atomicAdd(&v->counters[5], c.geo_eval);
+      atomicAdd(&v->counters[3], (unsigned long long)c.ncc_eval);
+      atomicAdd(&v->counters[4], (unsigned long long)c.ncc_samples);
+      atomicAdd(&v->counters[5], (unsigned long long)c.geo_eval);
     }
   }
 }
@@ -360,7 +363,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   } else {
     walk(std::true_type{});
   }
-  cnt.ncc_samples += (unsigned long long)(n * n);
+  cnt.ncc_samples += (uint32_t)(n * n);
   const float inv_sw = px.inv_sw;
   float var_s, cov;
   if (!px.shifted) {
