@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rcp", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--storage", choices=["u8", "f16"], default="u8",
+                    help="source image storage in HBM (f16 holds the 8-bit values exactly: same maps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
     ap.add_argument("--only-photometric", action="store_true",
@@ -81,10 +83,10 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def c2_options(pmm, geom: bool, rcp: int):
+def c2_options(pmm, geom: bool, rcp: int, storage: int = 0):
     return pmm.PatchMatchOptions(window_radius=7, window_step=1, sigma_spatial=3.0, sigma_color=0.2,
                                  num_iterations=2 if geom else 8, geom_consistency=geom,
-                                 init_depth_random=not geom, thk=2, seed=2, rcp_mode=rcp)
+                                 init_depth_random=not geom, thk=2, seed=2, rcp_mode=rcp, image_storage=storage)
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_1920x1080_pmc.md")
@@ -382,7 +384,8 @@ def main() -> int:
     engine = pmm.default_engine()
 
     # sources' photometric passes (inputs of the geometric pass), untimed
-    photo_opts = c2_options(pmm, False, rcp)
+    storage = abi.PM_STORAGE_F16 if args.storage == "f16" else abi.PM_STORAGE_U8
+    photo_opts = c2_options(pmm, False, rcp, storage)
     src_photo = []
     for k in range(1, 1 if args.only_photometric else n_views):
         prob = pmm.Problem(k, [j for j in range(n_views) if j != k], images, sc.depth_min, sc.depth_max)
@@ -392,7 +395,7 @@ def main() -> int:
     ref_prob = pmm.Problem(0, list(range(1, n_views)), images, sc.depth_min, sc.depth_max,
                            depth_maps=[v.depth for v in views], normal_maps=[v.normal for v in views])
     photo = pmm.PatchMatchACMMCuda(photo_opts, ref_prob, engine)
-    geo = pmm.PatchMatchACMMCuda(c2_options(pmm, True, rcp), ref_prob, engine)
+    geo = pmm.PatchMatchACMMCuda(c2_options(pmm, True, rcp, storage), ref_prob, engine)
     for i, p in enumerate(src_photo):
         geo.SetSrcDepthFrom(i, p)
 
@@ -449,7 +452,7 @@ def main() -> int:
         "config": {"workload": "1920x1080 reference + 10 sources, heightfield scene, window 15x15 step 1, "
                                "8 photometric + 2 geometric iterations, one view per launch",
                    "width": W, "height": H, "num_src": n_views - 1, "window_radius": 7, "window_step": 1,
-                   "iterations": iters, "rcp_mode": args.rcp, "image_storage": "u8"},
+                   "iterations": iters, "rcp_mode": args.rcp, "image_storage": args.storage},
         "views_per_s": round(1000.0 / ms_per_step, 4),
         "median_rel_depth_error": float(np.median(err)),
         "frac_within_1pct": float(np.mean(err < 0.01)),

@@ -158,29 +158,39 @@ struct Quad<PM_STORAGE_F16> {
 };
 
 template <int STORE>
-__device__ __forceinline__ typename Quad<STORE>::type load_quad(const void* tex, uint32_t off) {
+__device__ __forceinline__ typename Quad<STORE>::type load_quad(const void* tex, uint32_t boff) {
   const __attribute__((address_space(1))) char* b = G((const char*)tex);
   if constexpr (STORE == PM_STORAGE_U8) {
-    return *(const __attribute__((address_space(1))) uint32_t*)(b + (off << 2));
+    return *(const __attribute__((address_space(1))) uint32_t*)(b + boff);
   } else {
-    return *(const __attribute__((address_space(1))) u2v*)(b + (off << 3));
+    return *(const __attribute__((address_space(1))) u2v*)(b + boff);
   }
 }
 
+// Bilinear value of a quad at fraction (fx, fy):
+// top = t00 + fx (t01 - t00), bot = t10 + fx (t11 - t10), top + fy (bot - top).
+// The f16 quad holds (t00, t01 - t00, t10, t11 - t10), all exact, so the
+// two horizontal lerps read the halves directly (v_fma_mix_f32) and give the
+// same bits as the u8 path.
 template <int STORE>
-__device__ __forceinline__ void unpack_quad(typename Quad<STORE>::type q, float& t0, float& t1, float& t2,
-                                            float& t3) {
+__device__ __forceinline__ float bilerp(typename Quad<STORE>::type q, float fx, float fy) {
+  float top, bot;
   if constexpr (STORE == PM_STORAGE_U8) {
-    t0 = (float)(q & 0xffu);
-    t1 = (float)((q >> 8) & 0xffu);
-    t2 = (float)((q >> 16) & 0xffu);
-    t3 = (float)(q >> 24);
+    const float t0 = (float)(q & 0xffu);
+    const float t1 = (float)((q >> 8) & 0xffu);
+    const float t2 = (float)((q >> 16) & 0xffu);
+    const float t3 = (float)(q >> 24);
+    top = fmaf(fx, t1 - t0, t0);
+    bot = fmaf(fx, t3 - t2, t2);
   } else {
-    t0 = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
-    t1 = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
-    t2 = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
-    t3 = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
+    const float t0 = __half2float(__ushort_as_half((unsigned short)(q.x & 0xffffu)));
+    const float d01 = __half2float(__ushort_as_half((unsigned short)(q.x >> 16)));
+    const float t2 = __half2float(__ushort_as_half((unsigned short)(q.y & 0xffffu)));
+    const float d23 = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
+    top = fmaf(fx, d01, t0);
+    bot = fmaf(fx, d23, t2);
   }
+  return fmaf(fy, bot - top, top);
 }
 
 // floor(x) as int in one instruction (v_cvt_flr_i32_f32); the compiler
@@ -268,7 +278,9 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
   float e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
   const float mu_r = px.mu_r;
-  const uint32_t upitch = (uint32_t)pitch;
+  const int ipitch = pitch;
+  constexpr uint32_t kQuadShift = STORE == PM_STORAGE_U8 ? 2u : 3u;  // bytes per quad: 4 or 8
+  const uint32_t pad_bytes = (uint32_t)(kQuadPad * pitch + kQuadPad) << kQuadShift;
   // One window walk; SHIFTED selects the compensated moment update.  A
   // sample is issued (projection, fraction, texel load) ahead of its use:
   // with compile-time window shapes each row keeps kDepth loads in flight,
@@ -282,15 +294,23 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   auto issue = [&](float x, float y, float z) __attribute__((always_inline)) {
     const float rz = rcp_z<RCP>(z);
     float u = x * rz, w_ = y * rz;
-    u = fminf(fmaxf(u, -3.0f), umax);
-    w_ = fminf(fmaxf(w_, -3.0f), vmax);
+    // clamp to [-3, W+2]: v_med3_f32 equals fminf(fmaxf(u, -3), umax) for
+    // every input, NaN and infinities included (a NaN goes to -3)
+    u = __builtin_amdgcn_fmed3f(u, -3.0f, umax);
+    w_ = __builtin_amdgcn_fmed3f(w_, -3.0f, vmax);
     Smp sm;
     // fraction = min(u - floor(u), 1 - 2^-24): v_fract_f32
     sm.fx = __builtin_amdgcn_fractf(u);
     sm.fy = __builtin_amdgcn_fractf(w_);
     const int x0 = cvt_flr(u), y0 = cvt_flr(w_);
-    const uint32_t off = __umul24((uint32_t)(y0 + kQuadPad), upitch) + (uint32_t)(x0 + kQuadPad);
-    sm.q = load_quad<STORE>(tex, off);
+    // texel (x0 + 4, y0 + 4) of the padded image, as a byte offset: the
+    // signed y0 * pitch + x0 (>= -3 pitch - 3) plus the uniform padding offset
+    const int t = __mul24(y0, ipitch) + x0;
+    const uint32_t boff = ((uint32_t)t << kQuadShift) + pad_bytes;
+    sm.q = load_quad<STORE>(tex, boff);
+    // keep the load where it is issued: without this the scheduler sinks
+    // the f16 loads next to their use and the row's loads no longer overlap
+    __builtin_amdgcn_sched_barrier(0);
     return sm;
   };
   auto walk = [&](auto shifted_tag) __attribute__((always_inline)) {
@@ -299,11 +319,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
       const uint8_t* trow = tile + ir * S * tw;
       const float* sprow = sp + ir * n;
       auto consume = [&](int j, const Smp& sm) __attribute__((always_inline)) {
-        float t0, t1, t2, t3;
-        unpack_quad<STORE>(sm.q, t0, t1, t2, t3);
-        const float top = fmaf(sm.fx, t1 - t0, t0);
-        const float bot = fmaf(sm.fx, t3 - t2, t2);
-        const float sv = fmaf(sm.fy, bot - top, top);
+        const float sv = bilerp<STORE>(sm.q, sm.fx, sm.fy);
         const uint32_t ri = trow[j * S];
         const float r = (float)ri;
         const float w = sprow[j] * ct[__builtin_amdgcn_sad_u8((uint32_t)c, ri, 0u)];
@@ -1250,14 +1266,19 @@ __global__ void k_quads_f16(const uint8_t* __restrict__ img, int W, int H, int s
   const int py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= pw || py >= ph) return;
   const int x = px - kQuadPad, y = py - kQuadPad;
-  unsigned short h[4];
+  float t[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int xx = x + (k & 1), yy = y + (k >> 1);
-    const float t = (xx < 0 || xx >= W || yy < 0 || yy >= H) ? 0.0f : (float)img[yy * stride + xx];
-    h[k] = __half_as_ushort(__float2half(t));
+    t[k] = (xx < 0 || xx >= W || yy < 0 || yy >= H) ? 0.0f : (float)img[yy * stride + xx];
   }
-  out[py * pw + px] = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+  // (t00, t01 - t00, t10, t11 - t10): integers of magnitude <= 255, exact in
+  // f16, so each horizontal lerp is one mixed-precision FMA on the halves
+  const unsigned short h0 = __half_as_ushort(__float2half(t[0]));
+  const unsigned short h1 = __half_as_ushort(__float2half(t[1] - t[0]));
+  const unsigned short h2 = __half_as_ushort(__float2half(t[2]));
+  const unsigned short h3 = __half_as_ushort(__float2half(t[3] - t[2]));
+  out[py * pw + px] = make_uint2((uint32_t)h0 | ((uint32_t)h1 << 16), (uint32_t)h2 | ((uint32_t)h3 << 16));
 }
 
 __global__ void k_depth_quads(const float* __restrict__ d, int W, int H, float4* __restrict__ out) {
