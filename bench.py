@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--jobs", type=int, default=24, help="serve workload: jobs per arrival rate")
     ap.add_argument("--rates", default="5,20,50,200", help="serve workload: arrival rates, requests/s")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve workload: HBM admission budget")
+    ap.add_argument("--max-active", type=int, default=16, help="serve workload: jobs on the device at once")
     return ap.parse_args()
 
 
@@ -290,7 +291,8 @@ def run_serve_workload(args) -> dict:
     plain_s = time.perf_counter() - t
     progress(f"plain path: {J} jobs in {plain_s:.1f} s")
 
-    server = pmm.PatchMatchServer(eng, hbm_budget_bytes=int(args.budget_gb * (1 << 30)), max_active_views=64)
+    server = pmm.PatchMatchServer(eng, hbm_budget_bytes=int(args.budget_gb * (1 << 30)),
+                                  max_active_views=args.max_active)
     rng = np.random.default_rng(5)
     per_rate = []
     served_err = []
@@ -345,7 +347,7 @@ def run_serve_workload(args) -> dict:
         "config": {"workload": f"Poisson arrivals at {args.rates} req/s, {J} jobs per rate, 500-image orbit, "
                                "sizes 640x480..3000x2000, 5-20 sources, 6 iterations; serving loop with "
                                f"{args.budget_gb:g} GB HBM budget, device LRU source cache, continuous batching",
-                   "jobs_per_rate": J, "budget_gb": args.budget_gb},
+                   "jobs_per_rate": J, "budget_gb": args.budget_gb, "max_active_views": args.max_active},
         "per_rate": per_rate,
         "plain_views_per_s": round(J / plain_s, 4),
         "plain_job_p50_ms": round(float(np.percentile(plain_lat, 50)), 1),
@@ -436,6 +438,29 @@ def main() -> int:
     err_photo = np.abs(photo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
     avg_launch_ms = sweep_ms / (args.steps * launches_per_step)
     achieved = (flops_sweep / launches_per_step) / (avg_launch_ms / 1000.0) / 1e12
+    # the same step with f16 image storage (the 8-bit values exactly: the maps
+    # must come out identical), timed the same way
+    f16 = None
+    if storage == abi.PM_STORAGE_U8 and not args.only_photometric:
+        o16, g16 = c2_options(pmm, False, rcp, abi.PM_STORAGE_F16), c2_options(pmm, True, rcp, abi.PM_STORAGE_F16)
+        photo16 = pmm.PatchMatchACMMCuda(o16, ref_prob, engine)
+        geo16 = pmm.PatchMatchACMMCuda(g16, ref_prob, engine)
+        for i, p in enumerate(src_photo):
+            geo16.SetSrcDepthFrom(i, p)
+
+        def step16():
+            photo16.Run()
+            geo16.InitFrom(photo16)
+            geo16.Run()
+
+        for _ in range(args.warmup):
+            step16()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step16()
+        ms16 = (time.perf_counter() - t) * 1000.0 / args.steps
+        f16 = {"value": round(mpix_iter / (ms16 / 1000.0), 3), "ms_per_step": round(ms16, 2),
+               "depth_maps_identical": bool(np.array_equal(geo16.GetDepthMap(), geo.GetDepthMap()))}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -457,6 +482,7 @@ def main() -> int:
         "median_rel_depth_error": float(np.median(err)),
         "frac_within_1pct": float(np.mean(err < 0.01)),
         "median_rel_depth_error_photometric": float(np.median(err_photo)),
+        "f16_storage": f16,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "roofline": {
