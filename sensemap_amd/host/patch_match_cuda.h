@@ -20,6 +20,7 @@
 // std::runtime_error carrying pm_last_error() instead.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
@@ -64,6 +65,9 @@ struct PatchMatchOptions {
   bool conf_filter = false;
   float conf_threshold = 0.5;
   float depth_diff_threshold = 0.02;
+  int filter_min_num_consistent = 1;
+  double filter_geom_consistency_max_cost = 1.0f;
+  double max_normal_error = 15.0f;
   std::string gpu_index = "-1";
   // extensions of this engine
   int init_samples = 1;
@@ -102,6 +106,8 @@ struct PatchMatchOptions {
     o.conf_filter = conf_filter;
     o.conf_threshold = conf_threshold;
     o.depth_diff_threshold = depth_diff_threshold;
+    o.filter_geom_consistency_max_cost = (float)filter_geom_consistency_max_cost;
+    o.filter_min_num_consistent = filter_min_num_consistent;
     o.init_samples = init_samples;
     o.compat_lo_geo_bug = compat_lo_geo_bug;
     o.image_storage = image_storage;
@@ -131,6 +137,20 @@ class Mat {
   void Set(size_t row, size_t col, size_t slice, T v) { data_[(slice * height_ + row) * width_ + col] = v; }
   T* GetPtr() { return data_.data(); }
   const T* GetPtr() const { return data_.data(); }
+  // Mat<T>::Write / Read (util/mat.h:178-215).
+  void Write(const std::string& path) const {
+    PmCheck(pm_mat_write(path.c_str(), (int32_t)width_, (int32_t)height_, (int32_t)depth_, (int32_t)sizeof(T),
+                         data_.data()));
+  }
+  void Read(const std::string& path) {
+    int32_t w = 0, h = 0, d = 0;
+    PmCheck(pm_mat_read_header(path.c_str(), &w, &h, &d));
+    width_ = (size_t)w;
+    height_ = (size_t)h;
+    depth_ = (size_t)d;
+    data_.assign(width_ * height_ * depth_, T());
+    PmCheck(pm_mat_read(path.c_str(), w, h, d, (int32_t)sizeof(T), data_.data()));
+  }
 
  protected:
   size_t width_ = 0, height_ = 0, depth_ = 0;
@@ -153,6 +173,25 @@ class NormalMap : public Mat<float> {
  public:
   NormalMap() = default;
   NormalMap(size_t width, size_t height) : Mat<float>(width, height, 3) {}
+  // NormalMap::Encode / Decode / Write / Read (normal_map.cc:121-178).
+  Mat<uint32_t> Encode() const {
+    Mat<uint32_t> m(width_, height_, 1);
+    PmCheck(pm_normal_encode((int64_t)(width_ * height_), data_.data(), m.GetPtr()));
+    return m;
+  }
+  void Decode(const Mat<uint32_t>& m) {
+    width_ = m.GetWidth();
+    height_ = m.GetHeight();
+    depth_ = 3;
+    data_.assign(width_ * height_ * 3, 0.0f);
+    PmCheck(pm_normal_decode((int64_t)(width_ * height_), m.GetPtr(), data_.data()));
+  }
+  void Write(const std::string& path) const { Encode().Write(path); }
+  void Read(const std::string& path) {
+    Mat<uint32_t> m;
+    m.Read(path);
+    Decode(m);
+  }
 };
 
 // mvs::Image: grey u8 bitmap (row 0 at the top), K as a 3x3 row-major
@@ -182,6 +221,7 @@ class Image {
 struct Problem {
   int ref_image_idx = -1;
   std::vector<int> src_image_idxs;
+  std::vector<int> src_image_extend_idxs;  // sources of the cross-view filter
   std::vector<Image>* images = nullptr;
   std::vector<DepthMap>* depth_maps = nullptr;
   std::vector<NormalMap>* normal_maps = nullptr;
@@ -264,6 +304,12 @@ class PatchMatchACMMCuda {
     PmCheck(pm_problem_get_view_sel(problem_, m.GetPtr()));
     return m;
   }
+  // RunFilter (patch_match_cuda.cu:1074-1108) on the problem's ref depth
+  // map; read the result with GetDepthMap.
+  bool RunFilter(bool deduplication_flag = false) {
+    PmCheck(pm_problem_run_filter(problem_, deduplication_flag ? 1 : 0));
+    return true;
+  }
 
  private:
   static pm_image ToImage(const Image& im) {
@@ -292,6 +338,117 @@ class PatchMatchACMMCuda {
   pm_problem* problem_ = nullptr;
   size_t width_ = 0, height_ = 0;
   int num_src_ = 0;
+};
+
+struct DepthMapInfo {
+  int width = 0, height = 0;
+  float depth_min = -1.0f, depth_max = -1.0f;
+};
+
+// CrossFilterCuda (patch_match_cuda.h:205-220, Run at
+// patch_match_cuda.cu:1493-1551): every problem's ref depth map refined by
+// its extended sources' depth and normal maps; all problems in one device
+// pass.  whole_ids lists the image indices taking part; maps come from
+// problems[0].depth_maps / normal_maps.
+class CrossFilterCuda {
+ public:
+  CrossFilterCuda(const PatchMatchOptions& options, const std::vector<Problem>& problems,
+                  const std::vector<int>& whole_ids, const std::vector<bool>& ref_flags)
+      : options_(options), problems_(problems), ids_(whole_ids), ref_flags_(ref_flags) {
+    const int device = options.gpu_index.empty() ? 0 : std::max(0, std::stoi(options.gpu_index));
+    PmCheck(pm_engine_create(device, &engine_));
+  }
+  ~CrossFilterCuda() { pm_engine_destroy(engine_); }
+  CrossFilterCuda(const CrossFilterCuda&) = delete;
+  CrossFilterCuda& operator=(const CrossFilterCuda&) = delete;
+
+  bool Run() {
+    if (problems_.empty()) return true;
+    const Problem& p0 = problems_[0];
+    const size_t n = ids_.size();
+    std::vector<pm_cross_image> images(n);
+    std::vector<Mat<uint32_t>> codes(n);
+    for (size_t k = 0; k < n; ++k) {
+      const int id = ids_[k];
+      const Image& im = p0.images->at(id);
+      codes[k] = p0.normal_maps->at(id).Encode();
+      pm_cross_image& c = images[k];
+      c.cam = ToCamera(im);
+      c.width = (int32_t)im.GetWidth();
+      c.height = (int32_t)im.GetHeight();
+      c.depth = p0.depth_maps->at(id).GetPtr();
+      c.normal_codes = codes[k].GetPtr();
+    }
+    std::vector<pm_cross_view> views;
+    std::vector<std::vector<int32_t>> srcs;
+    std::vector<int> refs;
+    refined_.assign(n, DepthMap());
+    for (const Problem& p : problems_) {
+      const int r = Position(p.ref_image_idx);
+      if (r < 0) continue;
+      std::vector<int32_t> s;
+      for (int id : p.src_image_extend_idxs) {
+        const int k = Position(id);
+        if (k >= 0 && s.size() < 31) s.push_back(k);
+      }
+      if (s.empty()) continue;  // FindSrcImageIds found no source (:1502-1504)
+      srcs.push_back(std::move(s));
+      refs.push_back(r);
+    }
+    for (size_t v = 0; v < refs.size(); ++v) {
+      const int r = refs[v];
+      refined_[r] = DepthMap(images[r].width, images[r].height, p0.depth_maps->at(ids_[r]).GetDepthMin(),
+                             p0.depth_maps->at(ids_[r]).GetDepthMax());
+      pm_cross_view cv{};
+      cv.ref = r;
+      cv.num_src = (int32_t)srcs[v].size();
+      cv.src = srcs[v].data();
+      cv.refined = refined_[r].GetPtr();
+      views.push_back(cv);
+    }
+    if (views.empty()) return true;
+    pm_cross_options o;
+    PmCheck(pm_cross_options_init(&o));
+    o.filter_geom_consistency_max_cost = (float)options_.filter_geom_consistency_max_cost;
+    o.max_normal_error = (float)options_.max_normal_error;
+    o.depth_diff_threshold = options_.depth_diff_threshold;
+    o.filter_min_num_consistent = options_.filter_min_num_consistent;
+    PmCheck(pm_cross_filter(engine_, &o, (int32_t)n, images.data(), (int32_t)views.size(), views.data()));
+    return true;
+  }
+
+  // GetRefineDepthMap (patch_match_cuda.cu:1627-1639).
+  DepthMap GetRefineDepthMap(int image_id, DepthMapInfo info) const {
+    const int k = Position(image_id);
+    if (k < 0 || refined_.empty() || refined_[k].GetWidth() == 0)
+      return DepthMap(info.width, info.height, info.depth_min, info.depth_max);
+    return refined_[k];
+  }
+
+ private:
+  int Position(int image_id) const {
+    for (size_t k = 0; k < ids_.size(); ++k)
+      if (ids_[k] == image_id) return (int)k;
+    return -1;
+  }
+  static pm_camera ToCamera(const Image& im) {
+    pm_camera c{};
+    const float* K = im.GetK();
+    c.K[0] = K[0];
+    c.K[1] = K[2];
+    c.K[2] = K[4];
+    c.K[3] = K[5];
+    for (int i = 0; i < 9; ++i) c.R[i] = im.GetR()[i];
+    for (int i = 0; i < 3; ++i) c.T[i] = im.GetT()[i];
+    return c;
+  }
+
+  PatchMatchOptions options_;
+  std::vector<Problem> problems_;
+  std::vector<int> ids_;
+  std::vector<bool> ref_flags_;
+  std::vector<DepthMap> refined_;
+  pm_engine* engine_ = nullptr;
 };
 
 }  // namespace mvs

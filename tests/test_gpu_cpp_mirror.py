@@ -45,3 +45,38 @@ def test_cpp_mirror_header_compiles(tmp_path):
     src.write_text('#include "sensemap_amd/host/patch_match_cuda.h"\nint main() { '
                    'sensemap_amd::mvs::PatchMatchOptions o; return o.window_radius == 5 ? 0 : 1; }\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", ROOT, str(src)], check=True)
+
+
+@pytest.mark.gpu
+def test_cpp_cross_filter_matches_python_path(tmp_path):
+    from sensemap_amd import io as pmio
+    from sensemap_amd.cross_filter import CrossImage, cross_filter
+    binary = os.path.join(ROOT, "tests", "cpp", "bin", "shim_cross")
+    assert os.path.exists(binary), "tests/cpp/bin/shim_cross is built by __graft_entry__.build()"
+    o = scene.orbit_scene(seed=9, num_cams=10, min_px=100 * 75, max_px=160 * 120, grid=32, extent=20.0)
+    lines = [str(len(o.views))]
+    ims = []
+    for k, v in enumerate(o.views):
+        c = v.camera
+        vals = [c.width, c.height] + list(c.K) + [float(x) for x in np.asarray(c.R, np.float32).reshape(9)] + \
+            [float(x) for x in np.asarray(c.T, np.float32)]
+        lines.append(" ".join(repr(x) if isinstance(x, int) else repr(float(np.float32(x))) for x in vals))
+        pmio.write_depth_map(str(tmp_path / f"depth{k}.bin"), v.depth)
+        pmio.write_normal_map(str(tmp_path / f"normal{k}.bin"), v.normal)
+        # the images as the C++ side sees them: float32 cameras; normals read
+        # from the file (decoded) and encoded again, as InitNormalMaps does
+        ims.append(CrossImage(tuple(np.float32(x) for x in c.K), np.asarray(c.R, np.float32),
+                              np.asarray(c.T, np.float32), v.depth,
+                              pmio.read_normal_map(str(tmp_path / f"normal{k}.bin"))))
+    views = [(k, [(k + d) % 10 for d in (-2, -1, 1, 2)]) for k in range(0, 10, 2)]
+    lines.append(str(len(views)))
+    for r, srcs in views:
+        lines.append(f"{r} {len(srcs)} " + " ".join(map(str, srcs)))
+    lines.append("1.0 15.0 0.02 1")
+    (tmp_path / "meta.txt").write_text("\n".join(lines) + "\n")
+    subprocess.run([binary, str(tmp_path)], check=True, timeout=120)
+    want = cross_filter(ims, views)
+    for (r, _), w in zip(views, want):
+        got = pmio.read_depth_map(str(tmp_path / f"refined{r}.bin"))
+        assert (got > 0).sum() > 0
+        assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), r
