@@ -413,6 +413,60 @@ pm_status pm_cross_filter(pm_engine* engine, const pm_cross_options* options, in
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,
                       const pm_view_desc* view, const pm_outputs* outputs);
 
+/* ---- sparse model: source-view selection and depth ranges ----
+ * The sparse model mvs::Model (mvs/model.h:22-101) as the controller hands
+ * it to the patch-match setup (patch_match_controller.cc:3153, 3171, 3294):
+ * the registered images' world->camera poses and rig flags, and the sparse
+ * points, each with its track (the indices of the images observing it).
+ * Tracks are CSR: point p is observed by
+ * track_images[track_offsets[p] .. track_offsets[p+1]). */
+typedef struct pm_model {
+  int32_t num_images;            /* 1..65535 */
+  const float* R;                /* num_images x 9, row-major */
+  const float* T;                /* num_images x 3 */
+  const uint8_t* is_rig;         /* num_images flags (Image::IsRig), NULL = none */
+  int64_t num_points;
+  const float* xyz;              /* num_points x 3 */
+  const int64_t* track_offsets;  /* num_points + 1, non-decreasing, from 0 */
+  const int32_t* track_images;   /* track_offsets[num_points] image indices */
+} pm_model;
+
+/* Per-phase device times and sizes of one selection (optional output). */
+typedef struct pm_overlap_stats {
+  int64_t pair_records;   /* (observation, co-observation) records emitted */
+  int64_t valid_records;  /* of them, between distinct images of the subset */
+  int64_t image_pairs;    /* distinct (image, co-observing image) pairs */
+  int32_t fallback_images;/* images ranked by the spatial fallback */
+  int32_t reserved0;
+  float emit_ms, sort_ms, reduce_ms, rank_ms, total_ms;
+  int32_t reserved1;
+} pm_overlap_stats;
+
+/* Model::GetMaxOverlappingImages (mvs/model.cc:474-699; with an image
+ * subset, :701-934).  For every image of the subset (image_mask[i] != 0;
+ * NULL = every image): the co-observing images of the subset whose 75th
+ * percentile triangulation angle over the shared points is at least
+ * min_triangulation_angle degrees, ranked by the number of shared points
+ * (times 1.5, truncated, when both images' rig flags agree), at most
+ * max_num_images of them.  An image that observes no point takes the
+ * spatial fallback (SelectSpatialNeighborViews): the images whose centre is
+ * 0.1..100 mean-nearest-neighbour distances away and whose viewing
+ * direction is within 60 degrees, ranked by (1.1 - cos angle) * max(d/m, 1)
+ * with near-duplicates of a chosen centre skipped.  Ties rank the lower
+ * image index first (the reference leaves their order to its hash map and
+ * sort).  Output is CSR: out_offsets[num_images + 1], out_images with room
+ * for num_images * max_num_images indices. */
+pm_status pm_model_max_overlapping_images(pm_engine* engine, const pm_model* model, int32_t max_num_images,
+                                          double min_triangulation_angle, const uint8_t* image_mask,
+                                          int32_t* out_offsets, int32_t* out_images, pm_overlap_stats* stats);
+
+/* Model::ComputeDepthRanges (mvs/model.cc:941-960) over the bounding box
+ * Model::Read builds (model.cc:221-267: every point and every camera
+ * centre): per image the min and max camera z of the box's 8 corners
+ * (ComputeDepthRangeFromBBox, model.cc:22-35), min clamped to 1e-6.
+ * out: num_images x 2 (depth_min, depth_max). */
+pm_status pm_model_depth_ranges(pm_engine* engine, const pm_model* model, float* out);
+
 #ifdef __cplusplus
 }
 #endif

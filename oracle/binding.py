@@ -51,6 +51,11 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_model_max_overlapping_images.restype = C.c_int
+        lib.pmo_model_max_overlapping_images.argtypes = [P(abi.pm_model), C.c_int, C.c_double, P(C.c_uint8),
+                                                         P(C.c_int32), P(C.c_int32)]
+        lib.pmo_model_depth_ranges.restype = C.c_int
+        lib.pmo_model_depth_ranges.argtypes = [P(abi.pm_model), P(C.c_float)]
         lib.pmo_geom_cost.restype = C.c_float
         lib.pmo_geom_cost.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), C.c_int, C.c_int, C.c_float, C.c_int]
         for name in ("pmo_expf", "pmo_sinf", "pmo_cosf", "pmo_acosf"):
@@ -134,4 +139,30 @@ def run_filter(options, view, dedup=False, threads=0):
     if lib.pmo_run_filter(C.byref(opt), C.byref(view.desc), int(bool(dedup)),
                           out.ctypes.data_as(C.POINTER(C.c_float)), int(threads)) != 0:
         raise RuntimeError("pmo_run_filter failed")
+    return out
+
+
+def max_overlapping_images(model, max_num_images, min_triangulation_angle, unique_image_idxs=None):
+    """Model::GetMaxOverlappingImages' transcription on a sensemap_amd.model.Model."""
+    lib = load()
+    I = model.num_images
+    mask = model.subset_mask(unique_image_idxs)
+    off = np.zeros(I + 1, np.int32)
+    out = np.zeros(max(I * int(max_num_images), 1), np.int32)
+    m = model.to_c()
+    if lib.pmo_model_max_overlapping_images(C.byref(m), int(max_num_images), float(min_triangulation_angle),
+                                            mask.ctypes.data_as(C.POINTER(C.c_uint8)) if mask is not None else None,
+                                            off.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            out.ctypes.data_as(C.POINTER(C.c_int32))) != 0:
+        raise RuntimeError("pmo_model_max_overlapping_images failed")
+    return [out[off[i]:off[i + 1]].tolist() for i in range(I)]
+
+
+def depth_ranges(model):
+    """Model::ComputeDepthRanges' transcription: (I, 2) float32."""
+    lib = load()
+    out = np.zeros((model.num_images, 2), np.float32)
+    m = model.to_c()
+    if lib.pmo_model_depth_ranges(C.byref(m), out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+        raise RuntimeError("pmo_model_depth_ranges failed")
     return out

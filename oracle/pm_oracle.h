@@ -104,6 +104,12 @@ int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, f
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
+/* Model::GetMaxOverlappingImages and Model::ComputeDepthRanges
+ * (pm_model_oracle.c), same arguments and outputs as the C-ABI's
+ * pm_model_max_overlapping_images / pm_model_depth_ranges. */
+int pmo_model_max_overlapping_images(const pm_model* m, int max_num_images, double min_triangulation_angle,
+                                     const uint8_t* mask, int32_t* out_offsets, int32_t* out_images);
+int pmo_model_depth_ranges(const pm_model* m, float* out);
 /* Geometric-consistency term (FP32 canonical), K = ref {fx, cx, fy, cy}. */
 float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, int col,
                     float depth, int src);

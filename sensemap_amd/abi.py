@@ -177,6 +177,19 @@ class pm_cross_view(C.Structure):
                 ("refined", C.POINTER(C.c_float))]
 
 
+class pm_model(C.Structure):
+    _fields_ = [("num_images", C.c_int32), ("R", C.POINTER(C.c_float)), ("T", C.POINTER(C.c_float)),
+                ("is_rig", C.POINTER(C.c_uint8)), ("num_points", C.c_int64), ("xyz", C.POINTER(C.c_float)),
+                ("track_offsets", C.POINTER(C.c_int64)), ("track_images", C.POINTER(C.c_int32))]
+
+
+class pm_overlap_stats(C.Structure):
+    _fields_ = [("pair_records", C.c_int64), ("valid_records", C.c_int64), ("image_pairs", C.c_int64),
+                ("fallback_images", C.c_int32), ("reserved0", C.c_int32),
+                ("emit_ms", C.c_float), ("sort_ms", C.c_float), ("reduce_ms", C.c_float), ("rank_ms", C.c_float),
+                ("total_ms", C.c_float), ("reserved1", C.c_int32)]
+
+
 _SIGNATURES = {
     "pm_options_init": (C.c_int, [C.POINTER(pm_options)]),
     "pm_options_check": (C.c_int, [C.POINTER(pm_options)]),
@@ -234,6 +247,10 @@ _SIGNATURES = {
     "pm_cross_options_init": (C.c_int, [C.POINTER(pm_cross_options)]),
     "pm_cross_filter": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_cross_image),
                                   C.c_int32, C.POINTER(pm_cross_view)]),
+    "pm_model_max_overlapping_images": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.c_int32, C.c_double,
+                                                  C.POINTER(C.c_uint8), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                                  C.POINTER(pm_overlap_stats)]),
+    "pm_model_depth_ranges": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.POINTER(C.c_float)]),
 }
 
 # SENSEMAP_PM_LIB selects another build of the engine (tuning variants).

@@ -327,3 +327,86 @@ def batch_problems(orbit: Orbit, num_views: int = 64, seed: int = 3, n_min: int 
                                  seed=seed, **opt_overrides)
         items.append((opts, Problem(int(r), src, images, dmin, dmax)))
     return items
+
+
+@dataclass
+class SparseScene:
+    """Cameras and sparse points with tracks: the input of the source-view
+    selection (sensemap_amd.model.Model)."""
+    R: np.ndarray          # (I, 3, 3) world -> camera
+    T: np.ndarray          # (I, 3)
+    is_rig: np.ndarray     # (I,) uint8
+    xyz: np.ndarray        # (P, 3) float32
+    track_offsets: np.ndarray  # (P + 1,) int64
+    track_images: np.ndarray   # int32
+    blind: np.ndarray      # indices of the cameras that observe no point
+
+    def model(self):
+        from .model import Model
+        return Model(self.R, self.T, self.xyz, self.track_offsets, self.track_images, self.is_rig)
+
+
+def sparse_scene(seed: int = 5, num_cams: int = 500, num_points: int = 200_000, radius: float = 12.0,
+                 height: float = 12.0, extent: float = 20.0, width: int = 1600, track_mean: float = 6.0,
+                 rig_fraction: float = 0.3, num_blind: int = 3, chunk: int = 20_000) -> SparseScene:
+    """`num_cams` cameras on a jittered orbit (radius, height, looking near
+    the centre), image width x 0.75 width, focal 0.8 width; `num_points`
+    points on a bumpy ground over [-extent, extent]^2.  A point's track is a
+    run of 2 + Poisson(track_mean - 2) cameras, consecutive in orbit order
+    among the cameras that see it (in front, inside the image), as feature
+    tracks over neighbouring views are.  `num_blind` cameras observe
+    nothing; a `rig_fraction` of the cameras carry the rig flag."""
+    rng = np.random.default_rng(seed)
+    I = int(num_cams)
+    h_img = int(round(width * 0.75))
+    f = 0.8 * width
+    Rs = np.zeros((I, 3, 3))
+    Ts = np.zeros((I, 3))
+    for k in range(I):
+        a = 2 * math.pi * k / I + rng.uniform(-0.2, 0.2) * 2 * math.pi / I
+        r = radius * (1.0 + 0.05 * rng.uniform(-1, 1))
+        hh = height * (1.0 + 0.1 * rng.uniform(-1, 1))
+        c = np.array([r * math.cos(a), -hh, r * math.sin(a)])
+        tgt = np.array([rng.uniform(-2, 2), 0.0, rng.uniform(-2, 2)])
+        Rs[k], Ts[k] = look_at(c, tgt)
+    Rs = Rs.astype(np.float32)
+    Ts = Ts.astype(np.float32)
+    is_rig = (rng.uniform(size=I) < rig_fraction).astype(np.uint8)
+    blind = rng.choice(I, size=min(num_blind, I), replace=False) if num_blind > 0 else np.zeros(0, np.int64)
+    R64 = Rs.astype(np.float64)
+    T64 = Ts.astype(np.float64)
+    xs, offs, imgs = [], [np.zeros(1, np.int64)], []
+    total = 0
+    left = int(num_points)
+    while left > 0:
+        n = min(chunk * 2, left * 2)
+        X = np.empty((n, 3))
+        X[:, 0] = rng.uniform(-extent, extent, n)
+        X[:, 2] = rng.uniform(-extent, extent, n)
+        X[:, 1] = -0.6 * np.sin(0.7 * X[:, 0]) * np.cos(0.5 * X[:, 2]) - 0.3 * np.sin(1.3 * X[:, 2] + 0.4)
+        cam = np.einsum("cij,pj->pci", R64, X) + T64[None]
+        z = cam[..., 2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u = f * cam[..., 0] / z + (width - 1) / 2.0
+            v = f * cam[..., 1] / z + (h_img - 1) / 2.0
+        vis = (z > 0.1) & (u >= 0) & (u <= width - 1) & (v >= 0) & (v <= h_img - 1)
+        vis[:, blind] = False
+        cv = vis.sum(1)
+        keep = cv >= 2
+        X, vis, cv = X[keep][:left], vis[keep][:left], cv[keep][:left]
+        m = X.shape[0]
+        if m == 0:
+            continue
+        L = np.minimum(cv, 2 + rng.poisson(max(track_mean - 2.0, 0.0), m))
+        k0 = np.floor(rng.uniform(size=m) * cv).astype(np.int64)
+        cs = np.cumsum(vis, axis=1) - 1
+        rank = np.mod(cs - k0[:, None], cv[:, None])
+        sel = vis & (rank < L[:, None])
+        rows, cols = np.nonzero(sel)
+        xs.append(X.astype(np.float32))
+        offs.append(total + np.cumsum(np.bincount(rows, minlength=m)).astype(np.int64))
+        imgs.append(cols.astype(np.int32))
+        total += int(sel.sum())
+        left -= m
+    return SparseScene(Rs, Ts, is_rig, np.concatenate(xs), np.concatenate(offs), np.concatenate(imgs),
+                       np.sort(blind).astype(np.int64))

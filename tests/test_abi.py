@@ -37,7 +37,8 @@ def test_exports_every_declared_symbol():
 def test_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     structs = ["pm_options", "pm_camera", "pm_image", "pm_view_desc", "pm_outputs", "pm_run_stats", "pm_tuning",
-               "pm_tune_report", "pm_server_config", "pm_job_stats", "pm_server_stats"]
+               "pm_tune_report", "pm_server_config", "pm_job_stats", "pm_server_stats", "pm_cross_options",
+               "pm_cross_image", "pm_cross_view", "pm_model", "pm_overlap_stats"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for s in structs:
         lines.append(f'  printf("{s} %zu\\n", sizeof({s}));')
@@ -93,4 +94,6 @@ def test_null_arguments_return_errors():
     assert lib.pm_problem_run(None) == 1
     assert lib.pm_engine_set_counting(None, 1) == 1
     assert lib.pm_problem_create(None, None, None, None) == 1
+    assert lib.pm_model_max_overlapping_images(None, None, 8, 1.0, None, None, None, None) == 1
+    assert lib.pm_model_depth_ranges(None, None, None) == 1
     assert lib.pm_abi_version() == 1
