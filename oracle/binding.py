@@ -147,8 +147,9 @@ def max_overlapping_images(model, max_num_images, min_triangulation_angle, uniqu
     lib = load()
     I = model.num_images
     mask = model.subset_mask(unique_image_idxs)
+    max_num_images = min(int(max_num_images), I)
     off = np.zeros(I + 1, np.int32)
-    out = np.zeros(max(I * int(max_num_images), 1), np.int32)
+    out = np.zeros(max(I * max_num_images, 1), np.int32)
     m = model.to_c()
     if lib.pmo_model_max_overlapping_images(C.byref(m), int(max_num_images), float(min_triangulation_angle),
                                             mask.ctypes.data_as(C.POINTER(C.c_uint8)) if mask is not None else None,

@@ -389,6 +389,7 @@ PM_API pm_status pm_model_max_overlapping_images(pm_engine* e, const pm_model* m
   const int32_t I = m->num_images;
   const int64_t P = m->num_points;
   const int64_t T = P > 0 ? m->track_offsets[P] : 0;
+  max_num_images = std::min(max_num_images, I);  // a list holds at most I - 1 images
   if (P > 0 && m->track_offsets[0] != 0) return fail(PM_ERR_INVALID_ARG, "track offsets must start at 0");
   int64_t R = 0;
   for (int64_t p = 0; p < P; ++p) {

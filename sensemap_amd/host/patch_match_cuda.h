@@ -7,6 +7,8 @@
 //   Mat<T>, DepthMap, NormalMap   src/util/mat.h, src/mvs/depth_map.h, src/mvs/normal_map.h
 //   Image (K, R, T, bitmap)  src/mvs/image.h
 //   PatchMatchACMMCuda       src/mvs/patch_match_cuda.h:154-203
+//   Model (sparse model)     src/mvs/model.h:22-101: GetMaxOverlappingImages,
+//                            ComputeDepthRanges
 // so PatchMatchController::ProcessProblem-style code
 // (src/controllers/patch_match_controller.cc:4676-4686) reads the same:
 //
@@ -25,6 +27,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
+#include <utility>
 #include <vector>
 
 #include "../../include/sensemap_pm.h"
@@ -200,8 +204,8 @@ class Image {
  public:
   Image() = default;
   Image(size_t width, size_t height, const float K[9], const float R[9], const float T[3],
-        std::vector<uint8_t> bitmap)
-      : width_(width), height_(height), bitmap_(std::move(bitmap)) {
+        std::vector<uint8_t> bitmap, bool is_rig = false)
+      : width_(width), height_(height), is_rig_(is_rig), bitmap_(std::move(bitmap)) {
     for (int i = 0; i < 9; ++i) K_[i] = K[i], R_[i] = R[i];
     for (int i = 0; i < 3; ++i) T_[i] = T[i];
   }
@@ -211,9 +215,11 @@ class Image {
   const float* GetR() const { return R_; }
   const float* GetT() const { return T_; }
   const std::vector<uint8_t>& GetBitmap() const { return bitmap_; }
+  bool IsRig() const { return is_rig_; }
 
  private:
   size_t width_ = 0, height_ = 0;
+  bool is_rig_ = false;
   float K_[9] = {}, R_[9] = {}, T_[3] = {};
   std::vector<uint8_t> bitmap_;
 };
@@ -449,6 +455,99 @@ class CrossFilterCuda {
   std::vector<bool> ref_flags_;
   std::vector<DepthMap> refined_;
   pm_engine* engine_ = nullptr;
+};
+
+// The sparse model as the patch-match setup reads it (mvs::Model,
+// src/mvs/model.h:22-101): images and points with tracks.  The two methods
+// the controller calls before building its Problems
+// (patch_match_controller.cc:3153, 3171, 3294) run on the device.
+struct Model {
+  struct Point {
+    float x = 0, y = 0, z = 0;
+    float error = 0;
+    std::vector<int> track;
+  };
+  std::vector<Image> images;
+  std::vector<Point> points;
+  int gpu_index = 0;  // device the selection runs on (not a reference member)
+
+  // model.cc:474-699
+  std::vector<std::vector<int>> GetMaxOverlappingImages(size_t max_num_images,
+                                                        double min_triangulation_angle) const {
+    return Select(max_num_images, min_triangulation_angle, nullptr);
+  }
+  // model.cc:701-934: only the images of the subset are ranked and counted
+  std::vector<std::vector<int>> GetMaxOverlappingImages(size_t max_num_images, double min_triangulation_angle,
+                                                        const std::unordered_set<int>& unique_image_idxs) const {
+    std::vector<uint8_t> mask(images.size(), 0);
+    for (int i : unique_image_idxs)
+      if (i >= 0 && (size_t)i < images.size()) mask[i] = 1;
+    return Select(max_num_images, min_triangulation_angle, &mask);
+  }
+  // model.cc:941-960
+  std::vector<std::pair<float, float>> ComputeDepthRanges() const {
+    Flat f(*this);
+    pm_engine* e = nullptr;
+    PmCheck(pm_engine_create(gpu_index, &e));
+    std::vector<float> out(2 * images.size());
+    const pm_status st = pm_model_depth_ranges(e, &f.model, out.data());
+    pm_engine_destroy(e);
+    PmCheck(st);
+    std::vector<std::pair<float, float>> ranges(images.size());
+    for (size_t i = 0; i < images.size(); ++i) ranges[i] = {out[2 * i], out[2 * i + 1]};
+    return ranges;
+  }
+
+ private:
+  struct Flat {
+    std::vector<float> R, T, xyz;
+    std::vector<uint8_t> rig;
+    std::vector<int64_t> off;
+    std::vector<int32_t> trk;
+    pm_model model{};
+    explicit Flat(const Model& m) {
+      const size_t I = m.images.size();
+      R.resize(9 * I);
+      T.resize(3 * I);
+      rig.resize(I);
+      for (size_t i = 0; i < I; ++i) {
+        std::copy(m.images[i].GetR(), m.images[i].GetR() + 9, R.begin() + 9 * i);
+        std::copy(m.images[i].GetT(), m.images[i].GetT() + 3, T.begin() + 3 * i);
+        rig[i] = m.images[i].IsRig() ? 1 : 0;
+      }
+      off.assign(1, 0);
+      for (const Point& p : m.points) {
+        xyz.insert(xyz.end(), {p.x, p.y, p.z});
+        trk.insert(trk.end(), p.track.begin(), p.track.end());
+        off.push_back((int64_t)trk.size());
+      }
+      model.num_images = (int32_t)I;
+      model.R = R.data();
+      model.T = T.data();
+      model.is_rig = rig.data();
+      model.num_points = (int64_t)m.points.size();
+      model.xyz = xyz.data();
+      model.track_offsets = off.data();
+      model.track_images = trk.data();
+    }
+  };
+
+  std::vector<std::vector<int>> Select(size_t max_num_images, double min_angle,
+                                       const std::vector<uint8_t>* mask) const {
+    Flat f(*this);
+    const size_t I = images.size();
+    const int32_t max_n = (int32_t)std::min<size_t>(max_num_images, I);  // a list holds at most I - 1
+    std::vector<int32_t> off(I + 1), out(std::max<size_t>(I * (size_t)max_n, 1));
+    pm_engine* e = nullptr;
+    PmCheck(pm_engine_create(gpu_index, &e));
+    const pm_status st = pm_model_max_overlapping_images(e, &f.model, max_n, min_angle, mask ? mask->data() : nullptr,
+                                                         off.data(), out.data(), nullptr);
+    pm_engine_destroy(e);
+    PmCheck(st);
+    std::vector<std::vector<int>> lists(I);
+    for (size_t i = 0; i < I; ++i) lists[i].assign(out.begin() + off[i], out.begin() + off[i + 1]);
+    return lists;
+  }
 };
 
 }  // namespace mvs

@@ -92,7 +92,8 @@ class Model:
         I = self.num_images
         mask = self.subset_mask(unique_image_idxs)
         off = np.zeros(I + 1, np.int32)
-        out = np.zeros(max(I * int(max_num_images), 1), np.int32)
+        max_num_images = min(int(max_num_images), I)  # a list holds at most I - 1 images
+        out = np.zeros(max(I * max_num_images, 1), np.int32)
         st = abi.pm_overlap_stats()
         m = self.to_c()
         abi.check(eng.lib.pm_model_max_overlapping_images(

@@ -373,8 +373,7 @@ def sparse_scene(seed: int = 5, num_cams: int = 500, num_points: int = 200_000, 
     Ts = Ts.astype(np.float32)
     is_rig = (rng.uniform(size=I) < rig_fraction).astype(np.uint8)
     blind = rng.choice(I, size=min(num_blind, I), replace=False) if num_blind > 0 else np.zeros(0, np.int64)
-    R64 = Rs.astype(np.float64)
-    T64 = Ts.astype(np.float64)
+    RT = np.ascontiguousarray(Rs.transpose(2, 0, 1).reshape(3, 3 * I))  # X @ RT = [R_c X]_c
     xs, offs, imgs = [], [np.zeros(1, np.int64)], []
     total = 0
     left = int(num_points)
@@ -384,7 +383,7 @@ def sparse_scene(seed: int = 5, num_cams: int = 500, num_points: int = 200_000, 
         X[:, 0] = rng.uniform(-extent, extent, n)
         X[:, 2] = rng.uniform(-extent, extent, n)
         X[:, 1] = -0.6 * np.sin(0.7 * X[:, 0]) * np.cos(0.5 * X[:, 2]) - 0.3 * np.sin(1.3 * X[:, 2] + 0.4)
-        cam = np.einsum("cij,pj->pci", R64, X) + T64[None]
+        cam = (X.astype(np.float32) @ RT).reshape(-1, I, 3) + Ts[None]
         z = cam[..., 2]
         with np.errstate(divide="ignore", invalid="ignore"):
             u = f * cam[..., 0] / z + (width - 1) / 2.0

@@ -80,3 +80,33 @@ def test_cpp_cross_filter_matches_python_path(tmp_path):
         got = pmio.read_depth_map(str(tmp_path / f"refined{r}.bin"))
         assert (got > 0).sum() > 0
         assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), r
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("subset", [False, True])
+def test_cpp_model_selection_matches_transcription(tmp_path, subset):
+    """mvs::Model::GetMaxOverlappingImages / ComputeDepthRanges through the
+    C++ mirror (tests/cpp/shim_model.cpp) equal the transcription."""
+    from oracle import binding as ob
+    from sensemap_amd import scene as sc
+    binary = os.path.join(ROOT, "tests", "cpp", "bin", "shim_model")
+    assert os.path.exists(binary), "tests/cpp/bin/shim_model is built by __graft_entry__.build()"
+    s = sc.sparse_scene(seed=31, num_cams=36, num_points=2500, num_blind=2)
+    m = s.model()
+    sub = sorted(set(range(0, 36, 2)) | {int(b) for b in s.blind}) if subset else None
+    lines = [str(m.num_images)]
+    for i in range(m.num_images):
+        lines.append(" ".join(repr(float(x)) for x in m.R[i]) + " " + " ".join(repr(float(x)) for x in m.T[i]) +
+                     f" {int(m.is_rig[i])}")
+    lines.append(str(m.num_points))
+    for p in range(m.num_points):
+        t = m.track_images[m.track_offsets[p]:m.track_offsets[p + 1]]
+        lines.append(" ".join(repr(float(x)) for x in m.xyz[p]) + f" {len(t)} " + " ".join(str(int(x)) for x in t))
+    lines.append("12 1.5 " + ("-1" if sub is None else f"{len(sub)} " + " ".join(map(str, sub))))
+    (tmp_path / "model.txt").write_text("\n".join(lines) + "\n")
+    subprocess.run([binary, str(tmp_path)], check=True, timeout=120)
+    got = [[int(x) for x in l.split()[1:]] for l in (tmp_path / "lists.txt").read_text().splitlines()]
+    assert got == ob.max_overlapping_images(m, 12, 1.5, sub)
+    rng = np.array([[int(x) for x in l.split()] for l in (tmp_path / "ranges.txt").read_text().splitlines()],
+                   np.uint32)
+    assert np.array_equal(rng, ob.depth_ranges(m).view(np.uint32))
