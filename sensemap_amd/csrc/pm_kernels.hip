@@ -291,13 +291,15 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
     float fx, fy;
     typename Quad<STORE>::type q;
   };
-  auto issue = [&](float x, float y, float z) __attribute__((always_inline)) {
+  auto issue = [&](float x, float y, float z, auto clamp_tag) __attribute__((always_inline)) {
     const float rz = rcp_z<RCP>(z);
     float u = x * rz, w_ = y * rz;
-    // clamp to [-3, W+2]: v_med3_f32 equals fminf(fmaxf(u, -3), umax) for
-    // every input, NaN and infinities included (a NaN goes to -3)
-    u = __builtin_amdgcn_fmed3f(u, -3.0f, umax);
-    w_ = __builtin_amdgcn_fmed3f(w_, -3.0f, vmax);
+    if constexpr (decltype(clamp_tag)::value) {
+      // clamp to [-3, W+2]: v_med3_f32 equals fminf(fmaxf(u, -3), umax) for
+      // every input, NaN and infinities included (a NaN goes to -3)
+      u = __builtin_amdgcn_fmed3f(u, -3.0f, umax);
+      w_ = __builtin_amdgcn_fmed3f(w_, -3.0f, vmax);
+    }
     Smp sm;
     // fraction = min(u - floor(u), 1 - 2^-24): v_fract_f32
     sm.fx = __builtin_amdgcn_fractf(u);
@@ -313,7 +315,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
     __builtin_amdgcn_sched_barrier(0);
     return sm;
   };
-  auto walk = [&](auto shifted_tag) __attribute__((always_inline)) {
+  auto walk = [&](auto shifted_tag, auto clamp_tag) __attribute__((always_inline)) {
     constexpr bool kShifted = decltype(shifted_tag)::value;
     for (int ir = 0; ir < n; ++ir) {
       const uint8_t* trow = tile + ir * S * tw;
@@ -344,7 +346,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
         Smp ring[kDepth];
 #pragma unroll
         for (int j = 0; j < kDepth; ++j) {
-          ring[j] = issue(x, y, z);
+          ring[j] = issue(x, y, z, clamp_tag);
           x = x + hxc;
           y = y + hyc;
           z = z + hzc;
@@ -353,7 +355,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
         for (int j = 0; j < kN; ++j) {
           const Smp cur = ring[j % kDepth];
           if (j + kDepth < kN) {
-            ring[j % kDepth] = issue(x, y, z);
+            ring[j % kDepth] = issue(x, y, z, clamp_tag);
             x = x + hxc;
             y = y + hyc;
             z = z + hzc;
@@ -362,7 +364,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
         }
       } else {
         for (int j = 0; j < n; ++j) {
-          const Smp cur = issue(x, y, z);
+          const Smp cur = issue(x, y, z, clamp_tag);
           x = x + hxc;
           y = y + hyc;
           z = z + hzc;
@@ -374,10 +376,44 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
       bz = bz + hzr;
     }
   };
+  // The clamps are identities when every sample lands inside [-3, W+2] x
+  // [-3, H+2].  Under the homography the window maps onto the convex
+  // quadrilateral of its 4 corner samples when z > 0 at all of them (z is
+  // affine over the window, so its minimum is at a corner).  Corners at
+  // least kInsideMargin px inside the range with z varying by at most 4x
+  // bound the incremental walk's rounding (<= ~30 ulp of the homogeneous
+  // coordinates, < 0.2 px for images up to 2^14 px wide) well below the
+  // margin, so such a window needs no clamp.  The choice is wave-uniform:
+  // one lane near a border keeps the clamped walk for the wave.
+  bool inside;
+  {
+    constexpr float kInsideMargin = 0.5f;
+    const float span = (float)(n - 1);
+    const float ex = span * hxc, ey = span * hyc, ez = span * hzc;
+    const float fx_ = span * hxr, fy_ = span * hyr, fz_ = span * hzr;
+    const float cx[4] = {bx, bx + ex, bx + fx_, (bx + ex) + fx_};
+    const float cy[4] = {by, by + ey, by + fy_, (by + ey) + fy_};
+    const float cz[4] = {bz, bz + ez, bz + fz_, (bz + ez) + fz_};
+    float zmin = cz[0], zmax = cz[0];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      zmin = fminf(zmin, cz[k]);
+      zmax = fmaxf(zmax, cz[k]);
+      const float iz = __builtin_amdgcn_rcpf(cz[k]);  // a test with a margin: the approximation serves
+      const float u = cx[k] * iz, w_ = cy[k] * iz;
+      ok = ok && u >= -3.0f + kInsideMargin && u <= umax - kInsideMargin && w_ >= -3.0f + kInsideMargin &&
+           w_ <= vmax - kInsideMargin;
+    }
+    inside = __all(ok && zmin > 0.0f && zmax <= 4.0f * zmin);
+  }
   if (!px.shifted) {
-    walk(std::false_type{});
+    if (inside)
+      walk(std::false_type{}, std::false_type{});
+    else
+      walk(std::false_type{}, std::true_type{});
   } else {
-    walk(std::true_type{});
+    walk(std::true_type{}, std::true_type{});
   }
   cnt.ncc_samples += (uint32_t)(n * n);
   const float inv_sw = px.inv_sw;
