@@ -4,7 +4,8 @@
 // owns a 32x16 pixel tile (its 256 pixels of the colour); each wave64 covers
 // 4 rows x 16 same-colour pixels, so the 64 windows of a wave overlap and
 // their source footprints hit the same cache lines.  Per workgroup the
-// reference tile (+ window halo, u8) and the 256-entry bilateral colour
+// reference tile (+ window halo; one 32-bit word per texel holding the
+// intensity and 3x the intensity, see ncc_eval) and the 256-entry bilateral colour
 // table are staged in LDS; the spatial weight table and the per-source pose
 // blocks are read with uniform (scalar) loads because the source loop and
 // the window position are uniform across the wave.  Source images live in
@@ -218,7 +219,7 @@ __device__ __forceinline__ void neumaier_add(float& s, float& c, float x) {
 // window's top-left sample; tw is the tile row length.
 template <int R_, int S_, int STORE, int RCP>
 __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
-                                          const uint8_t* tile, int tw, const float* ct, const float* sp,
+                                          const uint32_t* tile, int tw, const float* ct, const float* sp,
                                           float nx, float ny, float nz, float d, Counters& cnt) {
   const int R = R_ >= 0 ? R_ : v->radius;
   const int S = R_ >= 0 ? S_ : v->step;
@@ -274,7 +275,7 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   const float umax = Wsf + 2.0f, vmax = Hsf + 2.0f;
   const int pitch = s.pitch;
   const void* tex = s.tex;
-  const int c = px.c;
+  const uint32_t c4 = ((uint32_t)px.c << 16) | (3u * (uint32_t)px.c);
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
   float e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
   const float mu_r = px.mu_r;
@@ -318,13 +319,17 @@ __device__ __forceinline__ float ncc_eval(VPtr v, int i, const Pix& px,
   auto walk = [&](auto shifted_tag, auto clamp_tag) __attribute__((always_inline)) {
     constexpr bool kShifted = decltype(shifted_tag)::value;
     for (int ir = 0; ir < n; ++ir) {
-      const uint8_t* trow = tile + ir * S * tw;
+      const uint32_t* trow = tile + ir * S * tw;
       const float* sprow = sp + ir * n;
       auto consume = [&](int j, const Smp& sm) __attribute__((always_inline)) {
         const float sv = bilerp<STORE>(sm.q, sm.fx, sm.fy);
-        const uint32_t ri = trow[j * S];
-        const float r = (float)ri;
-        const float w = sprow[j] * ct[__builtin_amdgcn_sad_u8((uint32_t)c, ri, 0u)];
+        // tile word (ri << 16) | 3 ri: byte 2 is the intensity; v_sad_u16
+        // adds the absolute differences of both 16-bit halves, so against
+        // the centre's (c << 16) | 3 c it gives 3|ri - c| + |ri - c| =
+        // 4|ri - c|, the colour weight's LDS byte offset
+        const uint32_t tw4 = trow[j * S];
+        const float r = (float)((tw4 >> 16) & 0xffu);  // v_cvt_f32_ubyte2
+        const float w = sprow[j] * *(const float*)((const char*)ct + __builtin_amdgcn_sad_u16(tw4, c4, 0u));
         if constexpr (!kShifted) {
           const float ws = w * sv;
           s1 = s1 + ws;
@@ -471,7 +476,7 @@ __device__ __forceinline__ float geom_eval(VPtr v, int i, const Pix& px, float d
 
 // ComputeMultiViewCostVector (kernel_functions.cu:1447-1495), priors off.
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ float mv_cost(VPtr v, int i, const Pix& px, const uint8_t* tile,
+__device__ __forceinline__ float mv_cost(VPtr v, int i, const Pix& px, const uint32_t* tile,
                                          int tw, const float* ct, const float* sp, float nx, float ny, float nz,
                                          float d, K4<float>* Kgeo, Counters& cnt) {
   float cost = ncc_eval<R_, S_, STORE, RCP>(v, i, px, tile, tw, ct, sp, nx, ny, nz, d, cnt);
@@ -594,7 +599,7 @@ struct TileSize {
   // constants; rows: the tallest tile plus the halo.
   static constexpr int kR = R_ >= 0 ? R_ : PM_MAX_WINDOW_RADIUS;
   static constexpr int kPitch = (1 << kTileLog2WMax) + 2 * kR;
-  static constexpr int kBytes = ((kTilePixels >> kTileLog2WMin) + 2 * kR) * kPitch;
+  static constexpr int kWords = ((kTilePixels >> kTileLog2WMin) + 2 * kR) * kPitch;
 };
 
 // Stage the colour table and the ref tile (zero outside the image) into
@@ -666,7 +671,7 @@ __device__ __forceinline__ Ctx resolve(const LaunchTable* __restrict__ lt, int t
 // Stage the colour table, the spatial weights and the ref tile (zero outside
 // the image) into LDS, then map the thread to its pixel.  Returns false for
 // threads whose pixel lies outside the image (after the barrier).
-__device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, int pitch, uint8_t* tile, float* ct, float* spl,
+__device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, int pitch, uint32_t* tile, float* ct, float* spl,
                                            Pix& px) {
   const VPtr v = cx.v;
   const int t = threadIdx.x;
@@ -684,7 +689,8 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, int pitch, uint
   for (int e = t; e < tw * th; e += kSweepThreads) {
     const int er = e / tw, ec = e - er * tw;
     const int rr = row0 - R + er, cc = col0 - R + ec;
-    tile[er * pitch + ec] = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? (uint8_t)0 : G(v->ref)[rr * W + cc];
+    const uint32_t ri = (rr < 0 || rr >= H || cc < 0 || cc >= W) ? 0u : (uint32_t)G(v->ref)[rr * W + cc];
+    tile[er * pitch + ec] = (ri << 16) | (3u * ri);
   }
   __syncthreads();
   px.lrow = t >> (lw - 1);
@@ -698,7 +704,7 @@ __device__ __forceinline__ bool setup_pass(const Ctx& cx, int R, int pitch, uint
   px.var_r = m.y;
   px.inv_sw = m.z;
   px.err_r = m.w;
-  px.c = tile[(px.lrow + R) * pitch + px.lcol + R];
+  px.c = (int)(tile[(px.lrow + R) * pitch + px.lcol + R] >> 16);
   px.shifted = (v->accum_mode == PM_ACCUM_COMPENSATED) && (px.var_r < kLowTexVar);
   px.np0 = v->refInvK[0] * (float)px.col + v->refInvK[1];
   px.np1 = v->refInvK[2] * (float)px.row + v->refInvK[3];
@@ -713,14 +719,14 @@ __device__ __forceinline__ int slot_index() { return blockIdx.x * kSweepThreads 
 // init: random (best of init_samples) or given depth/normal, then InitConf
 
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ void init_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
+__device__ __forceinline__ void init_tile(const Ctx& cx, uint32_t* tile, float* ct, float* spl) {
   const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
   const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
   if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
-  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const uint32_t* wt = tile + px.lrow * tw + px.lcol;
   const int N = v->N;
   const int plane = v->W * v->H;
   Ws ws{cx.ws, cx.ws_slots, slot_index(), N};
@@ -792,7 +798,7 @@ __device__ __forceinline__ void init_tile(const Ctx& cx, uint8_t* tile, float* c
 
 template <int R_, int S_, int STORE, int RCP>
 __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ uint32_t tile[TileSize<R_>::kWords];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
   __shared__ int next;
@@ -812,7 +818,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_init(const LaunchTable* __res
 #endif
 
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ void sweep_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
+__device__ __forceinline__ void sweep_tile(const Ctx& cx, uint32_t* tile, float* ct, float* spl) {
   const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const float th_mc = cx.th_mc;
@@ -820,7 +826,7 @@ __device__ __forceinline__ void sweep_tile(const Ctx& cx, uint8_t* tile, float* 
   Pix px;
   const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
   if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
-  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const uint32_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
@@ -1064,7 +1070,7 @@ __device__ __forceinline__ void sweep_tile(const Ctx& cx, uint8_t* tile, float* 
 
 template <int R_, int S_, int STORE, int RCP>
 __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ uint32_t tile[TileSize<R_>::kWords];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
   __shared__ int next;
@@ -1076,14 +1082,14 @@ __global__ __launch_bounds__(kSweepThreads) PM_SWEEP_OCC void k_sweep(const Laun
 // local optimisation
 
 template <int R_, int S_, int STORE, int RCP>
-__device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint8_t* tile, float* ct, float* spl) {
+__device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint32_t* tile, float* ct, float* spl) {
   const VPtr v = cx.v;
   const uint32_t pass = cx.pass;
   const int R = R_ >= 0 ? R_ : v->radius;
   Pix px;
   const int tw = R_ >= 0 ? TileSize<R_>::kPitch : (1 << kTileLog2WMax) + 2 * R;
   if (!setup_pass(cx, R, tw, tile, ct, spl, px)) return;
-  const uint8_t* wt = tile + px.lrow * tw + px.lcol;
+  const uint32_t* wt = tile + px.lrow * tw + px.lcol;
   const int W = v->W, H = v->H, N = v->N;
   const int plane = W * H;
   const int p = px.p, row = px.row, col = px.col;
@@ -1174,7 +1180,7 @@ __device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint8_t* tile, flo
 
 template <int R_, int S_, int STORE, int RCP>
 __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* __restrict__ lt) {
-  __shared__ uint8_t tile[TileSize<R_>::kBytes];
+  __shared__ uint32_t tile[TileSize<R_>::kWords];
   __shared__ float ct[256];
   __shared__ float spl[SpSize<R_, S_>::kFloats];
   __shared__ int next;
