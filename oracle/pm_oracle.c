@@ -944,18 +944,32 @@ static void local_opt_pixel(const pass_ctx* x, int row, int col) {
   float M[5][PM_MAX_NUM_SRC];
   int best = -1;
   hyp4 bh;
+  /* A trial whose running sum already gives a mean >= the pixel's initial
+   * conf cannot be taken (costs are >= 0; float sums of non-negative terms
+   * never decrease), so its remaining evaluations are skipped.  Not in the
+   * reference; the maps are unchanged, and the device skips exactly these
+   * evaluations, so the evaluation counts agree. */
+  const float mw_sel = (float)__builtin_popcount(sel);
+  const float conf0 = conf;
+  const int prune = !opt->geom_consistency || opt->geom_consistency_regularizer >= 0.0f;
   for (int t = 0; t < 5; ++t) {
     hyp4 h;
     h.d = random_depth(&g, low, up);
     perturb_normal(&g, mn, angle1, angle2, h.n);
     float mconf = 0.0f, mw = 0.0f;
+    int ruled_out = 0;
     for (int i = 0; i < N; ++i) {
       if (!((sel >> i) & 1u)) continue;
       const float c = mv_cost(opt, v, row, col, &h, np, i, v->refK, x->cnt);
       M[t][i] = c;
       mconf = mconf + c;
       mw = mw + 1.0f;
+      if (prune && mconf / mw_sel >= conf0) {
+        ruled_out = 1;
+        break;
+      }
     }
+    if (ruled_out) continue;
     mconf = mconf / mw;
     if (mconf < conf) {
       conf = mconf;

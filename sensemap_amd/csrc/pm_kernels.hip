@@ -1133,22 +1133,41 @@ __device__ __forceinline__ void local_opt_tile(const Ctx& cx, uint32_t* tile, fl
     ws.hyp(t, 2) = hn[2];
     ws.hyp(t, 3) = hd;
   }
+  // A trial's mean over the selected sources is its running sum / mw.
+  // Costs are >= 0 (NCC in [0, 2], the geometric term scaled by a
+  // non-negative regulariser) and a float sum of non-negative terms never
+  // decreases, so once sum / mw >= the pixel's conf the trial cannot be
+  // taken: its remaining evaluations are skipped (a wave skips a walk when
+  // every lane has ruled the trial out).  The maps are those of the full
+  // evaluation; the transcription prunes the same evaluations.
+  const float mw = (float)__popc(sel);
+  const float conf0 = conf;
+  const bool prune = !v->geom || v->geom_regularizer >= 0.0f;
+  float part0 = 0.0f, part1 = 0.0f, part2 = 0.0f, part3 = 0.0f, part4 = 0.0f;
+  uint32_t live = 0x1fu;
   for (int i = 0; i < N; ++i) {
     if (!((sel >> i) & 1u)) continue;
+#pragma unroll 1
     for (int t = 0; t < 5; ++t) {
-      ws.M(t, i) = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, ws.hyp(t, 0), ws.hyp(t, 1), ws.hyp(t, 2),
-                                               ws.hyp(t, 3), v->refK, cnt);
+      if (!((live >> t) & 1u)) continue;
+      const float m = mv_cost<R_, S_, STORE, RCP>(v, i, px, wt, tw, ct, spl, ws.hyp(t, 0), ws.hyp(t, 1),
+                                                  ws.hyp(t, 2), ws.hyp(t, 3), v->refK, cnt);
+      ws.M(t, i) = m;
+      float sum;
+      if (t == 0) sum = part0 = part0 + m;
+      else if (t == 1) sum = part1 = part1 + m;
+      else if (t == 2) sum = part2 = part2 + m;
+      else if (t == 3) sum = part3 = part3 + m;
+      else sum = part4 = part4 + m;
+      if (prune && sum / mw >= conf0) live &= ~(1u << t);
     }
   }
   int best = -1;
+  const float parts[5] = {part0, part1, part2, part3, part4};
+#pragma unroll
   for (int t = 0; t < 5; ++t) {
-    float mconf = 0.0f, mw = 0.0f;
-    for (int i = 0; i < N; ++i) {
-      if (!((sel >> i) & 1u)) continue;
-      mconf = mconf + ws.M(t, i);
-      mw = mw + 1.0f;
-    }
-    mconf = mconf / mw;
+    if (!((live >> t) & 1u)) continue;
+    const float mconf = parts[t] / mw;
     if (mconf < conf) {
       conf = mconf;
       best = t;

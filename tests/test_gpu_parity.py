@@ -132,3 +132,13 @@ def test_fast_reciprocal_within_tolerance():
     me, mf = np.median(res[abi.PM_RCP_EXACT]), np.median(res[abi.PM_RCP_FAST])
     print("[parity] median rel depth error exact", me, "fast", mf, flush=True)
     assert abs(mf - me) <= 0.01 * max(me, 1e-3) + 2e-4
+
+
+@pytest.mark.parametrize("width,height,num_src", [(97, 61, 1), (45, 130, 17), (33, 17, 3)])
+def test_ragged_shapes_and_source_counts_bitwise(width, height, num_src):
+    # sizes that are no multiple of the 32x16 tile (partial tiles, windows
+    # over the image border), one source, and more sources than the
+    # reference's 8
+    sc = scene.planes_scene(seed=5, width=width, height=height, f=0.8 * max(width, height), num_src=num_src)
+    gpu, cpu = run_both(sc, helpers.c1_options(num_iterations=4))
+    assert_bitwise(gpu, cpu)
