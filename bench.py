@@ -138,9 +138,29 @@ def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
     binding.run_view(opts, view, r0, r1, threads)
     dt = time.perf_counter() - t
     mpix_iter = (r1 - r0) * W * opts.num_iterations / 1e6
+    # one thread on a 2-row band, 1 iteration (scaling reference)
+    one = c2_options(pmm, False, 0)
+    one.num_iterations = 1
+    t = time.perf_counter()
+    binding.run_view(one, view, mid, mid + 2, 1)
+    d1 = time.perf_counter() - t
     return {"value": round(mpix_iter / dt, 4), "unit": "Mpixel·iter/s", "cores": threads, "kind": "port",
             "sample": f"C/OpenMP transcription, photometric pass of the 1920x1080 view, rows [{r0},{r1}) "
-                      f"x {W} px, {opts.num_iterations} iterations, 10 sources, 15x15 step 1; {dt:.1f} s"}
+                      f"x {W} px, {opts.num_iterations} iterations, 10 sources, 15x15 step 1; {dt:.1f} s",
+            "cpu_model": cpu_model(),
+            "single_thread": {"value": round(2 * W / 1e6 / d1, 5), "unit": "Mpixel·iter/s",
+                              "sample": f"rows [{mid},{mid + 2}) x {W} px, 1 iteration; {d1:.1f} s"}}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def progress(msg: str) -> None:
