@@ -144,12 +144,17 @@ def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
     t = time.perf_counter()
     binding.run_view(one, view, mid, mid + 2, 1)
     d1 = time.perf_counter() - t
+    t = time.perf_counter()
+    binding.run_view(one, view, mid, mid + 2, threads)
+    dn = time.perf_counter() - t
     return {"value": round(mpix_iter / dt, 4), "unit": "Mpixel·iter/s", "cores": threads, "kind": "port",
             "sample": f"C/OpenMP transcription, photometric pass of the 1920x1080 view, rows [{r0},{r1}) "
                       f"x {W} px, {opts.num_iterations} iterations, 10 sources, 15x15 step 1; {dt:.1f} s",
             "cpu_model": cpu_model(),
             "single_thread": {"value": round(2 * W / 1e6 / d1, 5), "unit": "Mpixel·iter/s",
-                              "sample": f"rows [{mid},{mid + 2}) x {W} px, 1 iteration; {d1:.1f} s"}}
+                              "same_sample_all_threads": round(2 * W / 1e6 / dn, 5),
+                              "sample": f"rows [{mid},{mid + 2}) x {W} px, 1 iteration (with the init); "
+                                        f"{d1:.1f} s on 1 thread"}}
 
 
 def cpu_model() -> str:

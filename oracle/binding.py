@@ -51,6 +51,9 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_select_views.restype = C.c_int
+        lib.pmo_select_views.argtypes = [P(abi.pm_options), C.c_float, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
+                                         P(C.c_float), P(C.c_uint32)]
         lib.pmo_model_max_overlapping_images.restype = C.c_int
         lib.pmo_model_max_overlapping_images.argtypes = [P(abi.pm_model), C.c_int, C.c_double, P(C.c_uint8),
                                                          P(C.c_int32), P(C.c_int32)]
@@ -167,3 +170,24 @@ def depth_ranges(model):
     if lib.pmo_model_depth_ranges(C.byref(m), out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
         raise RuntimeError("pmo_model_depth_ranges failed")
     return out
+
+
+def select_views(options, th_mc, M, draws):
+    """View selection + weighted argmin of one sweep pixel: M (nh, N) costs
+    (row 0 the current hypothesis), draws the pixel's num_samples uniforms.
+    Returns (best hypothesis or -1, weights (N,), view_sel mask)."""
+    lib = load()
+    M = np.asarray(M, np.float32)
+    nh, N = M.shape
+    full = np.zeros((nh, abi.PM_MAX_NUM_SRC), np.float32)
+    full[:, :N] = M
+    d = np.ascontiguousarray(draws, np.float32)
+    opt = options.to_c()
+    if len(d) != opt.num_samples:
+        raise ValueError("need num_samples draws")
+    w = np.zeros(abi.PM_MAX_NUM_SRC, np.float32)
+    sel = C.c_uint32(0)
+    best = lib.pmo_select_views(C.byref(opt), float(th_mc), nh, N, full.ctypes.data_as(C.POINTER(C.c_float)),
+                                d.ctypes.data_as(C.POINTER(C.c_float)), w.ctypes.data_as(C.POINTER(C.c_float)),
+                                C.byref(sel))
+    return best, w[:N], sel.value

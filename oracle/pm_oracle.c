@@ -662,6 +662,82 @@ static void init_pixel(const pass_ctx* x, int row, int col) {
 
 /* SweepFromCheckerBoardOpt (kernel_functions.cu:2136-2560), priors off and
  * propagate_depth off. */
+/* View selection (kernel_functions.cu:2364-2426), TransformPDFToCDF
+ * (:672-686) and FindMinCost (:649-659) for one pixel.  M holds nh rows of
+ * PM_MAX_NUM_SRC costs (row k = hypothesis k, column i = source i; row 0 the
+ * current hypothesis); draws holds num_samples uniforms in (0, 1], the
+ * pixel's first draws.  A source is flagged when at least num_good_hypothesis
+ * of its costs are below th_mc and at most num_bad_hypothesis above
+ * max_ncc_matching_cost; its probability is sum_k exp(-c^2 / (2 beta^2)),
+ * zeroed when unflagged and normalised to a CDF.  Each draw u - FLT_EPSILON
+ * picks the first source whose CDF exceeds it; picked sources get weight 1
+ * (random_optimization) or a count, normalised.  Returns the hypothesis of
+ * least weighted cost ('<=': the last minimum wins), with the weights and
+ * the view_sel mask, or -1 when no source is flagged or the weights vanish
+ * (the pixel keeps its state). */
+int pmo_select_views(const pm_options* opt, float th_mc, int nh, int N, const float* M, const float* draws,
+                     float* wts, uint32_t* sel_out) {
+  const float inv_beta2 = 0.5f / (opt->beta * opt->beta);
+  float prob[PM_MAX_NUM_SRC];
+  uint32_t flags = 0;
+  for (int i = 0; i < N; ++i) {
+    float pr = 0.0f;
+    int n1 = 0, n2 = 0;
+    for (int k = 0; k < nh; ++k) {
+      const float c = M[k * PM_MAX_NUM_SRC + i];
+      pr = pr + pm_expf(-(c * c) * inv_beta2);
+      n1 += (c < th_mc);
+      n2 += (c > opt->max_ncc_matching_cost);
+    }
+    prob[i] = pr;
+    if (n1 >= opt->num_good_hypothesis && n2 <= opt->num_bad_hypothesis) flags |= 1u << i;
+  }
+  if (flags == 0) return -1;
+  for (int i = 0; i < N; ++i) {
+    if (!((flags >> i) & 1u)) prob[i] = 0.0f;
+  }
+  float psum = 0.0f;
+  for (int i = 0; i < N; ++i) psum = psum + prob[i];
+  const float inv_psum = 1.0f / psum;
+  float cum = 0.0f;
+  for (int i = 0; i < N; ++i) {
+    cum = cum + prob[i] * inv_psum;
+    prob[i] = cum;
+  }
+  for (int i = 0; i < N; ++i) wts[i] = 0.0f;
+  for (int smp = 0; smp < opt->num_samples; ++smp) {
+    const float rp = draws[smp] - PM_FLT_EPSILON;
+    for (int i = 0; i < N; ++i) {
+      if (prob[i] > rp) {
+        if (opt->random_optimization) wts[i] = 1.0f;
+        else wts[i] = wts[i] + 1.0f;
+        break;
+      }
+    }
+  }
+  float wnorm = 0.0f;
+  for (int i = 0; i < N; ++i) wnorm = wnorm + wts[i];
+  if (wnorm <= 1e-5f) return -1;
+  const float inv_wnorm = 1.0f / wnorm;
+  uint32_t sel = 0;
+  for (int i = 0; i < N; ++i) {
+    wts[i] = wts[i] * inv_wnorm;
+    if (wts[i] > 0.0f) sel |= 1u << i;
+  }
+  *sel_out = sel;
+  int best = -1;
+  float minc = FLT_MAX;
+  for (int k = 0; k < nh; ++k) {
+    float c = 0.0f;
+    for (int i = 0; i < N; ++i) c = c + wts[i] * M[k * PM_MAX_NUM_SRC + i];
+    if (c <= minc) {
+      minc = c;
+      best = k;
+    }
+  }
+  return best < 0 ? 0 : best;
+}
+
 static void sweep_pixel(const pass_ctx* x, int row, int col) {
   const pm_options* opt = x->opt;
   const pmo_view* v = x->v;
@@ -675,7 +751,6 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
   float normal[3];
   load_normal(st, plane, p, normal);
 
-  const float inv_beta2 = 0.5f / (opt->beta * opt->beta);
   const float depth_min = v->depth_min;
   const float depth_max = (float)(1.2 * (double)v->depth_max);
   const float angle1 = fminf(opt->random_angle1_range, PM_PI_F);
@@ -769,74 +844,16 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
     for (int i = 0; i < N; ++i) M[k][i] = mv_cost(opt, v, row, col, &hyps[k], np, i, v->refK, x->cnt);
   }
 
-  /* view selection (kernel_functions.cu:2364-2426) */
-  float prob[PM_MAX_NUM_SRC];
-  uint32_t flags = 0;
-  for (int i = 0; i < N; ++i) {
-    float pr = 0.0f;
-    int n1 = 0, n2 = 0;
-    for (int k = 0; k < nh; ++k) {
-      const float c = M[k][i];
-      pr = pr + pm_expf(-(c * c) * inv_beta2);
-      n1 += (c < x->th_mc);
-      n2 += (c > opt->max_ncc_matching_cost);
-    }
-    prob[i] = pr;
-    if (n1 >= opt->num_good_hypothesis && n2 <= opt->num_bad_hypothesis) flags |= 1u << i;
-  }
-  if (flags == 0) return;
-  for (int i = 0; i < N; ++i) {
-    if (!((flags >> i) & 1u)) prob[i] = 0.0f;
-  }
-  /* TransformPDFToCDF (kernel_functions.cu:672-686) */
-  float psum = 0.0f;
-  for (int i = 0; i < N; ++i) psum = psum + prob[i];
-  const float inv_psum = 1.0f / psum;
-  float cum = 0.0f;
-  for (int i = 0; i < N; ++i) {
-    cum = cum + prob[i] * inv_psum;
-    prob[i] = cum;
-  }
+  /* view selection and weighted argmin (pmo_select_views) */
   pm_rng g;
   pm_rng_init(&g, opt->seed, v->view_id, (uint32_t)p, x->pass);
+  float draws[opt->num_samples > 0 ? opt->num_samples : 1];
+  for (int smp = 0; smp < opt->num_samples; ++smp) draws[smp] = pm_rng_uniform(&g);
   float wts[PM_MAX_NUM_SRC];
-  for (int i = 0; i < N; ++i) wts[i] = 0.0f;
-  for (int smp = 0; smp < opt->num_samples; ++smp) {
-    const float rp = pm_rng_uniform(&g) - PM_FLT_EPSILON;
-    for (int i = 0; i < N; ++i) {
-      if (prob[i] > rp) {
-        if (opt->random_optimization) wts[i] = 1.0f;
-        else wts[i] = wts[i] + 1.0f;
-        break;
-      }
-    }
-  }
-  float wnorm = 0.0f;
-  for (int i = 0; i < N; ++i) wnorm = wnorm + wts[i];
-  if (wnorm <= 1e-5f) return;
-  const float inv_wnorm = 1.0f / wnorm;
   uint32_t sel = 0;
-  for (int i = 0; i < N; ++i) {
-    wts[i] = wts[i] * inv_wnorm;
-    if (wts[i] > 0.0f) sel |= 1u << i;
-  }
-  float costs[MAX_HYP];
-  for (int k = 0; k < nh; ++k) {
-    float c = 0.0f;
-    for (int i = 0; i < N; ++i) c = c + wts[i] * M[k][i];
-    costs[k] = c;
-  }
+  int best = pmo_select_views(opt, x->th_mc, nh, N, &M[0][0], draws, wts, &sel);
+  if (best == -1) return;
   st->view_sel[p] = sel;
-
-  /* FindMinCost: '<=' so the last minimum wins (kernel_functions.cu:649-659) */
-  int best = -1;
-  float minc = FLT_MAX;
-  for (int k = 0; k < nh; ++k) {
-    if (costs[k] <= minc) {
-      minc = costs[k];
-      best = k;
-    }
-  }
   if (best < 0) best = 0;
   depth = hyps[best].d;
   normal[0] = hyps[best].n[0];
@@ -885,7 +902,7 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
     rc[k] = c;
   }
   int b2 = -1;
-  minc = FLT_MAX;
+  float minc = FLT_MAX;
   for (int k = 0; k < 5; ++k) {
     if (rc[k] <= minc) {
       minc = rc[k];

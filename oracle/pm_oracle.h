@@ -25,6 +25,18 @@
  *
  * plus an FP64 restatement of the NCC cost (pmo_ncc_cost_f64) used as the
  * precision reference.
+ *
+ * Pinned independently (tests/test_oracle_*.py): the random generator
+ * (Random123 known answers), the math functions (libm FP64), the pose
+ * packing (closed forms), the reference moments (FP64 brute force), the NCC
+ * cost (closed forms: identical, affine and inverted windows, flat windows,
+ * integer shifts), the geometric term, the view selection and weighted
+ * argmin (pmo_select_views: hand-computed cases), the filters and the
+ * sparse-model selection (pm_model_oracle.c).
+ * Parity unpinned: the sweep's candidate gathering, the refinement
+ * hypotheses, LocalOptimization and the iteration schedule have no
+ * independent closed form; they are checked only end to end (depth against
+ * rendered ground truth) and by the device's bitwise agreement.
  */
 #ifndef SENSEMAP_PM_ORACLE_H_
 #define SENSEMAP_PM_ORACLE_H_
@@ -104,6 +116,11 @@ int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, f
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
+/* View selection and weighted argmin of one sweep pixel (see pm_oracle.c):
+ * M is nh rows of PM_MAX_NUM_SRC costs, draws num_samples uniforms.  Returns
+ * the chosen hypothesis, or -1 when the pixel keeps its state. */
+int pmo_select_views(const pm_options* opt, float th_mc, int nh, int N, const float* M, const float* draws,
+                     float* wts, uint32_t* sel);
 /* Model::GetMaxOverlappingImages and Model::ComputeDepthRanges
  * (pm_model_oracle.c), same arguments and outputs as the C-ABI's
  * pm_model_max_overlapping_images / pm_model_depth_ranges. */
