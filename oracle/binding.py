@@ -51,6 +51,9 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_gather_candidates.restype = C.c_int
+        lib.pmo_gather_candidates.argtypes = [C.c_int, C.c_int, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int,
+                                              P(C.c_float), C.c_int, C.c_int, P(C.c_float)]
         lib.pmo_select_views.restype = C.c_int
         lib.pmo_select_views.argtypes = [P(abi.pm_options), C.c_float, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                          P(C.c_float), P(C.c_uint32)]
@@ -191,3 +194,20 @@ def select_views(options, th_mc, M, draws):
                                 d.ctypes.data_as(C.POINTER(C.c_float)), w.ctypes.data_as(C.POINTER(C.c_float)),
                                 C.byref(sel))
     return best, w[:N], sel.value
+
+
+def gather_candidates(conf, depth, normal, grad_ok, iK, row, col):
+    """Candidate hypotheses of sweep pixel (row, col): conf, depth (H, W),
+    normal (3, H, W), iK = (1/fx, -cx/fx, 1/fy, -cy/fy).  Returns (n, 4)
+    rows {nx, ny, nz, d}."""
+    lib = load()
+    conf = np.ascontiguousarray(conf, np.float32)
+    depth = np.ascontiguousarray(depth, np.float32)
+    normal = np.ascontiguousarray(normal, np.float32)
+    H, W = conf.shape
+    k = np.ascontiguousarray(iK, np.float32)
+    out = np.zeros((11, 4), np.float32)
+    P = C.POINTER(C.c_float)
+    n = lib.pmo_gather_candidates(W, H, conf.ctypes.data_as(P), depth.ctypes.data_as(P), normal.ctypes.data_as(P),
+                                  int(bool(grad_ok)), k.ctypes.data_as(P), int(row), int(col), out.ctypes.data_as(P))
+    return out[:n]

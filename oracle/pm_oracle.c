@@ -662,6 +662,105 @@ static void init_pixel(const pass_ctx* x, int row, int col) {
 
 /* SweepFromCheckerBoardOpt (kernel_functions.cu:2136-2560), priors off and
  * propagate_depth off. */
+/* Candidate hypotheses of one sweep pixel (kernel_functions.cu:2232-2327):
+ * the pixel's own (n, d); when grad_ok, up to two 4-neighbours by ascending
+ * conf (bubble sort); then the least-conf sample of each of the 8
+ * directional groups (the first minimum wins).  Each neighbour's normal is
+ * CorrectNormal-ed and its plane propagated to the pixel (PropagateDepth).
+ * conf, depth: W*H row-major; normal: 3 planes of W*H.  out: up to
+ * (1 + 2 + 8) rows {nx, ny, nz, d}.  Returns the number of rows. */
+int pmo_gather_candidates(int W, int Hh, const float* conf, const float* depth, const float* normal, int grad_ok,
+                          const float* iK, int row, int col, float* out) {
+  const size_t plane = (size_t)W * Hh;
+  const size_t p = (size_t)row * W + col;
+  const float vd[3] = {iK[0] * (float)col + iK[1], iK[2] * (float)row + iK[3], 1.0f};
+  const float inv_view_len = 1.0f / sqrtf(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+  const float np[3] = {vd[0], vd[1], 1.0f};
+  hyp4* hyps = (hyp4*)out;
+  int nh = 0;
+  hyps[nh].d = depth[p];
+  hyps[nh].n[0] = normal[p];
+  hyps[nh].n[1] = normal[plane + p];
+  hyps[nh].n[2] = normal[2 * plane + p];
+  nh++;
+
+  /* up to two 4-neighbours by ascending conf, gated by the gradient */
+  if (grad_ok) {
+    float nconf[4];
+    int order[4];
+    int nvalid = 0;
+    for (int i = 0; i < 4; ++i) {
+      order[i] = i;
+      const int u = col + kNeighborOffs[i][1], w = row + kNeighborOffs[i][0];
+      if (u < 0 || u >= W || w < 0 || w >= Hh) {
+        nconf[i] = FLT_MAX;
+        continue;
+      }
+      nvalid++;
+      nconf[i] = conf[(size_t)w * W + u];
+    }
+    for (int a = 0; a < 3; ++a) {
+      for (int b = 0; b < 3 - a; ++b) {
+        if (nconf[b] > nconf[b + 1]) {
+          const float t = nconf[b];
+          nconf[b] = nconf[b + 1];
+          nconf[b + 1] = t;
+          const int o = order[b];
+          order[b] = order[b + 1];
+          order[b + 1] = o;
+        }
+      }
+    }
+    if (nvalid > 2) nvalid = 2;
+    for (int i = 0; i < nvalid; ++i) {
+      const int idx = order[i];
+      const int u = col + kNeighborOffs[idx][1], w = row + kNeighborOffs[idx][0];
+      const size_t q = (size_t)w * W + u;
+      float nn[3];
+      nn[0] = normal[q];
+      nn[1] = normal[plane + q];
+      nn[2] = normal[2 * plane + q];
+      correct_normal(nn, vd, inv_view_len);
+      hyps[nh].d = propagate_depth(depth[q], nn, iK, w, u, np);
+      hyps[nh].n[0] = nn[0];
+      hyps[nh].n[1] = nn[1];
+      hyps[nh].n[2] = nn[2];
+      nh++;
+    }
+  }
+  /* best-conf sample of each of the 8 directional groups */
+  for (int gi = 0; gi < K_NUM_SAMPS; ++gi) {
+    float best_conf = FLT_MAX;
+    int br = 0, bc = 0;
+    for (int f = 0; f < K_NUM_EACH_SAMP; ++f) {
+      const int nr = row + kDirSamples[gi][2 * f];
+      const int ncn = col + kDirSamples[gi][2 * f + 1];
+      if (nr < 0 || nr >= Hh || ncn < 0 || ncn >= W) continue;
+      const float c = conf[(size_t)nr * W + ncn];
+      if (best_conf > c) {
+        best_conf = c;
+        br = nr;
+        bc = ncn;
+      }
+    }
+    if (best_conf < FLT_MAX) {
+      const size_t q = (size_t)br * W + bc;
+      float nn[3];
+      nn[0] = normal[q];
+      nn[1] = normal[plane + q];
+      nn[2] = normal[2 * plane + q];
+      correct_normal(nn, vd, inv_view_len);
+      hyps[nh].d = propagate_depth(depth[q], nn, iK, br, bc, np);
+      hyps[nh].n[0] = nn[0];
+      hyps[nh].n[1] = nn[1];
+      hyps[nh].n[2] = nn[2];
+      nh++;
+    }
+  }
+
+  return nh;
+}
+
 /* View selection (kernel_functions.cu:2364-2426), TransformPDFToCDF
  * (:672-686) and FindMinCost (:649-659) for one pixel.  M holds nh rows of
  * PM_MAX_NUM_SRC costs (row k = hypothesis k, column i = source i; row 0 the
@@ -760,82 +859,8 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
   const float np[3] = {vd[0], vd[1], 1.0f};
 
   hyp4 hyps[MAX_HYP];
-  int nh = 0;
-  hyps[nh].d = depth;
-  hyps[nh].n[0] = normal[0];
-  hyps[nh].n[1] = normal[1];
-  hyps[nh].n[2] = normal[2];
-  nh++;
-
-  /* up to two 4-neighbours by ascending conf, gated by the gradient */
-  if (v->grad_ok[p]) {
-    float nconf[4];
-    int order[4];
-    int nvalid = 0;
-    for (int i = 0; i < 4; ++i) {
-      order[i] = i;
-      const int u = col + kNeighborOffs[i][1], w = row + kNeighborOffs[i][0];
-      if (u < 0 || u >= W || w < 0 || w >= Hh) {
-        nconf[i] = FLT_MAX;
-        continue;
-      }
-      nvalid++;
-      nconf[i] = st->conf[(size_t)w * W + u];
-    }
-    for (int a = 0; a < 3; ++a) {
-      for (int b = 0; b < 3 - a; ++b) {
-        if (nconf[b] > nconf[b + 1]) {
-          const float t = nconf[b];
-          nconf[b] = nconf[b + 1];
-          nconf[b + 1] = t;
-          const int o = order[b];
-          order[b] = order[b + 1];
-          order[b + 1] = o;
-        }
-      }
-    }
-    if (nvalid > 2) nvalid = 2;
-    for (int i = 0; i < nvalid; ++i) {
-      const int idx = order[i];
-      const int u = col + kNeighborOffs[idx][1], w = row + kNeighborOffs[idx][0];
-      const size_t q = (size_t)w * W + u;
-      float nn[3];
-      load_normal(st, plane, q, nn);
-      correct_normal(nn, vd, inv_view_len);
-      hyps[nh].d = propagate_depth(st->depth[q], nn, iK, w, u, np);
-      hyps[nh].n[0] = nn[0];
-      hyps[nh].n[1] = nn[1];
-      hyps[nh].n[2] = nn[2];
-      nh++;
-    }
-  }
-  /* best-conf sample of each of the 8 directional groups */
-  for (int gi = 0; gi < K_NUM_SAMPS; ++gi) {
-    float best_conf = FLT_MAX;
-    int br = 0, bc = 0;
-    for (int f = 0; f < K_NUM_EACH_SAMP; ++f) {
-      const int nr = row + kDirSamples[gi][2 * f];
-      const int ncn = col + kDirSamples[gi][2 * f + 1];
-      if (nr < 0 || nr >= Hh || ncn < 0 || ncn >= W) continue;
-      const float c = st->conf[(size_t)nr * W + ncn];
-      if (best_conf > c) {
-        best_conf = c;
-        br = nr;
-        bc = ncn;
-      }
-    }
-    if (best_conf < FLT_MAX) {
-      const size_t q = (size_t)br * W + bc;
-      float nn[3];
-      load_normal(st, plane, q, nn);
-      correct_normal(nn, vd, inv_view_len);
-      hyps[nh].d = propagate_depth(st->depth[q], nn, iK, br, bc, np);
-      hyps[nh].n[0] = nn[0];
-      hyps[nh].n[1] = nn[1];
-      hyps[nh].n[2] = nn[2];
-      nh++;
-    }
-  }
+  const int nh = pmo_gather_candidates(W, Hh, st->conf, st->depth, st->normal, v->grad_ok[p], iK, row, col,
+                                       &hyps[0].n[0]);
 
   /* cost matrix: row 0 is the stored cost of the current hypothesis */
   float M[MAX_HYP][PM_MAX_NUM_SRC];

@@ -30,13 +30,15 @@
  * (Random123 known answers), the math functions (libm FP64), the pose
  * packing (closed forms), the reference moments (FP64 brute force), the NCC
  * cost (closed forms: identical, affine and inverted windows, flat windows,
- * integer shifts), the geometric term, the view selection and weighted
- * argmin (pmo_select_views: hand-computed cases), the filters and the
+ * integer shifts), the geometric term, the sweep's candidate gathering
+ * (pmo_gather_candidates: a designed conf map whose picks follow from the
+ * reference's offset table), the view selection and weighted argmin
+ * (pmo_select_views: hand-computed cases), the filters and the
  * sparse-model selection (pm_model_oracle.c).
- * Parity unpinned: the sweep's candidate gathering, the refinement
- * hypotheses, LocalOptimization and the iteration schedule have no
- * independent closed form; they are checked only end to end (depth against
- * rendered ground truth) and by the device's bitwise agreement.
+ * Parity unpinned: the refinement hypotheses, LocalOptimization and the
+ * iteration schedule have no independent closed form; they are checked
+ * only end to end (depth against rendered ground truth) and by the
+ * device's bitwise agreement.
  */
 #ifndef SENSEMAP_PM_ORACLE_H_
 #define SENSEMAP_PM_ORACLE_H_
@@ -116,6 +118,10 @@ int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, f
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
+/* Candidate hypotheses of one sweep pixel (see pm_oracle.c): out gets up to
+ * 11 rows {nx, ny, nz, d}; returns the count. */
+int pmo_gather_candidates(int W, int H, const float* conf, const float* depth, const float* normal, int grad_ok,
+                          const float* iK, int row, int col, float* out);
 /* View selection and weighted argmin of one sweep pixel (see pm_oracle.c):
  * M is nh rows of PM_MAX_NUM_SRC costs, draws num_samples uniforms.  Returns
  * the chosen hypothesis, or -1 when the pixel keeps its state. */
