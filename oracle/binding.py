@@ -51,6 +51,13 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_refine_hypotheses.restype = C.c_int
+        lib.pmo_refine_hypotheses.argtypes = [P(abi.pm_options), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                              P(C.c_float), C.c_float, P(C.c_float), C.c_float, C.c_float,
+                                              P(C.c_float)]
+        lib.pmo_lo_trials.restype = C.c_int
+        lib.pmo_lo_trials.argtypes = [C.c_int, C.c_int, P(C.c_float), P(C.c_float), C.c_int, C.c_int, C.c_uint64,
+                                      C.c_uint32, C.c_uint32, P(C.c_float)]
         lib.pmo_gather_candidates.restype = C.c_int
         lib.pmo_gather_candidates.argtypes = [C.c_int, C.c_int, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int,
                                               P(C.c_float), C.c_int, C.c_int, P(C.c_float)]
@@ -211,3 +218,30 @@ def gather_candidates(conf, depth, normal, grad_ok, iK, row, col):
     n = lib.pmo_gather_candidates(W, H, conf.ctypes.data_as(P), depth.ctypes.data_as(P), normal.ctypes.data_as(P),
                                   int(bool(grad_ok)), k.ctypes.data_as(P), int(row), int(col), out.ctypes.data_as(P))
     return out[:n]
+
+
+def refine_hypotheses(options, view_id, pass_id, width, row, col, iK, depth, normal, depth_min, depth_max):
+    """The 5 refinement hypotheses (rows {nx, ny, nz, d}) of sweep pixel (row, col)."""
+    lib = load()
+    P = C.POINTER(C.c_float)
+    k = np.ascontiguousarray(iK, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    out = np.zeros((5, 4), np.float32)
+    opt = options.to_c()
+    lib.pmo_refine_hypotheses(C.byref(opt), int(view_id), int(pass_id), int(width), int(row), int(col),
+                              k.ctypes.data_as(P), float(depth), n.ctypes.data_as(P), float(depth_min),
+                              float(depth_max), out.ctypes.data_as(P))
+    return out
+
+
+def lo_trials(depth, normal, row, col, seed, view_id, pass_id):
+    """LocalOptimization's 5 trial hypotheses (rows {nx, ny, nz, d}) of pixel (row, col)."""
+    lib = load()
+    P = C.POINTER(C.c_float)
+    d = np.ascontiguousarray(depth, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    H, W = d.shape
+    out = np.zeros((5, 4), np.float32)
+    lib.pmo_lo_trials(W, H, d.ctypes.data_as(P), n.ctypes.data_as(P), int(row), int(col), int(seed), int(view_id),
+                      int(pass_id), out.ctypes.data_as(P))
+    return out

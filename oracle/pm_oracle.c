@@ -662,6 +662,100 @@ static void init_pixel(const pass_ctx* x, int row, int col) {
 
 /* SweepFromCheckerBoardOpt (kernel_functions.cu:2136-2560), priors off and
  * propagate_depth off. */
+/* The refinement hypotheses of a sweep pixel (kernel_functions.cu:
+ * 2449-2486), drawn in order from the pixel's stream g: ref[0] the winner
+ * (n, d); ref[1] (PerturbNormal(n), d); ref[2] (n, U[(1 - rho) d,
+ * (1 + rho) d]); ref[3] (random normal facing the camera, d); ref[4] (n,
+ * U[depth_min, depth_max]).  Perturbed and random normals are
+ * CorrectNormal-ed.  depth_max here is the pass's 1.2 x depth_max. */
+static void refine_hyps(const pm_options* opt, pm_rng* g, float depth, const float normal[3], const float vd[3],
+                        float inv_view_len, float depth_min, float depth_max, hyp4 ref[5]) {
+  const float angle1 = fminf(opt->random_angle1_range, PM_PI_F);
+  const float angle2 = fminf(opt->random_angle2_range, PM_HALF_PI_F);
+  float nd;
+  {
+    const float dlo = (1.0f - opt->random_depth_ratio) * depth;
+    const float dhi = (1.0f + opt->random_depth_ratio) * depth;
+    nd = random_depth(g, dlo, dhi);
+  }
+  float nn[3];
+  perturb_normal(g, normal, angle1, angle2, nn);
+  correct_normal(nn, vd, inv_view_len);
+  const float rd = random_depth(g, depth_min, depth_max);
+  float rn[3];
+  random_normal(g, vd, rn);
+  correct_normal(rn, vd, inv_view_len);
+  ref[0].d = depth;
+  memcpy(ref[0].n, normal, 3 * sizeof(float));
+  ref[1].d = depth;
+  memcpy(ref[1].n, nn, sizeof(nn));
+  ref[2].d = nd;
+  memcpy(ref[2].n, normal, 3 * sizeof(float));
+  ref[3].d = depth;
+  memcpy(ref[3].n, rn, sizeof(rn));
+  ref[4].d = rd;
+  memcpy(ref[4].n, normal, 3 * sizeof(float));
+}
+
+/* LocalOptimization's trial hypotheses (kernel_functions.cu:2736-2860): the
+ * depth range [low, up] and the normalised mean normal over the pixel and
+ * its in-image 4-neighbours, then 5 trials (perturbed mean normal by
+ * +-4 / +-2 degrees, uniform depth in [low, up]) drawn in order from g. */
+static void lo_trials(const float* depth, const float* normal, int W, int Hh, int row, int col, pm_rng* g,
+                      hyp4 out[5]) {
+  const size_t plane = (size_t)W * Hh;
+  const size_t p = (size_t)row * W + col;
+  /* DEG2RAD(4.0) and DEG2RAD(2.0) are double products (kernel_functions.h:39) */
+  const float angle1 = (float)(4.0 * 0.0174532925199432);
+  const float angle2 = (float)(2.0 * 0.0174532925199432);
+  float low = depth[p], up = depth[p];
+  float m0 = normal[p], m1 = normal[plane + p], m2 = normal[2 * plane + p];
+  for (int i = 0; i < 4; ++i) {
+    const int u = col + kNeighborOffs[i][1], w = row + kNeighborOffs[i][0];
+    if (u < 0 || u >= W || w < 0 || w >= Hh) continue;
+    const size_t q = (size_t)w * W + u;
+    const float ndp = depth[q];
+    m0 = m0 + normal[q];
+    m1 = m1 + normal[plane + q];
+    m2 = m2 + normal[2 * plane + q];
+    if (ndp <= low) low = ndp;
+    if (ndp > up) up = ndp;
+  }
+  float mn[3] = {m0, m1, m2};
+  const float inv_norm = 1.0f / sqrtf(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
+  mn[0] = mn[0] * inv_norm;
+  mn[1] = mn[1] * inv_norm;
+  mn[2] = mn[2] * inv_norm;
+  for (int t = 0; t < 5; ++t) {
+    out[t].d = random_depth(g, low, up);
+    perturb_normal(g, mn, angle1, angle2, out[t].n);
+  }
+}
+
+/* The refinement hypotheses of sweep pixel (row, col) after its num_samples
+ * view-selection draws: out 5 rows {nx, ny, nz, d}. */
+int pmo_refine_hypotheses(const pm_options* opt, uint32_t view_id, uint32_t pass, int W, int row, int col,
+                          const float* iK, float depth, const float* normal, float depth_min, float depth_max,
+                          float* out) {
+  pm_rng g;
+  pm_rng_init(&g, opt->seed, view_id, (uint32_t)(row * W + col), pass);
+  for (int smp = 0; smp < opt->num_samples; ++smp) (void)pm_rng_uniform(&g);
+  const float vd[3] = {iK[0] * (float)col + iK[1], iK[2] * (float)row + iK[3], 1.0f};
+  const float inv_view_len = 1.0f / sqrtf(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
+  refine_hyps(opt, &g, depth, normal, vd, inv_view_len, depth_min, (float)(1.2 * (double)depth_max),
+              (hyp4*)out);
+  return 0;
+}
+
+/* LocalOptimization's 5 trials of pixel (row, col): out 5 rows {nx, ny, nz, d}. */
+int pmo_lo_trials(int W, int Hh, const float* depth, const float* normal, int row, int col, uint64_t seed,
+                  uint32_t view_id, uint32_t pass, float* out) {
+  pm_rng g;
+  pm_rng_init(&g, seed, view_id, (uint32_t)(row * W + col), pass);
+  lo_trials(depth, normal, W, Hh, row, col, &g, (hyp4*)out);
+  return 0;
+}
+
 /* Candidate hypotheses of one sweep pixel (kernel_functions.cu:2232-2327):
  * the pixel's own (n, d); when grad_ok, up to two 4-neighbours by ascending
  * conf (bubble sort); then the least-conf sample of each of the 8
@@ -852,8 +946,6 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
 
   const float depth_min = v->depth_min;
   const float depth_max = (float)(1.2 * (double)v->depth_max);
-  const float angle1 = fminf(opt->random_angle1_range, PM_PI_F);
-  const float angle2 = fminf(opt->random_angle2_range, PM_HALF_PI_F);
   const float vd[3] = {iK[0] * (float)col + iK[1], iK[2] * (float)row + iK[3], 1.0f};
   const float inv_view_len = 1.0f / sqrtf(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
   const float np[3] = {vd[0], vd[1], 1.0f};
@@ -887,29 +979,7 @@ static void sweep_pixel(const pass_ctx* x, int row, int col) {
 
   /* refinement (kernel_functions.cu:2449-2534) */
   hyp4 ref[5];
-  float nd;
-  {
-    const float dlo = (1.0f - opt->random_depth_ratio) * depth;
-    const float dhi = (1.0f + opt->random_depth_ratio) * depth;
-    nd = random_depth(&g, dlo, dhi);
-  }
-  float nn[3];
-  perturb_normal(&g, normal, angle1, angle2, nn);
-  correct_normal(nn, vd, inv_view_len);
-  const float rd = random_depth(&g, depth_min, depth_max);
-  float rn[3];
-  random_normal(&g, vd, rn);
-  correct_normal(rn, vd, inv_view_len);
-  ref[0].d = depth;
-  memcpy(ref[0].n, normal, sizeof(normal));
-  ref[1].d = depth;
-  memcpy(ref[1].n, nn, sizeof(nn));
-  ref[2].d = nd;
-  memcpy(ref[2].n, normal, sizeof(normal));
-  ref[3].d = depth;
-  memcpy(ref[3].n, rn, sizeof(rn));
-  ref[4].d = rd;
-  memcpy(ref[4].n, normal, sizeof(normal));
+  refine_hyps(opt, &g, depth, normal, vd, inv_view_len, depth_min, depth_max, ref);
 
   float RM[5][PM_MAX_NUM_SRC];
   float rc[5];
@@ -952,37 +1022,14 @@ static void local_opt_pixel(const pass_ctx* x, int row, int col) {
   const float* iK = v->refInvK;
 
   float conf = st->conf[p];
-  float depth = st->depth[p];
-  float normal[3];
-  load_normal(st, plane, p, normal);
   const uint32_t sel = st->view_sel[p];
-  /* DEG2RAD(4.0) and DEG2RAD(2.0) are double products (kernel_functions.h:39) */
-  const float angle1 = (float)(4.0 * 0.0174532925199432);
-  const float angle2 = (float)(2.0 * 0.0174532925199432);
   const float np[3] = {iK[0] * (float)col + iK[1], iK[2] * (float)row + iK[3], 1.0f};
-
-  float low = depth, up = depth;
-  float m0 = normal[0], m1 = normal[1], m2 = normal[2];
-  for (int i = 0; i < 4; ++i) {
-    const int u = col + kNeighborOffs[i][1], w = row + kNeighborOffs[i][0];
-    if (u < 0 || u >= W || w < 0 || w >= Hh) continue;
-    const size_t q = (size_t)w * W + u;
-    const float ndp = st->depth[q];
-    m0 = m0 + st->normal[q];
-    m1 = m1 + st->normal[plane + q];
-    m2 = m2 + st->normal[2 * plane + q];
-    if (ndp <= low) low = ndp;
-    if (ndp > up) up = ndp;
-  }
-  float mn[3] = {m0, m1, m2};
-  const float inv_norm = 1.0f / sqrtf(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
-  mn[0] = mn[0] * inv_norm;
-  mn[1] = mn[1] * inv_norm;
-  mn[2] = mn[2] * inv_norm;
 
   if (sel == 0) return; /* m_weight <= 0 on the first trial */
   pm_rng g;
   pm_rng_init(&g, opt->seed, v->view_id, (uint32_t)p, x->pass);
+  hyp4 trials[5];
+  lo_trials(st->depth, st->normal, W, Hh, row, col, &g, trials);
   float M[5][PM_MAX_NUM_SRC];
   int best = -1;
   hyp4 bh;
@@ -995,9 +1042,7 @@ static void local_opt_pixel(const pass_ctx* x, int row, int col) {
   const float conf0 = conf;
   const int prune = !opt->geom_consistency || opt->geom_consistency_regularizer >= 0.0f;
   for (int t = 0; t < 5; ++t) {
-    hyp4 h;
-    h.d = random_depth(&g, low, up);
-    perturb_normal(&g, mn, angle1, angle2, h.n);
+    const hyp4 h = trials[t];
     float mconf = 0.0f, mw = 0.0f;
     int ruled_out = 0;
     for (int i = 0; i < N; ++i) {

@@ -33,9 +33,11 @@
  * integer shifts), the geometric term, the sweep's candidate gathering
  * (pmo_gather_candidates: a designed conf map whose picks follow from the
  * reference's offset table), the view selection and weighted argmin
- * (pmo_select_views: hand-computed cases), the filters and the
+ * (pmo_select_views: hand-computed cases), the refinement hypotheses and
+ * LocalOptimization's trials (pmo_refine_hypotheses, pmo_lo_trials: the
+ * ranges they are drawn from, over many pixels), the filters and the
  * sparse-model selection (pm_model_oracle.c).
- * Parity unpinned: the refinement hypotheses, LocalOptimization and the
+ * Parity unpinned: LocalOptimization's acceptance and cost refresh and the
  * iteration schedule have no independent closed form; they are checked
  * only end to end (depth against rendered ground truth) and by the
  * device's bitwise agreement.
@@ -118,6 +120,14 @@ int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, f
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
+/* Refinement hypotheses of sweep pixel (row, col) after its num_samples
+ * view-selection draws, and LocalOptimization's 5 trials of a pixel (see
+ * pm_oracle.c): out gets 5 rows {nx, ny, nz, d}. */
+int pmo_refine_hypotheses(const pm_options* opt, uint32_t view_id, uint32_t pass, int W, int row, int col,
+                          const float* iK, float depth, const float* normal, float depth_min, float depth_max,
+                          float* out);
+int pmo_lo_trials(int W, int H, const float* depth, const float* normal, int row, int col, uint64_t seed,
+                  uint32_t view_id, uint32_t pass, float* out);
 /* Candidate hypotheses of one sweep pixel (see pm_oracle.c): out gets up to
  * 11 rows {nx, ny, nz, d}; returns the count. */
 int pmo_gather_candidates(int W, int H, const float* conf, const float* depth, const float* normal, int grad_ok,
