@@ -51,6 +51,10 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_th_mc.restype = C.c_float
+        lib.pmo_th_mc.argtypes = [P(abi.pm_options), C.c_int]
+        lib.pmo_lo_decide.restype = C.c_int
+        lib.pmo_lo_decide.argtypes = [P(C.c_float), C.c_uint32, C.c_int, C.c_uint32, C.c_float, P(C.c_float)]
         lib.pmo_refine_hypotheses.restype = C.c_int
         lib.pmo_refine_hypotheses.argtypes = [P(abi.pm_options), C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                               P(C.c_float), C.c_float, P(C.c_float), C.c_float, C.c_float,
@@ -245,3 +249,20 @@ def lo_trials(depth, normal, row, col, seed, view_id, pass_id):
     lib.pmo_lo_trials(W, H, d.ctypes.data_as(P), n.ctypes.data_as(P), int(row), int(col), int(seed), int(view_id),
                       int(pass_id), out.ctypes.data_as(P))
     return out
+
+
+def th_mc(options, it):
+    """Iteration it's good-matching threshold."""
+    opt = options.to_c()
+    return load().pmo_th_mc(C.byref(opt), int(it))
+
+
+def lo_decide(M, sel, conf, live=0x1f):
+    """LocalOptimization's choice: M (5, N) trial costs.  Returns (trial or -1, conf)."""
+    M = np.asarray(M, np.float32)
+    full = np.zeros((5, abi.PM_MAX_NUM_SRC), np.float32)
+    full[:, :M.shape[1]] = M
+    out = C.c_float(0.0)
+    best = load().pmo_lo_decide(full.ctypes.data_as(C.POINTER(C.c_float)), int(sel), M.shape[1], int(live),
+                                float(conf), C.byref(out))
+    return best, out.value

@@ -662,6 +662,38 @@ static void init_pixel(const pass_ctx* x, int row, int col) {
 
 /* SweepFromCheckerBoardOpt (kernel_functions.cu:2136-2560), priors off and
  * propagate_depth off. */
+/* Iteration iter's good-matching threshold (patch_match_cuda.cu:778-779,
+ * 853): init_ncc_matching_cost * exp(-iter^2 / alpha). */
+float pmo_th_mc(const pm_options* opt, int iter) {
+  return (float)((double)opt->init_ncc_matching_cost * exp((double)(-(float)(iter * iter) / opt->alpha)));
+}
+
+/* LocalOptimization's choice among its 5 trials (kernel_functions.cu:
+ * 2860-2900): trial t's cost is the mean of M[t][i] over the selected
+ * sources (sel); trials are taken in order when their mean is below the
+ * running conf, so the least mean wins and the first of equal means stays.
+ * live masks the trials evaluated in full.  Returns the trial (-1: none)
+ * and its mean in *conf_out (conf otherwise). */
+int pmo_lo_decide(const float* M, uint32_t sel, int N, uint32_t live, float conf, float* conf_out) {
+  int best = -1;
+  for (int t = 0; t < 5; ++t) {
+    if (!((live >> t) & 1u)) continue;
+    float mconf = 0.0f, mw = 0.0f;
+    for (int i = 0; i < N; ++i) {
+      if (!((sel >> i) & 1u)) continue;
+      mconf = mconf + M[t * PM_MAX_NUM_SRC + i];
+      mw = mw + 1.0f;
+    }
+    mconf = mconf / mw;
+    if (mconf < conf) {
+      conf = mconf;
+      best = t;
+    }
+  }
+  *conf_out = conf;
+  return best;
+}
+
 /* The refinement hypotheses of a sweep pixel (kernel_functions.cu:
  * 2449-2486), drawn in order from the pixel's stream g: ref[0] the winner
  * (n, d); ref[1] (PerturbNormal(n), d); ref[2] (n, U[(1 - rho) d,
@@ -1041,30 +1073,26 @@ static void local_opt_pixel(const pass_ctx* x, int row, int col) {
   const float mw_sel = (float)__builtin_popcount(sel);
   const float conf0 = conf;
   const int prune = !opt->geom_consistency || opt->geom_consistency_regularizer >= 0.0f;
+  uint32_t live = 0;
   for (int t = 0; t < 5; ++t) {
     const hyp4 h = trials[t];
-    float mconf = 0.0f, mw = 0.0f;
+    float mconf = 0.0f;
     int ruled_out = 0;
     for (int i = 0; i < N; ++i) {
       if (!((sel >> i) & 1u)) continue;
       const float c = mv_cost(opt, v, row, col, &h, np, i, v->refK, x->cnt);
       M[t][i] = c;
       mconf = mconf + c;
-      mw = mw + 1.0f;
       if (prune && mconf / mw_sel >= conf0) {
         ruled_out = 1;
         break;
       }
     }
-    if (ruled_out) continue;
-    mconf = mconf / mw;
-    if (mconf < conf) {
-      conf = mconf;
-      bh = h;
-      best = t;
-    }
+    if (!ruled_out) live |= 1u << t;
   }
+  best = pmo_lo_decide(&M[0][0], sel, N, live, conf, &conf);
   if (best < 0) return;
+  bh = trials[best];
   st->conf[p] = conf;
   st->depth[p] = bh.d;
   store_normal(st, plane, p, bh.n);
@@ -1621,8 +1649,7 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   x.pass = PM_PASS_INIT + goff;
   run_all(&x, init_pixel, i0, i1);
   for (int iter = 0; iter < opt->num_iterations; ++iter) {
-    x.th_mc = (float)((double)opt->init_ncc_matching_cost *
-                      exp((double)(-(float)(iter * iter) / opt->alpha)));
+    x.th_mc = pmo_th_mc(opt, iter);
     for (int color = 0; color < 2; ++color) {
       x.pass = PM_PASS_SWEEP(iter, color) + goff;
       run_color(&x, sweep_pixel, color, r0, r1);

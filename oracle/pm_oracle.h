@@ -37,10 +37,12 @@
  * LocalOptimization's trials (pmo_refine_hypotheses, pmo_lo_trials: the
  * ranges they are drawn from, over many pixels), the filters and the
  * sparse-model selection (pm_model_oracle.c).
- * Parity unpinned: LocalOptimization's acceptance and cost refresh and the
- * iteration schedule have no independent closed form; they are checked
- * only end to end (depth against rendered ground truth) and by the
- * device's bitwise agreement.
+ * LocalOptimization's choice among its trials and the schedule's threshold
+ * (pmo_lo_decide, pmo_th_mc) are pinned by hand-computed cases.
+ * Parity unpinned: LocalOptimization's cost refresh for the sources it did
+ * not select, and the order of the passes, are checked only end to end
+ * (depth against rendered ground truth), by the device's bitwise agreement
+ * and by the launch counts of a run.
  */
 #ifndef SENSEMAP_PM_ORACLE_H_
 #define SENSEMAP_PM_ORACLE_H_
@@ -120,6 +122,10 @@ int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, f
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
+/* Iteration iter's good-matching threshold, and LocalOptimization's choice
+ * among its 5 trials given their cost rows (see pm_oracle.c). */
+float pmo_th_mc(const pm_options* opt, int iter);
+int pmo_lo_decide(const float* M, uint32_t sel, int N, uint32_t live, float conf, float* conf_out);
 /* Refinement hypotheses of sweep pixel (row, col) after its num_samples
  * view-selection draws, and LocalOptimization's 5 trials of a pixel (see
  * pm_oracle.c): out gets 5 rows {nx, ny, nz, d}. */
