@@ -55,10 +55,11 @@ def parse():
                     help="profiling mode: the step is the photometric pass alone, no source passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="profiling mode: one device-to-device copy of MIB MiB, then exit")
-    ap.add_argument("--workload", choices=["view", "batch", "serve", "select"], default="view",
+    ap.add_argument("--workload", choices=["view", "batch", "serve", "select", "filters"], default="view",
                     help="view: the 1920x1080 view (headline); batch: 64 orbit views, plain vs batched; "
                          "serve: Poisson arrivals into the serving loop; select: source-view selection "
-                         "on a 500-camera sparse model")
+                         "on a 500-camera sparse model; filters: RunFilter, cross filter and normal "
+                         "encoding after the path")
     ap.add_argument("--views", type=int, default=64, help="batch workload: reference views")
     ap.add_argument("--jobs", type=int, default=24, help="serve workload: jobs per arrival rate")
     ap.add_argument("--rates", default="5,20,50,200", help="serve workload: arrival rates, requests/s")
@@ -466,6 +467,106 @@ def run_source_selection(scene_mod, steps: int, with_cpu: bool) -> dict:
     return out
 
 
+def run_filters_workload(args) -> dict:
+    """The steps after the path on the 1920x1080 heightfield scene: each of the
+    11 views' photometric maps (untimed), then RunFilter of the reference
+    view against its 10 sources (geometric consistency), the cross-view
+    filter of all 11 views against the other 10, and the NormalMap encoding
+    of the reference's normals on the device.  Each is timed as the call a
+    caller makes (device-resident inputs for RunFilter and the encoding; the
+    cross filter takes host maps, so its call includes their upload) and
+    beside its CPU transcription on the same inputs, with the results
+    compared bit for bit."""
+    from oracle import binding, io_oracle
+    from sensemap_amd import abi, scene
+    from sensemap_amd import patch_match as pmm
+    from sensemap_amd.cross_filter import CrossImage, cross_filter
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    sc = scene.heightfield_scene(seed=2)
+    views = [sc.ref] + sc.srcs
+    n = len(views)
+    images = [pmm.Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
+    engine = pmm.default_engine()
+    photo_opts = c2_options(pmm, False, abi.PM_RCP_EXACT, abi.PM_STORAGE_U8)
+    progress("photometric maps of the 11 views")
+    depths, normals, ref_photo = [], [], None
+    for k in range(n):
+        prob = pmm.Problem(k, [j for j in range(n) if j != k], images, sc.depth_min, sc.depth_max)
+        e = pmm.PatchMatchACMMCuda(photo_opts, prob, engine)
+        e.Run()
+        depths.append(e.GetDepthMap())
+        normals.append(e.GetNormalMap())
+        if k == 0:
+            ref_photo = e
+        else:
+            e.close()
+    H, W = depths[0].shape
+    px = W * H
+    reps = max(args.steps, 3)
+
+    # RunFilter (k_run_filter): a fresh problem per repetition (the filter
+    # rewrites the depth map it is given)
+    fo = c2_options(pmm, True, abi.PM_RCP_EXACT, abi.PM_STORAGE_U8)
+    fo.init_depth_random = False
+    fprob = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths,
+                        normal_maps=normals)
+    times, got = [], None
+    for _ in range(reps):
+        e = pmm.PatchMatchACMMCuda(fo, fprob, engine)
+        t = time.perf_counter()
+        e.RunFilter(False)
+        times.append((time.perf_counter() - t) * 1000.0)
+        got = e.GetDepthMap()
+        e.close()
+    t = time.perf_counter()
+    want = binding.run_filter(fo, pmm.ViewDesc(fprob, fo), False, threads)
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    run_filter = {"call_ms": round(float(np.median(times)), 3), "cpu_ms": round(cpu_ms, 1), "cpu_cores": threads,
+                  "mpixel_per_s": round(px / 1e6 / (np.median(times) / 1000.0), 1),
+                  "kept": int((got > 0).sum()), "bitwise_equal": bool(np.array_equal(got.view(np.uint32),
+                                                                                     want.view(np.uint32)))}
+
+    # cross-view filter (k_cross_filter): all 11 views against the other 10
+    ims = [CrossImage(v.camera.K, v.camera.R, v.camera.T, depths[k], normals[k]) for k, v in enumerate(views)]
+    cviews = [(k, [j for j in range(n) if j != k]) for k in range(n)]
+    cross_filter(ims, cviews)
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        outs = cross_filter(ims, cviews)
+        times.append((time.perf_counter() - t) * 1000.0)
+    t = time.perf_counter()
+    couts = binding.cross_filter(ims, cviews, threads=threads)
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    cross = {"views": n, "sources_per_view": n - 1, "call_ms": round(float(np.median(times)), 3),
+             "cpu_ms": round(cpu_ms, 1), "cpu_cores": threads,
+             "mpixel_per_s": round(n * px / 1e6 / (np.median(times) / 1000.0), 1),
+             "bitwise_equal": bool(all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                                       for a, b in zip(outs, couts)))}
+
+    # NormalMap encoding on the device (k_encode_normals + read back)
+    ref_photo.GetNormalCodes()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        codes = ref_photo.GetNormalCodes()
+        times.append((time.perf_counter() - t) * 1000.0)
+    t = time.perf_counter()
+    want_codes = io_oracle.encode(ref_photo.GetNormalMap())
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    enc = {"call_ms": round(float(np.median(times)), 3), "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1,
+           "mpixel_per_s": round(px / 1e6 / (np.median(times) / 1000.0), 1),
+           "bitwise_equal": bool(np.array_equal(codes, want_codes))}
+    ref_photo.close()
+    return {"metric": "filter and encoding throughput (Mpixel/s) after the path", "value": cross["mpixel_per_s"],
+            "unit": "Mpixel/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": cross["call_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1920x1080 heightfield scene, 11 views: RunFilter of the reference (10 sources), "
+                                   "cross-view filter of all 11 views (10 sources each), normal encoding"},
+            "run_filter": run_filter, "cross_filter": cross, "normal_encoding": enc}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -479,6 +580,9 @@ def main() -> int:
         return 0
     if args.workload == "serve":
         print(json.dumps(run_serve_workload(args)))
+        return 0
+    if args.workload == "filters":
+        print(json.dumps(run_filters_workload(args)))
         return 0
     if args.workload == "select":
         from sensemap_amd import scene
@@ -575,6 +679,33 @@ def main() -> int:
         ms16 = (time.perf_counter() - t) * 1000.0 / args.steps
         f16 = {"value": round(mpix_iter / (ms16 / 1000.0), 3), "ms_per_step": round(ms16, 2),
                "depth_maps_identical": bool(np.array_equal(geo16.GetDepthMap(), geo.GetDepthMap()))}
+    # the same step with the fast reciprocal (one Newton step, within one ulp
+    # of 1/z, not bitwise): what the exact reciprocal costs
+    fast = None
+    if rcp == abi.PM_RCP_EXACT and not args.only_photometric:
+        of, gf = c2_options(pmm, False, abi.PM_RCP_FAST, storage), c2_options(pmm, True, abi.PM_RCP_FAST, storage)
+        photof = pmm.PatchMatchACMMCuda(of, ref_prob, engine)
+        geof = pmm.PatchMatchACMMCuda(gf, ref_prob, engine)
+        for i, p in enumerate(src_photo):
+            geof.SetSrcDepthFrom(i, p)
+
+        def stepf():
+            photof.Run()
+            geof.InitFrom(photof)
+            geof.Run()
+
+        for _ in range(args.warmup):
+            stepf()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            stepf()
+        msf = (time.perf_counter() - t) * 1000.0 / args.steps
+        df, de = geof.GetDepthMap(), geo.GetDepthMap()
+        m = (df > 0) & (de > 0)
+        errf = np.abs(df - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
+        fast = {"value": round(mpix_iter / (msf / 1000.0), 3), "ms_per_step": round(msf, 2),
+                "median_rel_depth_error": float(np.median(errf)),
+                "median_rel_diff_vs_exact": float(np.median(np.abs(df[m] - de[m]) / de[m]))}
     # the whole C2 pipeline once: the 10 sources' photometric passes, whose
     # depth maps feed the geometric pass, then the reference's photometric and
     # geometric passes (11 views, 8 photometric iterations each, plus 2)
@@ -613,6 +744,7 @@ def main() -> int:
         "frac_within_1pct": float(np.mean(err < 0.01)),
         "median_rel_depth_error_photometric": float(np.median(err_photo)),
         "f16_storage": f16,
+        "fast_reciprocal": fast,
         "full_pipeline": full,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
