@@ -524,7 +524,6 @@ PM_API pm_status pm_model_max_overlapping_images(pm_engine* e, const pm_model* m
     return s;
   PM_HIP(hipMemsetAsync(d_first.p, 0, (size_t)I * 4, st));
   PM_HIP(hipMemsetAsync(d_last.p, 0, (size_t)I * 4, st));
-  int64_t nranked = 0;
   if (nruns > 0) {
     b = tmp_bytes;
     PM_HIP(rocprim::radix_sort_pairs(d_tmp.p, b, (const uint64_t*)d_key.p, (uint64_t*)d_key2.p,
@@ -537,7 +536,6 @@ PM_API pm_status pm_model_max_overlapping_images(pm_engine* e, const pm_model* m
                        (const uint64_t*)d_key2.p, (const int64_t*)d_nv.p + 1, (int32_t*)d_first.p,
                        (int32_t*)d_last.p);
     PM_HIP(hipGetLastError());
-    PM_HIP(hipMemcpyAsync(&nranked, (int64_t*)d_nv.p + 1, 8, hipMemcpyDeviceToHost, st));
   }
   std::vector<int32_t> first(I), last(I);
   std::vector<uint8_t> observed(I);
@@ -608,7 +606,6 @@ PM_API pm_status pm_model_max_overlapping_images(pm_engine* e, const pm_model* m
     stats->rank_ms = tm.between(3, 4);
     stats->total_ms = tm.between(0, 4);
   }
-  (void)nranked;
   return PM_OK;
 }
 
