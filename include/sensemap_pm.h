@@ -408,6 +408,15 @@ typedef struct pm_cross_view {
 pm_status pm_cross_options_init(pm_cross_options* options);
 pm_status pm_cross_filter(pm_engine* engine, const pm_cross_options* options, int32_t num_images,
                           const pm_cross_image* images, int32_t num_views, const pm_cross_view* views);
+/* The same filter (CrossFilterCuda::Run, patch_match_cuda.cu:1493-1551) over
+ * problems of this engine whose current depth and normal maps are on the
+ * device (after pm_problem_run), with each problem's reference camera:
+ * views[k].ref and .src index the problems.  The normals are encoded on the
+ * device; only the refined maps asked for cross PCIe.  A view whose refined
+ * is NULL writes its refined map back into its problem's depth map, after
+ * every view has read its sources (two such views may not share a problem). */
+pm_status pm_cross_filter_problems(pm_engine* engine, const pm_cross_options* options, int32_t num_problems,
+                                   pm_problem* const* problems, int32_t num_views, const pm_cross_view* views);
 
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,

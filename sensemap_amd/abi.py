@@ -247,6 +247,8 @@ _SIGNATURES = {
     "pm_cross_options_init": (C.c_int, [C.POINTER(pm_cross_options)]),
     "pm_cross_filter": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_cross_image),
                                   C.c_int32, C.POINTER(pm_cross_view)]),
+    "pm_cross_filter_problems": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32,
+                                           C.POINTER(C.c_void_p), C.c_int32, C.POINTER(pm_cross_view)]),
     "pm_model_max_overlapping_images": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.c_int32, C.c_double,
                                                   C.POINTER(C.c_uint8), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                                   C.POINTER(pm_overlap_stats)]),

@@ -88,3 +88,34 @@ def cross_filter(images: List[CrossImage], views: List[Tuple[int, Sequence[int]]
     inp = CrossInputs(images, views)
     abi.check(eng.lib.pm_cross_filter(eng.handle, C.byref(opt), len(images), inp.images, len(views), inp.views))
     return inp.outputs
+
+
+def cross_filter_problems(problems, views: List[Tuple[int, Sequence[int]]],
+                          options: Optional[CrossFilterOptions] = None, write_back: bool = False,
+                          engine=None) -> List[Optional[np.ndarray]]:
+    """The filter over PatchMatchACMMCuda problems whose maps are on the device
+    (pm_cross_filter_problems).  views index `problems`.  Returns each view's
+    refined map, or, with write_back, replaces each view's problem depth map
+    on the device and returns Nones."""
+    from .patch_match import default_engine
+    eng = engine or default_engine()
+    opt = (options or CrossFilterOptions()).to_c()
+    handles = (C.c_void_p * len(problems))(*[p.handle.value for p in problems])
+    cv = (abi.pm_cross_view * len(views))()
+    keep, outs = [], []
+    for k, (ref, srcs) in enumerate(views):
+        s = (C.c_int32 * len(srcs))(*[int(x) for x in srcs])
+        keep.append(s)
+        cv[k].ref = int(ref)
+        cv[k].num_src = len(srcs)
+        cv[k].src = C.cast(s, C.POINTER(C.c_int32))
+        if write_back:
+            cv[k].refined = None
+            outs.append(None)
+        else:
+            out = np.zeros((problems[ref].H, problems[ref].W), np.float32)
+            keep.append(out)
+            outs.append(out)
+            cv[k].refined = out.ctypes.data_as(C.POINTER(C.c_float))
+    abi.check(eng.lib.pm_cross_filter_problems(eng.handle, C.byref(opt), len(problems), handles, len(views), cv))
+    return outs

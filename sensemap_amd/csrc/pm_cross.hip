@@ -178,22 +178,25 @@ PM_API pm_status pm_cross_options_init(pm_cross_options* o) {
   return PM_OK;
 }
 
-PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_t num_images,
-                                 const pm_cross_image* images, int32_t num_views, const pm_cross_view* views) {
-  if (!e || !o || !images || !views || num_images <= 0 || num_views <= 0)
-    return fail(PM_ERR_INVALID_ARG, "invalid cross-filter arguments");
+namespace {
+
+// One image as the launch reads it: device depth map and normal codes.
+struct CrossImgDev {
+  const float* depth;
+  const uint32_t* codes;
+  int32_t width, height;
+  pm_camera cam;
+};
+
+pm_status check_cross_args(const pm_cross_options* o, int32_t num_images, int32_t num_views,
+                           const pm_cross_view* views) {
+  if (!o || !views || num_images <= 0 || num_views <= 0) return fail(PM_ERR_INVALID_ARG, "invalid cross-filter arguments");
   if (!(o->filter_geom_consistency_max_cost >= 0.0f) || !(o->depth_diff_threshold >= 0.0f) ||
       !(o->max_normal_error >= 0.0f) || o->filter_min_num_consistent < 0)
     return fail(PM_ERR_INVALID_ARG, "invalid cross-filter options");
-  for (int32_t i = 0; i < num_images; ++i) {
-    const pm_cross_image& im = images[i];
-    if (im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > ((int64_t)1 << 28) || !im.depth ||
-        !im.normal_codes || !finite_camera(im.cam))
-      return fail(PM_ERR_INVALID_ARG, "invalid cross-filter image " + std::to_string(i));
-  }
   for (int32_t k = 0; k < num_views; ++k) {
     const pm_cross_view& cv = views[k];
-    if (cv.ref < 0 || cv.ref >= num_images || !cv.refined || !cv.src)
+    if (cv.ref < 0 || cv.ref >= num_images || !cv.src)
       return fail(PM_ERR_INVALID_ARG, "invalid cross-filter view " + std::to_string(k));
     if (cv.num_src < 1 || cv.num_src > kCrossMaxSrc)
       return fail(PM_ERR_TOO_MANY_SOURCES, "cross-filter views take 1..31 sources");
@@ -201,52 +204,28 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
       if (cv.src[s] < 0 || cv.src[s] >= num_images)
         return fail(PM_ERR_INVALID_ARG, "cross-filter source index out of range in view " + std::to_string(k));
   }
-  PM_HIP(hipSetDevice(e->device));
+  return PM_OK;
+}
+
+// The launch over device-resident images: builds the view table and runs
+// k_cross_filter, writing view k's refined map at d_out + out_off[k].
+// Leaves the work enqueued on the engine's stream.
+pm_status run_cross(pm_engine* e, const pm_cross_options* o, const std::vector<CrossImgDev>& imgs, int32_t num_views,
+                    const pm_cross_view* views, float* d_out, const std::vector<int64_t>& out_off, DevBuf& d_poses,
+                    DevBuf& d_tab) {
   hipStream_t st = e->stream;
-
-  // images used by some view: depth and codes once each
-  std::vector<int64_t> img_off(num_images, -1);
-  int64_t img_total = 0;
-  auto use = [&](int32_t i) {
-    if (img_off[i] < 0) {
-      img_off[i] = img_total;
-      img_total += (int64_t)images[i].width * images[i].height;
-    }
-  };
-  int64_t out_total = 0;
-  std::vector<int64_t> out_off(num_views);
-  for (int32_t k = 0; k < num_views; ++k) {
-    use(views[k].ref);
-    for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
-    out_off[k] = out_total;
-    out_total += (int64_t)images[views[k].ref].width * images[views[k].ref].height;
-  }
-  DevBuf d_depth, d_codes, d_out, d_poses, d_tab;
-  PM_HIP(hipMalloc(&d_depth.p, (size_t)img_total * sizeof(float)));
-  PM_HIP(hipMalloc(&d_codes.p, (size_t)img_total * sizeof(uint32_t)));
-  PM_HIP(hipMalloc(&d_out.p, (size_t)out_total * sizeof(float)));
-  for (int32_t i = 0; i < num_images; ++i) {
-    if (img_off[i] < 0) continue;
-    const size_t n = (size_t)images[i].width * images[i].height;
-    PM_HIP(hipMemcpyAsync((float*)d_depth.p + img_off[i], images[i].depth, n * sizeof(float), hipMemcpyHostToDevice,
-                          st));
-    PM_HIP(hipMemcpyAsync((uint32_t*)d_codes.p + img_off[i], images[i].normal_codes, n * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, st));
-  }
-
-  // poses and the view table
   std::vector<float> poses;
   std::vector<CrossViewDev> tab(num_views);
   std::vector<size_t> pose_off(num_views);
   int64_t tiles = 0;
   for (int32_t k = 0; k < num_views; ++k) {
     const pm_cross_view& cv = views[k];
-    const pm_cross_image& ref = images[cv.ref];
+    const CrossImgDev& ref = imgs[cv.ref];
     pm_view_desc vd{};
     std::vector<pm_image> srcs(cv.num_src);
     std::vector<pm_camera> cams(cv.num_src);
     for (int32_t s = 0; s < cv.num_src; ++s) {
-      const pm_cross_image& si = images[cv.src[s]];
+      const CrossImgDev& si = imgs[cv.src[s]];
       srcs[s].width = si.width;
       srcs[s].height = si.height;
       cams[s] = si.cam;
@@ -267,15 +246,15 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
     t.min_consistent = std::min(o->filter_min_num_consistent, cv.num_src);
     std::memcpy(t.K, k8, sizeof(t.K));
     std::memcpy(t.iK, k8 + 4, sizeof(t.iK));
-    t.depth = (const float*)d_depth.p + img_off[cv.ref];
-    t.codes = (const uint32_t*)d_codes.p + img_off[cv.ref];
-    t.out = (float*)d_out.p + out_off[k];
+    t.depth = ref.depth;
+    t.codes = ref.codes;
+    t.out = d_out + out_off[k];
     for (int32_t s = 0; s < cv.num_src; ++s) {
-      const int32_t i = cv.src[s];
-      t.src[s].depth = (const float*)d_depth.p + img_off[i];
-      t.src[s].codes = (const uint32_t*)d_codes.p + img_off[i];
-      t.src[s].width = images[i].width;
-      t.src[s].height = images[i].height;
+      const CrossImgDev& si = imgs[cv.src[s]];
+      t.src[s].depth = si.depth;
+      t.src[s].codes = si.codes;
+      t.src[s].width = si.width;
+      t.src[s].height = si.height;
     }
     t.tile_start = (int32_t)tiles;
     tiles += (int64_t)((ref.width + kCrossTile - 1) / kCrossTile) * ((ref.height + kCrossTile - 1) / kCrossTile);
@@ -296,10 +275,138 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
   hipLaunchKernelGGL(k_cross_filter, dim3((unsigned)tiles), dim3(kCrossTile * kCrossTile), 0, st,
                      (const CrossViewDev*)d_tab.p, prm);
   PM_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+// Output offsets of the views' refined maps in one device buffer.
+std::vector<int64_t> out_offsets(const std::vector<CrossImgDev>& imgs, int32_t num_views, const pm_cross_view* views,
+                                 int64_t* total) {
+  std::vector<int64_t> off(num_views);
+  int64_t t = 0;
+  for (int32_t k = 0; k < num_views; ++k) {
+    off[k] = t;
+    t += (int64_t)imgs[views[k].ref].width * imgs[views[k].ref].height;
+  }
+  *total = t;
+  return off;
+}
+
+}  // namespace
+
+PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_t num_images,
+                                 const pm_cross_image* images, int32_t num_views, const pm_cross_view* views) {
+  if (!e || !images) return fail(PM_ERR_INVALID_ARG, "invalid cross-filter arguments");
+  if (pm_status s = check_cross_args(o, num_images, num_views, views)) return s;
+  for (int32_t k = 0; k < num_views; ++k)
+    if (!views[k].refined) return fail(PM_ERR_INVALID_ARG, "invalid cross-filter view " + std::to_string(k));
+  for (int32_t i = 0; i < num_images; ++i) {
+    const pm_cross_image& im = images[i];
+    if (im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > ((int64_t)1 << 28) || !im.depth ||
+        !im.normal_codes || !finite_camera(im.cam))
+      return fail(PM_ERR_INVALID_ARG, "invalid cross-filter image " + std::to_string(i));
+  }
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+
+  // images used by some view: depth and codes uploaded once each
+  std::vector<int64_t> img_off(num_images, -1);
+  int64_t img_total = 0;
+  auto use = [&](int32_t i) {
+    if (img_off[i] < 0) {
+      img_off[i] = img_total;
+      img_total += (int64_t)images[i].width * images[i].height;
+    }
+  };
+  for (int32_t k = 0; k < num_views; ++k) {
+    use(views[k].ref);
+    for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
+  }
+  DevBuf d_depth, d_codes, d_out, d_poses, d_tab;
+  PM_HIP(hipMalloc(&d_depth.p, (size_t)img_total * sizeof(float)));
+  PM_HIP(hipMalloc(&d_codes.p, (size_t)img_total * sizeof(uint32_t)));
+  std::vector<CrossImgDev> imgs(num_images);
+  for (int32_t i = 0; i < num_images; ++i) {
+    imgs[i].width = images[i].width;
+    imgs[i].height = images[i].height;
+    imgs[i].cam = images[i].cam;
+    if (img_off[i] < 0) continue;
+    const size_t n = (size_t)images[i].width * images[i].height;
+    imgs[i].depth = (const float*)d_depth.p + img_off[i];
+    imgs[i].codes = (const uint32_t*)d_codes.p + img_off[i];
+    PM_HIP(hipMemcpyAsync((float*)d_depth.p + img_off[i], images[i].depth, n * sizeof(float), hipMemcpyHostToDevice,
+                          st));
+    PM_HIP(hipMemcpyAsync((uint32_t*)d_codes.p + img_off[i], images[i].normal_codes, n * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, st));
+  }
+  int64_t out_total = 0;
+  const std::vector<int64_t> off = out_offsets(imgs, num_views, views, &out_total);
+  PM_HIP(hipMalloc(&d_out.p, (size_t)out_total * sizeof(float)));
+  if (pm_status s = run_cross(e, o, imgs, num_views, views, (float*)d_out.p, off, d_poses, d_tab)) return s;
   for (int32_t k = 0; k < num_views; ++k) {
     const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
-    PM_HIP(hipMemcpyAsync(views[k].refined, (const float*)d_out.p + out_off[k], n * sizeof(float),
-                          hipMemcpyDeviceToHost, st));
+    PM_HIP(hipMemcpyAsync(views[k].refined, (const float*)d_out.p + off[k], n * sizeof(float), hipMemcpyDeviceToHost,
+                          st));
+  }
+  PM_HIP(hipStreamSynchronize(st));
+  return PM_OK;
+}
+
+PM_API pm_status pm_cross_filter_problems(pm_engine* e, const pm_cross_options* o, int32_t num_problems,
+                                          pm_problem* const* problems, int32_t num_views, const pm_cross_view* views) {
+  if (!e || !problems) return fail(PM_ERR_INVALID_ARG, "invalid cross-filter arguments");
+  if (pm_status s = check_cross_args(o, num_problems, num_views, views)) return s;
+  for (int32_t i = 0; i < num_problems; ++i) {
+    const pm_problem* p = problems[i];
+    if (!p || p->engine != e || !p->host.depth || !p->host.normal || !finite_camera(p->ref_cam))
+      return fail(PM_ERR_INVALID_ARG, "invalid cross-filter problem " + std::to_string(i) +
+                                          " (null, on another engine, or without maps)");
+  }
+  for (int32_t a = 0; a < num_views; ++a)
+    for (int32_t b = a + 1; b < num_views; ++b)
+      if (!views[a].refined && views[a].ref == views[b].ref && !views[b].refined)
+        return fail(PM_ERR_INVALID_ARG, "two views write back into the same problem");
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+
+  // normal codes of the problems that some view reads, encoded on the device
+  std::vector<int64_t> code_off(num_problems, -1);
+  int64_t code_total = 0;
+  auto use = [&](int32_t i) {
+    if (code_off[i] < 0) {
+      code_off[i] = code_total;
+      code_total += (int64_t)problems[i]->W * problems[i]->H;
+    }
+  };
+  for (int32_t k = 0; k < num_views; ++k) {
+    use(views[k].ref);
+    for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
+  }
+  DevBuf d_codes, d_out, d_poses, d_tab;
+  PM_HIP(hipMalloc(&d_codes.p, (size_t)std::max<int64_t>(code_total, 1) * sizeof(uint32_t)));
+  std::vector<CrossImgDev> imgs(num_problems);
+  for (int32_t i = 0; i < num_problems; ++i) {
+    const pm_problem* p = problems[i];
+    imgs[i].width = p->W;
+    imgs[i].height = p->H;
+    imgs[i].cam = p->ref_cam;
+    imgs[i].depth = p->host.depth;
+    if (code_off[i] < 0) continue;
+    uint32_t* c = (uint32_t*)d_codes.p + code_off[i];
+    imgs[i].codes = c;
+    PM_HIP(encode_normals_async(p->host.normal, (int64_t)p->W * p->H, c, st));
+  }
+  int64_t out_total = 0;
+  const std::vector<int64_t> off = out_offsets(imgs, num_views, views, &out_total);
+  PM_HIP(hipMalloc(&d_out.p, (size_t)std::max<int64_t>(out_total, 1) * sizeof(float)));
+  if (pm_status s = run_cross(e, o, imgs, num_views, views, (float*)d_out.p, off, d_poses, d_tab)) return s;
+  // after the launch (every view has read its sources): results out
+  for (int32_t k = 0; k < num_views; ++k) {
+    const pm_problem* p = problems[views[k].ref];
+    const size_t bytes = (size_t)p->W * p->H * sizeof(float);
+    if (views[k].refined)
+      PM_HIP(hipMemcpyAsync(views[k].refined, (const float*)d_out.p + off[k], bytes, hipMemcpyDeviceToHost, st));
+    else
+      PM_HIP(hipMemcpyAsync(p->host.depth, (const float*)d_out.p + off[k], bytes, hipMemcpyDeviceToDevice, st));
   }
   PM_HIP(hipStreamSynchronize(st));
   return PM_OK;

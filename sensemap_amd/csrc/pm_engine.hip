@@ -409,6 +409,7 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   p->H = v->ref.height;
   p->N = v->num_src;
   p->view_id = v->view_id;
+  p->ref_cam = v->ref_cam;
   p->depth_min = v->depth_min;
   p->depth_max = v->depth_max;
   if (env.arena) {

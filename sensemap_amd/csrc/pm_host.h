@@ -109,6 +109,7 @@ struct pm_problem {
   unsigned long long* counters = nullptr;
   std::vector<float> poses;      // N * PM_POSE_STRIDE
   float refk8[8] = {};
+  pm_camera ref_cam{};           // the reference view's camera as given
   pm_run_stats stats{};
   size_t device_bytes = 0;
   struct pm_schedule* schedule = nullptr;
@@ -172,6 +173,9 @@ pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N);
 // Validation shared by every entry point that takes a view.
 pm_status check_view(const pm_options* o, const pm_view_desc* v);
 bool finite_camera(const pm_camera& c);
+// NormalMap codes of a planar normal map on the device (k_encode_normals),
+// enqueued on s.
+hipError_t encode_normals_async(const float* normal, int64_t plane, uint32_t* out, hipStream_t s);
 // InitTransforms (patch_match_cuda.cu:493-580): PM_POSE_STRIDE floats per
 // source, FP64 algebra rounded once, and the ref {fx,cx,fy,cy, 1/fx,...}.
 void pack_poses(const pm_view_desc* view, float* poses, float k8[8]);

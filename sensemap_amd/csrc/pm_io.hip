@@ -150,6 +150,14 @@ PM_API pm_status pm_normal_decode(int64_t pixels, const uint32_t* codes, float* 
   return PM_OK;
 }
 
+namespace pmh {
+hipError_t encode_normals_async(const float* normal, int64_t plane, uint32_t* out, hipStream_t s) {
+  if (plane <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_encode_normals, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, normal, plane, out);
+  return hipGetLastError();
+}
+}  // namespace pmh
+
 PM_API pm_status pm_problem_get_normal_codes(const pm_problem* p, uint32_t* out) {
   if (!p || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
   PM_HIP(hipSetDevice(p->engine->device));

@@ -7,6 +7,7 @@ import pytest
 from oracle import binding
 from sensemap_amd import scene
 from sensemap_amd.cross_filter import CrossFilterOptions, CrossImage, cross_filter
+from tests import helpers
 
 pytestmark = pytest.mark.gpu
 
@@ -29,3 +30,35 @@ def test_cross_filter_equals_transcription():
             diff = int(np.sum(a.view(np.uint32) != b.view(np.uint32)))
             print("[cross] view", k, a.shape, "kept", int((a > 0).sum()), "diff", diff, flush=True)
             assert diff == 0, k
+
+
+def test_cross_filter_over_device_problems_equals_host_maps():
+    """pm_cross_filter_problems (maps already on the device) gives the same
+    refined maps as pm_cross_filter on the read-back maps, bit for bit, and
+    its write-back replaces each problem's depth map with them."""
+    from sensemap_amd import patch_match as pmm
+    from sensemap_amd.cross_filter import cross_filter_problems
+    sc = scene.planes_scene(seed=6, width=96, height=72, f=80.0, num_src=4)
+    views = [sc.ref] + sc.srcs
+    n = len(views)
+    images = [pmm.Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
+    opts = helpers.c1_options(num_iterations=3)
+    probs = []
+    for k in range(n):
+        prob = pmm.Problem(k, [j for j in range(n) if j != k], images, sc.depth_min, sc.depth_max)
+        e = pmm.PatchMatchACMMCuda(opts, prob)
+        e.Run()
+        probs.append(e)
+    cviews = [(k, [j for j in range(n) if j != k][: 2 + k % 3]) for k in range(n)]
+    ims = [CrossImage(v.camera.K, v.camera.R, v.camera.T, p.GetDepthMap(), p.GetNormalMap())
+           for v, p in zip(views, probs)]
+    want = cross_filter(ims, cviews)
+    got = cross_filter_problems(probs, cviews)
+    for k, (a, b) in enumerate(zip(got, want)):
+        diff = int(np.sum(a.view(np.uint32) != b.view(np.uint32)))
+        print("[cross-dev] view", k, "kept", int((a > 0).sum()), "diff", diff, flush=True)
+        assert diff == 0
+    cross_filter_problems(probs, cviews, write_back=True)
+    for k, p in enumerate(probs):
+        assert np.array_equal(p.GetDepthMap().view(np.uint32), want[k].view(np.uint32))
+        p.close()
