@@ -142,3 +142,19 @@ def test_ragged_shapes_and_source_counts_bitwise(width, height, num_src):
     sc = scene.planes_scene(seed=5, width=width, height=height, f=0.8 * max(width, height), num_src=num_src)
     gpu, cpu = run_both(sc, helpers.c1_options(num_iterations=4))
     assert_bitwise(gpu, cpu)
+
+
+def test_maximum_source_count_bitwise():
+    # PM_MAX_NUM_SRC = 32 sources: every bit of view_sel is a source
+    sc = scene.planes_scene(seed=7, width=64, height=48, f=50.0, num_src=32)
+    gpu, cpu = run_both(sc, helpers.c1_options(num_iterations=3))
+    assert_bitwise(gpu, cpu)
+    assert (np.asarray(gpu["view_sel"]) >> 16).any()
+
+
+def test_maximum_window_radius_bitwise():
+    # PM_MAX_WINDOW_RADIUS = 32 (65x65 window, step 2: 33x33 samples) on the runtime-shape path
+    sc = helpers.small_planes(seed=3, width=64, height=48, f=50.0)
+    gpu, cpu = run_both(sc, helpers.c1_options(window_radius=32, window_step=2, sigma_spatial=12.0,
+                                               num_iterations=2))
+    assert_bitwise(gpu, cpu)
