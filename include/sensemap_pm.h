@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 1
+#define PM_ABI_VERSION 2
 #define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
 #define PM_MAX_WINDOW_RADIUS 32
 #define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
@@ -63,6 +63,7 @@ typedef struct pm_options {
   int32_t geom_consistency;           /* 1 */
   float geom_consistency_regularizer; /* 0.3 */
   float geom_consistency_max_cost;    /* 3.0 */
+  int32_t est_curvature;              /* 0: ComputeCurvatureMap runs when geom_consistency && est_curvature (:71) */
   int32_t random_optimization;        /* 1 */
   int32_t local_optimization;         /* 1 */
   float random_depth_ratio;           /* 0.004 */
@@ -133,6 +134,10 @@ typedef struct pm_view_desc {
    * planar (Problem::depth_maps / normal_maps at ref_image_idx). */
   const float* init_depth;
   const float* init_normal;
+  /* Problem::mask_maps at ref_image_idx (patch_match_cuda.cu:113,173-196):
+   * an 8-bit mask of the reference image's size; data NULL = no mask.  When
+   * given, the run ends with MaskDepthMapKernel (:1040-1052). */
+  pm_image ref_mask;
 } pm_view_desc;
 
 /* Caller-allocated outputs; a NULL pointer skips that map. */
@@ -142,6 +147,7 @@ typedef struct pm_outputs {
   float* conf;         /* W x H */
   float* cost;         /* num_src x W x H planar: per-source cost of the winning hypothesis */
   uint32_t* view_sel;  /* W x H bitmask of selected source views */
+  uint16_t* curvature; /* 2 x W x H planar {gradient, curvature}; written when geom_consistency && est_curvature */
 } pm_outputs;
 
 /* Timings of the last pm_problem_run, measured with hipEvents on the
@@ -195,6 +201,14 @@ pm_status pm_problem_get_normal_map(const pm_problem* problem, float* out);
 pm_status pm_problem_get_conf_map(const pm_problem* problem, float* out);
 pm_status pm_problem_get_cost_map(const pm_problem* problem, float* out);
 pm_status pm_problem_get_view_sel(const pm_problem* problem, uint32_t* out);
+/* GetCurvatureMap (patch_match_cuda.h:162, patch_match_cuda.cu:444-446):
+ * the 2 x W x H planar unsigned-short map of ComputeCurvatureMap
+ * (kernel_functions.cu:1603-1651), slice 0 = min(10000 |Scharr gradient| of
+ * the reference image in [0,1] units, 32767), slice 1 = 10000 x the median
+ * angle (radians) between the pixel's normal and its 5x5 neighbours'.
+ * PM_ERR_INVALID_ARG unless the problem was run with geom_consistency and
+ * est_curvature (the reference allocates the map only then, :682-684). */
+pm_status pm_problem_get_curvature_map(const pm_problem* problem, uint16_t* out);
 pm_status pm_problem_get_stats(const pm_problem* problem, pm_run_stats* out);
 /* Packed per-source pose block (PM_POSE_STRIDE floats per source) and the
  * reference {fx,cx,fy,cy, 1/fx,-cx/fx,1/fy,-cy/fy} (InitTransforms,
@@ -258,7 +272,9 @@ typedef enum pm_phase {
   PM_PHASE_INIT = 0,
   PM_PHASE_SWEEP = 1,
   PM_PHASE_LOCAL = 2,
-  PM_PHASE_FILTER = 3
+  PM_PHASE_FILTER = 3,
+  PM_PHASE_CURVATURE = 4,
+  PM_PHASE_MASK = 5
 } pm_phase;
 
 typedef struct pm_tuning {

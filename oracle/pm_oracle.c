@@ -1598,6 +1598,91 @@ int pmo_cross_filter(const pm_cross_options* o, int num_images, const pm_cross_i
 }
 
 /* ------------------------------------------------------------------------ */
+/* ComputeCurvatureMap: kernel_functions.cu:1603-1651 */
+
+/* float -> unsigned short of the reference's GpuMat<unsigned short>::Set:
+ * toward zero; NaN and negatives give 0, values above 65535 saturate. */
+static uint16_t store_u16(float x) {
+  if (x != x || x <= 0.0f) return 0;
+  if (x >= 65535.0f) return 65535;
+  return (uint16_t)x;
+}
+
+/* utils.h:206-221, ascending */
+static void bubble_sort(float* vals, int length) {
+  for (int i = 0; i < length - 1; ++i)
+    for (int j = 0; j < length - 1 - i; ++j)
+      if (vals[j] > vals[j + 1]) {
+        const float t = vals[j];
+        vals[j] = vals[j + 1];
+        vals[j + 1] = t;
+      }
+}
+
+/* filter_kernel_x / filter_kernel_y, kernel_functions.cu:56-60 */
+static const float kScharrX[3][3] = {{-3, 0, 3}, {-10, 0, 10}, {-3, 0, 3}};
+static const float kScharrY[3][3] = {{-3, -10, -3}, {0, 0, 0}, {3, 10, 3}};
+
+int pmo_curvature_map(const uint8_t* ref, int W, int H, const float* normal, uint16_t* out) {
+  if (!ref || !normal || !out || W <= 0 || H <= 0) return -1;
+  const long plane = (long)W * H;
+#pragma omp parallel for schedule(static)
+  for (int row = 0; row < H; ++row) {
+    for (int col = 0; col < W; ++col) {
+      /* tex2D with border addressing and point filtering on the 8-bit image
+       * read as normalised float: v / 255 inside, 0 outside */
+      float conv_x = 0, conv_y = 0;
+      for (int dr = -1; dr <= 1; ++dr) {
+        for (int dc = -1; dc <= 1; ++dc) {
+          const int x = col + dc, y = row + dr;
+          const float color = (x < 0 || x >= W || y < 0 || y >= H) ? 0.0f : (float)ref[(long)y * W + x] / 255.0f;
+          conv_x += color * kScharrX[dr + 1][dc + 1];
+          conv_y += color * kScharrY[dr + 1][dc + 1];
+        }
+      }
+      const float grad = sqrtf(conv_x * conv_x + conv_y * conv_y);
+
+      const int min_x = col - 2 > 0 ? col - 2 : 0;
+      const int max_x = col + 2 < W - 1 ? col + 2 : W - 1;
+      const int min_y = row - 2 > 0 ? row - 2 : 0;
+      const int max_y = row + 2 < H - 1 ? row + 2 : H - 1;
+      const long p = (long)row * W + col;
+      const float n[3] = {normal[p], normal[plane + p], normal[2 * plane + p]};
+      int counter = 0;
+      float angles[24];
+      for (int y = min_y; y <= max_y; ++y) {
+        for (int x = min_x; x <= max_x; ++x) {
+          if (y == row && x == col) continue;
+          const long q = (long)y * W + x;
+          const float dr = n[0] * normal[q] + n[1] * normal[plane + q] + n[2] * normal[2 * plane + q];
+          /* acos is NaN outside [-1, 1] */
+          angles[counter++] = (dr >= -1.0f && dr <= 1.0f) ? pm_acosf(dr) : NAN;
+        }
+      }
+      const float g = grad * 10000.0f;
+      out[p] = store_u16(g < 32767.0f ? g : 32767.0f);
+      if (counter > 0) {
+        bubble_sort(angles, counter);
+        out[plane + p] = store_u16(angles[counter / 2] * 10000.0f);
+      } else {
+        out[plane + p] = store_u16((float)(M_PI * 10000.0));
+      }
+    }
+  }
+  return 0;
+}
+
+/* MaskDepthMapKernel: kernel_functions.cu:3080-3093 (the last row and
+ * column are left as they are) */
+int pmo_mask_depth(const uint8_t* mask, int W, int H, float* depth) {
+  if (!mask || !depth || W <= 0 || H <= 0) return -1;
+  for (int row = 0; row < H - 1; ++row)
+    for (int col = 0; col < W - 1; ++col)
+      if (mask[(long)row * W + col] == 0) depth[(long)row * W + col] = 0.0f;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Run(): patch_match_cuda.cu:748-1057 */
 int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, int row_begin,
                  int row_end, int num_threads, pmo_counters* counters) {
@@ -1661,8 +1746,17 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
       }
     }
   }
+  if (opt->geom_consistency && opt->est_curvature && out->curvature)
+    pmo_curvature_map(v.ref, v.W, v.H, st.normal, out->curvature);
   if (opt->filter && opt->conf_filter) {
     for (int color = 0; color < 2; ++color) run_color(&x, filter_pixel, color, r0, r1);
+  }
+  if (view->ref_mask.data) {
+    uint8_t* m = (uint8_t*)malloc(plane);
+    for (int r = 0; r < v.H; ++r)
+      memcpy(m + (size_t)r * v.W, view->ref_mask.data + (size_t)r * view->ref_mask.stride, v.W);
+    pmo_mask_depth(m, v.W, v.H, st.depth);
+    free(m);
   }
   if (counters) *counters = local;
   if (out->depth) memcpy(out->depth, st.depth, sizeof(float) * plane);

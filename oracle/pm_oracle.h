@@ -14,9 +14,11 @@
  *   src/mvs/kernel_functions.cu:689-737     InitNormalMap, InitDepthMap
  *   src/mvs/kernel_functions.cu:1245-1340   ComputeNCCCost<R,S>
  *   src/mvs/kernel_functions.cu:1447-1601   ComputeMultiViewCostVector, InitConfMapKernel
+ *   src/mvs/kernel_functions.cu:1603-1651   ComputeCurvatureMap (+ BubbleSort, utils.h:206-221)
  *   src/mvs/kernel_functions.cu:1655-1675   ComputeGradientMap
  *   src/mvs/kernel_functions.cu:2136-2560   SweepFromCheckerBoardOpt
  *   src/mvs/kernel_functions.cu:2736-2931   LocalOptimization
+ *   src/mvs/kernel_functions.cu:3080-3093   MaskDepthMapKernel
  *   src/mvs/kernel_functions.cu:3282-3354   FilterDepthMap
  *   src/mvs/patch_match_cuda.cu:493-580     InitTransforms (pose packing)
  *   src/mvs/patch_match_cuda.cu:582-698     InitWorkspace
@@ -38,7 +40,12 @@
  * ranges they are drawn from, over many pixels), the filters and the
  * sparse-model selection (pm_model_oracle.c).
  * LocalOptimization's choice among its trials and the schedule's threshold
- * (pmo_lo_decide, pmo_th_mc) are pinned by hand-computed cases.
+ * (pmo_lo_decide, pmo_th_mc) are pinned by hand-computed cases.  The
+ * curvature map (pmo_curvature_map) is pinned by closed forms (constant and
+ * ramp images, a normal field rotating per column), scipy's correlation, a
+ * float64 brute-force median and a designed case of NaN angles; the mask
+ * (pmo_mask_depth) by its skipped last row and column and its place after
+ * FilterDepthMap in a run.
  * Parity unpinned: LocalOptimization's cost refresh for the sources it did
  * not select, and the order of the passes, are checked only end to end
  * (depth against rendered ground truth), by the device's bitwise agreement
@@ -154,6 +161,12 @@ float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, in
                     float depth, int src);
 
 /* Deterministic math and RNG, exported for the tests. */
+/* ComputeCurvatureMap (kernel_functions.cu:1603-1651): out is 2 x W x H
+ * planar {gradient, curvature}; ref is the W x H 8-bit reference image and
+ * normal the 3 x W x H planar normal map. */
+int pmo_curvature_map(const uint8_t* ref, int W, int H, const float* normal, uint16_t* out);
+/* MaskDepthMapKernel (kernel_functions.cu:3080-3093) on a W x H depth map. */
+int pmo_mask_depth(const uint8_t* mask, int W, int H, float* depth);
 float pmo_expf(float x);
 float pmo_sinf(float x);
 float pmo_cosf(float x);

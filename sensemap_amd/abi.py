@@ -35,6 +35,7 @@ class pm_options(C.Structure):
         ("geom_consistency", C.c_int32),
         ("geom_consistency_regularizer", C.c_float),
         ("geom_consistency_max_cost", C.c_float),
+        ("est_curvature", C.c_int32),
         ("random_optimization", C.c_int32),
         ("local_optimization", C.c_int32),
         ("random_depth_ratio", C.c_float),
@@ -92,6 +93,7 @@ class pm_view_desc(C.Structure):
         ("src_depth_maps", C.POINTER(C.POINTER(C.c_float))),
         ("init_depth", C.POINTER(C.c_float)),
         ("init_normal", C.POINTER(C.c_float)),
+        ("ref_mask", pm_image),
     ]
 
 
@@ -102,6 +104,7 @@ class pm_outputs(C.Structure):
         ("conf", C.POINTER(C.c_float)),
         ("cost", C.POINTER(C.c_float)),
         ("view_sel", C.POINTER(C.c_uint32)),
+        ("curvature", C.POINTER(C.c_uint16)),
     ]
 
 
@@ -123,7 +126,7 @@ class pm_run_stats(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
-PM_PHASE_INIT, PM_PHASE_SWEEP, PM_PHASE_LOCAL, PM_PHASE_FILTER = 0, 1, 2, 3
+PM_PHASE_INIT, PM_PHASE_SWEEP, PM_PHASE_LOCAL, PM_PHASE_FILTER, PM_PHASE_CURVATURE, PM_PHASE_MASK = 0, 1, 2, 3, 4, 5
 
 
 class pm_tuning(C.Structure):
@@ -206,6 +209,7 @@ _SIGNATURES = {
     "pm_problem_get_conf_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pm_problem_get_cost_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "pm_problem_get_view_sel": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pm_problem_get_curvature_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16)]),
     "pm_problem_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_run_stats)]),
     "pm_problem_get_poses": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pm_problem_destroy": (None, [C.c_void_p]),

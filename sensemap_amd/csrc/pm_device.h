@@ -63,6 +63,8 @@ struct ViewDev {
   float* conf;              // W*H
   float* cost;              // N*W*H planar
   uint32_t* view_sel;       // W*H
+  uint16_t* curv;           // 2*W*H planar {gradient, curvature} or null (no curvature pass)
+  const uint8_t* mask;      // W*H depth mask or null
   unsigned long long* counters;  // [ncc_eval, ncc_samples, geo_eval] or null
   SrcDev src[PM_MAX_NUM_SRC];
 };
@@ -134,7 +136,16 @@ hipError_t launch_ref_moments(const ViewDev* v, int W, int H, int accum_mode, fl
 // Table-driven colour-pass launches: `table` is a device copy of a
 // LaunchTable whose queue counters are zero; `grid` comes from phase_grid
 // for the table's total_tiles (and sized its ws_slots).
-enum PhaseKind { kPhaseInit = 0, kPhaseSweep = 1, kPhaseLocal = 2, kPhaseFilter = 3 };
+enum PhaseKind {
+  kPhaseInit = 0,
+  kPhaseSweep = 1,
+  kPhaseLocal = 2,
+  kPhaseFilter = 3,     // FilterDepthMap, one colour per entry
+  kPhaseCurvature = 4,  // ComputeCurvatureMap, one colour per entry
+  kPhaseMask = 5        // MaskDepthMapKernel, one colour per entry
+};
+// Phases with no window, storage or reciprocal variants.
+inline bool plain_phase(PhaseKind k) { return k == kPhaseFilter || k == kPhaseCurvature || k == kPhaseMask; }
 // Persistent grid for a launch of `tiles` tiles on the current device: the
 // number of CUs times the workgroups one CU holds (capped by blocks_per_cu
 // when > 0), and never more than the tiles.

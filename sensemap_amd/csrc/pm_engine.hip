@@ -122,6 +122,7 @@ PM_API pm_status pm_options_init(pm_options* o) {
   o->init_depth_random = 0;  // = !geom_consistency
   o->filter = 1;
   o->conf_filter = 0;
+  o->est_curvature = 0;
   o->conf_threshold = 0.5f;
   o->depth_diff_threshold = 0.02f;
   o->filter_geom_consistency_max_cost = 1.0f;
@@ -330,6 +331,9 @@ pm_status check_view(const pm_options* o, const pm_view_desc* v) {
   }
   if (!o->init_depth_random && (!v->init_depth || !v->init_normal))
     return fail(PM_ERR_INVALID_ARG, "init_depth_random = 0 needs init_depth and init_normal");
+  if (v->ref_mask.data && (v->ref_mask.width != v->ref.width || v->ref_mask.height != v->ref.height ||
+                           v->ref_mask.stride < v->ref_mask.width))
+    return fail(PM_ERR_INVALID_ARG, "mask map must have the reference image's size");
   return PM_OK;
 }
 
@@ -366,6 +370,8 @@ size_t job_block_bytes(const pm_options* o, const pm_view_desc* v, const bool* s
   add((size_t)N * plane * sizeof(float));             // cost
   add(plane * sizeof(uint32_t));                      // view_sel
   add(6 * sizeof(unsigned long long));                // counters
+  if (o->geom_consistency && o->est_curvature) add(2 * plane * sizeof(uint16_t));  // curvature map
+  if (v->ref_mask.data) add(plane);                   // mask
   if (!o->init_depth_random) {
     add(plane * sizeof(float));
     add(3 * plane * sizeof(float));
@@ -506,6 +512,16 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   if ((st = p->alloc(&h.cost, (size_t)N * plane)) != PM_OK) return st;
   if ((st = p->alloc(&h.view_sel, plane)) != PM_OK) return st;
   if ((st = p->alloc(&p->counters, 6)) != PM_OK) return st;
+  if (o->geom_consistency && o->est_curvature) {
+    if ((st = p->alloc(&h.curv, 2 * plane)) != PM_OK) return st;
+    PM_HIP(hipMemsetAsync(h.curv, 0, 2 * plane * sizeof(uint16_t), s));
+  }
+  if (v->ref_mask.data) {
+    uint8_t* m = nullptr;
+    if ((st = p->alloc(&m, plane)) != PM_OK) return st;
+    PM_HIP(hipMemcpy2DAsync(m, W, v->ref_mask.data, v->ref_mask.stride, W, H, hipMemcpyHostToDevice, s));
+    h.mask = m;
+  }
   if (!o->init_depth_random) {
     if ((st = p->alloc(&p->init_depth, plane)) != PM_OK) return st;
     if ((st = p->alloc(&p->init_normal, 3 * plane)) != PM_OK) return st;
@@ -594,7 +610,8 @@ float th_mc_of(const pm_options& o, int iter) {
   return (float)((double)o.init_ncc_matching_cost * std::exp((double)(-(float)(iter * iter) / o.alpha)));
 }
 
-std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o) {
+std::vector<std::vector<PhaseRec>> view_ticks(const pm_problem& p) {
+  const pm_options& o = p.opt;
   std::vector<std::vector<PhaseRec>> t;
   const uint32_t goff = o.geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
   // both colours of the init in one tick (init touches only its own pixel)
@@ -605,8 +622,13 @@ std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o) {
     if (o.local_optimization && iter > 2)
       for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseLocal, c, PM_PASS_LOCAL(iter, c) + goff, 0.0f}});
   }
+  // after PropagationAndOptim: curvature (:985-989), FilterDepthMap (:1015-1037),
+  // mask (:1040-1052); the curvature and mask passes touch only their own
+  // pixel's outputs, so both colours share a tick
+  if (p.host.curv) t.push_back({{pm::kPhaseCurvature, 0, 0u, 0.0f}, {pm::kPhaseCurvature, 1, 0u, 0.0f}});
   if (o.filter && o.conf_filter)
     for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseFilter, c, 0u, 0.0f}});
+  if (p.host.mask) t.push_back({{pm::kPhaseMask, 0, 0u, 0.0f}, {pm::kPhaseMask, 1, 0u, 0.0f}});
   return t;
 }
 
@@ -655,7 +677,7 @@ pm_status plan_launches(const std::vector<std::pair<pm_problem*, const std::vect
   for (const auto& ph : phases) {
     pm_problem* p = ph.first;
     for (const PhaseRec& r : *ph.second) {
-      const pm::LaunchCfg cfg = r.kind == pm::kPhaseFilter ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
+      const pm::LaunchCfg cfg = pm::plain_phase(r.kind) ? pm::LaunchCfg{0, 0, 0, 0} : cfg_of(p->opt);
       const pm_tuning shape = force ? *force : shape_of(p->engine, r.kind, p->opt, p->N);
       Group* g = nullptr;
       for (Group& gg : groups)
@@ -735,7 +757,7 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, const Ticks& tic
 
 Ticks all_ticks(const std::vector<pm_problem*>& views) {
   Ticks t;
-  for (pm_problem* p : views) t.push_back(view_ticks(p->opt));
+  for (pm_problem* p : views) t.push_back(view_ticks(*p));
   return t;
 }
 
@@ -827,7 +849,7 @@ pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_sche
 namespace pmh {
 
 pm_status check_phase(int32_t phase) {
-  if (phase < PM_PHASE_INIT || phase > PM_PHASE_FILTER) return fail(PM_ERR_INVALID_ARG, "unknown phase");
+  if (phase < PM_PHASE_INIT || phase > PM_PHASE_MASK) return fail(PM_ERR_INVALID_ARG, "unknown phase");
   return PM_OK;
 }
 
@@ -1194,6 +1216,11 @@ PM_API pm_status pm_problem_get_view_sel(const pm_problem* p, uint32_t* out) {
   if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
   return copy_out(p, out, p->host.view_sel, (size_t)p->W * p->H * sizeof(uint32_t));
 }
+PM_API pm_status pm_problem_get_curvature_map(const pm_problem* p, uint16_t* out) {
+  if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
+  if (!p->host.curv) return fail(PM_ERR_INVALID_ARG, "no curvature map: needs geom_consistency and est_curvature");
+  return copy_out(p, out, p->host.curv, 2 * (size_t)p->W * p->H * sizeof(uint16_t));
+}
 PM_API pm_status pm_problem_get_stats(const pm_problem* p, pm_run_stats* out) {
   if (!p || !out) return fail(PM_ERR_INVALID_ARG, "null argument");
   *out = p->stats;
@@ -1258,6 +1285,7 @@ PM_API pm_status pm_run_view(pm_engine* e, const pm_options* o, const pm_view_de
   if (st == PM_OK && out->conf) st = pm_problem_get_conf_map(p, out->conf);
   if (st == PM_OK && out->cost) st = pm_problem_get_cost_map(p, out->cost);
   if (st == PM_OK && out->view_sel) st = pm_problem_get_view_sel(p, out->view_sel);
+  if (st == PM_OK && out->curvature && p->host.curv) st = pm_problem_get_curvature_map(p, out->curvature);
   pm_problem_destroy(p);
   return st;
 }

@@ -163,7 +163,7 @@ struct PhaseRec {
 using Ticks = std::vector<std::vector<std::vector<PhaseRec>>>;  // [view][tick][phase]
 
 float th_mc_of(const pm_options& o, int iter);
-std::vector<std::vector<PhaseRec>> view_ticks(const pm_options& o);
+std::vector<std::vector<PhaseRec>> view_ticks(const pm_problem& p);
 Ticks all_ticks(const std::vector<pm_problem*>& views);
 
 pm::LaunchCfg cfg_of(const pm_options& o);

@@ -24,6 +24,8 @@
 //   k_sweep         <- Black/RedPixelUpdateOpt -> SweepFromCheckerBoardOpt (:2136-2560, :2634-2734)
 //   k_local_opt     <- Black/RedPixelLocalOptimization -> LocalOptimization (:2736-3017)
 //   k_filter        <- FilterDepthMap (:3282-3354)
+//   k_curvature     <- ComputeCurvatureMap (:1603-1651)
+//   k_mask          <- MaskDepthMapKernel (:3080-3093)
 //   ncc_eval        <- ComputeNCCCost<R,S> (:1245-1340) with ComposeHomography (:431-503)
 //   geom_eval       <- ComputeGeomConsistencyCostShared (:505-568)
 #include <hip/hip_fp16.h>
@@ -1260,6 +1262,121 @@ __global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __r
   for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) filter_tile(resolve(lt, t));
 }
 
+// Pixel of a colour-pass tile (the mapping of filter_tile); false outside the view.
+__device__ __forceinline__ bool tile_pixel(const Ctx& cx, int& row, int& col) {
+  const int W = cx.v->W;
+  const int lw = cx.lw;
+  const int t = threadIdx.x;
+  const int tiles_x = (W + (1 << lw) - 1) >> lw;
+  const int ty = cx.local_block / tiles_x;
+  const int tx = cx.local_block - ty * tiles_x;
+  row = ty * (kTilePixels >> lw) + (t >> (lw - 1));
+  col = (tx << lw) + 2 * (t & ((1 << (lw - 1)) - 1)) + ((row + cx.color) & 1);
+  return row < cx.v->H && col < W;
+}
+
+// float -> unsigned short as the reference's stores convert: toward zero,
+// NaN and negatives to 0, saturating at 65535.
+__device__ __forceinline__ uint16_t sat_u16(float x) {
+  if (!(x > 0.0f)) return 0;
+  if (x >= 65535.0f) return 65535;
+  return (uint16_t)(uint32_t)x;
+}
+
+// acos as the reference calls it: NaN outside [-1, 1] (a dot product of two
+// unit normals can round above 1).
+__device__ __forceinline__ float acos_or_nan(float x) { return fabsf(x) <= 1.0f ? pm_acosf(x) : __builtin_nanf(""); }
+
+// ComputeCurvatureMap (kernel_functions.cu:1603-1651).  Slice 0: the Scharr
+// gradient magnitude of the reference image (intensities / 255, zero
+// outside the image, filter_kernel_x/y at :56-60) x 10000, capped at 32767.
+// Slice 1: 10000 x the median of the angles between the pixel's normal and
+// the normals of its 5x5 neighbourhood clipped to the image, taken as the
+// reference does (angles in window order, bubble sort, element count / 2),
+// so NaN angles keep their positions as they do there.  The compacted angle
+// list and the sort are unrolled over the 24 slots with selects, so they
+// stay in registers.
+__device__ __forceinline__ void curvature_tile(const Ctx& cx) {
+  int row, col;
+  if (!tile_pixel(cx, row, col)) return;
+  const VPtr v = cx.v;
+  const int W = v->W, H = v->H;
+  const int plane = W * H;
+  const int p = row * W + col;
+  constexpr float kx[3][3] = {{-3.0f, 0.0f, 3.0f}, {-10.0f, 0.0f, 10.0f}, {-3.0f, 0.0f, 3.0f}};
+  constexpr float ky[3][3] = {{-3.0f, -10.0f, -3.0f}, {0.0f, 0.0f, 0.0f}, {3.0f, 10.0f, 3.0f}};
+  float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+  for (int dr = -1; dr <= 1; ++dr) {
+#pragma unroll
+    for (int dc = -1; dc <= 1; ++dc) {
+      const int u = col + dc, w = row + dr;
+      const float color = (u >= 0 && u < W && w >= 0 && w < H) ? (float)G(v->ref)[w * W + u] / 255.0f : 0.0f;
+      gx = gx + color * kx[dr + 1][dc + 1];
+      gy = gy + color * ky[dr + 1][dc + 1];
+    }
+  }
+  const float grad = sqrtf(gx * gx + gy * gy);
+
+  const float n0 = G(v->normal)[p], n1 = G(v->normal)[plane + p], n2 = G(v->normal)[2 * plane + p];
+  float a[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) a[k] = 0.0f;
+  int cnt = 0;
+#pragma unroll
+  for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dy == 0 && dx == 0) continue;
+      const int y = row + dy, x = col + dx;
+      if (y < 0 || y >= H || x < 0 || x >= W) continue;
+      const int q = y * W + x;
+      const float d = n0 * G(v->normal)[q] + n1 * G(v->normal)[plane + q] + n2 * G(v->normal)[2 * plane + q];
+      const float ang = acos_or_nan(d);
+#pragma unroll
+      for (int j = 0; j < 24; ++j) a[j] = j == cnt ? ang : a[j];
+      ++cnt;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 23; ++i) {
+#pragma unroll
+    for (int j = 0; j < 23 - i; ++j) {
+      if (j < cnt - 1 - i && a[j] > a[j + 1]) {
+        const float t = a[j];
+        a[j] = a[j + 1];
+        a[j + 1] = t;
+      }
+    }
+  }
+  float med = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 24; ++j) med = j == (cnt >> 1) ? a[j] : med;
+  G(v->curv)[p] = sat_u16(fminf(grad * 10000.0f, 32767.0f));
+  G(v->curv)[plane + p] = cnt > 0 ? sat_u16(med * 10000.0f) : (uint16_t)31415;  // M_PI * 10000
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_curvature(const LaunchTable* __restrict__ lt) {
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) curvature_tile(resolve(lt, t));
+}
+
+// MaskDepthMapKernel (kernel_functions.cu:3080-3093): zero the depth where
+// the mask is 0.  The reference leaves the last row and column as they are.
+__device__ __forceinline__ void mask_tile(const Ctx& cx) {
+  int row, col;
+  if (!tile_pixel(cx, row, col)) return;
+  const VPtr v = cx.v;
+  if (row >= v->H - 1 || col >= v->W - 1) return;
+  const int p = row * v->W + col;
+  if (G(v->mask)[p] == 0) G(v->depth)[p] = 0.0f;
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_mask(const LaunchTable* __restrict__ lt) {
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) mask_tile(resolve(lt, t));
+}
+
 // RunFilter (patch_match_cuda.cu:1074-1108): FilterDepthMapsKernel
 // (kernel_functions.cu:3356-3450) or, with dedup, DeduplicDepthMapsKernel
 // (:3452-3490).  The geometric-consistency cost is the sweep's geometric
@@ -1492,6 +1609,8 @@ void* phase_kernel(PhaseKind kind, const LaunchCfg& cfg) {
     case kPhaseSweep: return pick_kernel<SweepK>(cfg);
     case kPhaseLocal: return pick_kernel<LocalK>(cfg);
     case kPhaseFilter: return (void*)&k_filter;
+    case kPhaseCurvature: return (void*)&k_curvature;
+    case kPhaseMask: return (void*)&k_mask;
   }
   return nullptr;
 }

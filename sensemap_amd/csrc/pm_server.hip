@@ -285,7 +285,7 @@ pm_status upload_job(pm_server* sv, Job* j) {
                           hipMemcpyDeviceToDevice, sv->upload));
     PM_HIP(hipStreamSynchronize(sv->upload));
   }
-  j->ticks = view_ticks(j->opt);
+  j->ticks = view_ticks(*j->prob);
   PM_HIP(hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming));
   return PM_OK;
 }
@@ -463,6 +463,8 @@ pm_status copy_outputs(pm_server* sv, Job* j) {
     PM_HIP(hipMemcpyAsync(j->out.cost, p->host.cost, (size_t)p->N * plane * sizeof(float), hipMemcpyDeviceToHost, s));
   if (j->out.view_sel)
     PM_HIP(hipMemcpyAsync(j->out.view_sel, p->host.view_sel, plane * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (j->out.curvature && p->host.curv)
+    PM_HIP(hipMemcpyAsync(j->out.curvature, p->host.curv, 2 * plane * sizeof(uint16_t), hipMemcpyDeviceToHost, s));
   PM_HIP(hipStreamSynchronize(s));
   return PM_OK;
 }

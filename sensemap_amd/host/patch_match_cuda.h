@@ -50,6 +50,7 @@ struct PatchMatchOptions {
   bool geom_consistency = true;
   double geom_consistency_regularizer = 0.3f;
   double geom_consistency_max_cost = 3.0f;
+  bool est_curvature = false;
   bool random_optimization = true;
   bool local_optimization = true;
   double depth_min = -1.0;
@@ -93,6 +94,7 @@ struct PatchMatchOptions {
     o.geom_consistency = geom_consistency;
     o.geom_consistency_regularizer = (float)geom_consistency_regularizer;
     o.geom_consistency_max_cost = (float)geom_consistency_max_cost;
+    o.est_curvature = est_curvature;
     o.random_optimization = random_optimization;
     o.local_optimization = local_optimization;
     o.random_depth_ratio = random_depth_ratio;
@@ -224,6 +226,23 @@ class Image {
   std::vector<uint8_t> bitmap_;
 };
 
+// Bitmap as a mask map holds it (util/bitmap.h): one 8-bit channel, row 0
+// at the top; Data() is null for an empty bitmap (no mask).
+class Bitmap {
+ public:
+  Bitmap() = default;
+  Bitmap(size_t width, size_t height, std::vector<uint8_t> data)
+      : width_(width), height_(height), data_(std::move(data)) {}
+  size_t Width() const { return width_; }
+  size_t Height() const { return height_; }
+  const uint8_t* Data() const { return data_.empty() ? nullptr : data_.data(); }
+  std::vector<uint8_t> ConvertToRowMajorArray() const { return data_; }
+
+ private:
+  size_t width_ = 0, height_ = 0;
+  std::vector<uint8_t> data_;
+};
+
 struct Problem {
   int ref_image_idx = -1;
   std::vector<int> src_image_idxs;
@@ -231,6 +250,7 @@ struct Problem {
   std::vector<Image>* images = nullptr;
   std::vector<DepthMap>* depth_maps = nullptr;
   std::vector<NormalMap>* normal_maps = nullptr;
+  std::vector<Bitmap>* mask_maps = nullptr;
 };
 
 class PatchMatchACMMCuda {
@@ -265,6 +285,15 @@ class PatchMatchACMMCuda {
     if (!options.init_depth_random) {
       v.init_depth = problem.depth_maps->at(problem.ref_image_idx).GetPtr();
       v.init_normal = problem.normal_maps->at(problem.ref_image_idx).GetPtr();
+    }
+    if (problem.mask_maps) {
+      const Bitmap& m = problem.mask_maps->at(problem.ref_image_idx);
+      if (m.Data()) {
+        v.ref_mask.data = m.Data();
+        v.ref_mask.width = (int32_t)m.Width();
+        v.ref_mask.height = (int32_t)m.Height();
+        v.ref_mask.stride = (int32_t)m.Width();
+      }
     }
     const pm_options o = options.ToC();
     pm_status st = pm_problem_create(engine_, &o, &v, &problem_);
@@ -308,6 +337,13 @@ class PatchMatchACMMCuda {
   Mat<uint32_t> GetViewSelMap() const {
     Mat<uint32_t> m(width_, height_, 1);
     PmCheck(pm_problem_get_view_sel(problem_, m.GetPtr()));
+    return m;
+  }
+  // GetCurvatureMap (patch_match_cuda.cu:444-446); throws unless the run had
+  // geom_consistency and est_curvature.
+  Mat<unsigned short> GetCurvatureMap() const {
+    Mat<unsigned short> m(width_, height_, 2);
+    PmCheck(pm_problem_get_curvature_map(problem_, m.GetPtr()));
     return m;
   }
   // RunFilter (patch_match_cuda.cu:1074-1108) on the problem's ref depth

@@ -31,6 +31,7 @@ class PatchMatchOptions:
     geom_consistency: bool = True
     geom_consistency_regularizer: float = 0.3
     geom_consistency_max_cost: float = 3.0
+    est_curvature: bool = False
     random_optimization: bool = True
     local_optimization: bool = True
     random_depth_ratio: float = 0.004
@@ -98,6 +99,7 @@ class Problem:
     depth_max: float
     depth_maps: Optional[list] = None    # per image index, float32 (H, W) (geometric pass)
     normal_maps: Optional[list] = None   # per image index, float32 (3, H, W)
+    mask_maps: Optional[list] = None     # per image index, uint8 (H, W) or None (Bitmap masks)
 
 
 def _camera(img: Image) -> abi.pm_camera:
@@ -157,6 +159,11 @@ class ViewDesc:
             self._keep += [dm, nm]
             d.init_depth = dm.ctypes.data_as(C.POINTER(C.c_float))
             d.init_normal = nm.ctypes.data_as(C.POINTER(C.c_float))
+        mask = problem.mask_maps[problem.ref_image_idx] if problem.mask_maps is not None else None
+        if mask is not None:
+            mb = np.ascontiguousarray(mask, dtype=np.uint8)
+            self._keep.append(mb)
+            d.ref_mask = self._image(mb)
         self.desc = d
         self.width = ref.GetWidth()
         self.height = ref.GetHeight()
@@ -250,7 +257,7 @@ class PatchMatchACMMCuda:
 
     def _get(self, fn, shape, dtype):
         out = np.empty(shape, dtype=dtype)
-        ctype = C.c_float if dtype == np.float32 else C.c_uint32
+        ctype = {np.float32: C.c_float, np.uint32: C.c_uint32, np.uint16: C.c_uint16}[dtype]
         abi.check(fn(self.handle, out.ctypes.data_as(C.POINTER(ctype))))
         return out
 
@@ -268,6 +275,11 @@ class PatchMatchACMMCuda:
 
     def GetViewSelMap(self) -> np.ndarray:
         return self._get(self.lib.pm_problem_get_view_sel, (self.H, self.W), np.uint32)
+
+    def GetCurvatureMap(self) -> np.ndarray:
+        """2 x H x W uint16 {gradient, curvature} (patch_match_cuda.cu:444-446);
+        raises unless the run had geom_consistency and est_curvature."""
+        return self._get(self.lib.pm_problem_get_curvature_map, (2, self.H, self.W), np.uint16)
 
     def GetStats(self) -> abi.pm_run_stats:
         s = abi.pm_run_stats()
@@ -334,6 +346,7 @@ class _ProblemView:
     GetConfMap = PatchMatchACMMCuda.GetConfMap
     GetCostMap = PatchMatchACMMCuda.GetCostMap
     GetViewSelMap = PatchMatchACMMCuda.GetViewSelMap
+    GetCurvatureMap = PatchMatchACMMCuda.GetCurvatureMap
 
 
 class PatchMatchBatch:
@@ -396,10 +409,14 @@ class ServerJob:
         self.conf = np.empty((H, W), np.float32)
         self.cost = np.empty((N, H, W), np.float32)
         self.view_sel = np.empty((H, W), np.uint32)
+        self.curvature = (np.empty((2, H, W), np.uint16)
+                          if options.geom_consistency and options.est_curvature else None)
         f = C.POINTER(C.c_float)
         self._out = abi.pm_outputs(self.depth.ctypes.data_as(f), self.normal.ctypes.data_as(f),
                                    self.conf.ctypes.data_as(f), self.cost.ctypes.data_as(f),
                                    self.view_sel.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if self.curvature is not None:
+            self._out.curvature = self.curvature.ctypes.data_as(C.POINTER(C.c_uint16))
         self._copt = options.to_c()
         t = C.c_uint64()
         abi.check(server.lib.pm_server_submit(server.handle, C.byref(self._copt), C.byref(self.desc.desc),
@@ -423,8 +440,11 @@ class ServerJob:
 
     def maps(self) -> dict:
         self.wait()
-        return {"depth": self.depth, "normal": self.normal, "conf": self.conf, "cost": self.cost,
-                "view_sel": self.view_sel}
+        m = {"depth": self.depth, "normal": self.normal, "conf": self.conf, "cost": self.cost,
+             "view_sel": self.view_sel}
+        if self.curvature is not None:
+            m["curvature"] = self.curvature
+        return m
 
 
 class PatchMatchServer:

@@ -3,6 +3,7 @@
 //   <dir>/meta.txt   : W H N depth_min depth_max radius step sigma_s sigma_c iters thk seed
 //                      then per image (ref first): fx cx fy cy R[9] T[3]
 //   <dir>/img<k>.u8  : W*H bytes per image
+//   <dir>/mask.u8    : optional W*H mask of the reference image (Problem::mask_maps)
 // and writes <dir>/depth.f32 and <dir>/normal.f32.
 #include <cstdio>
 #include <fstream>
@@ -54,6 +55,14 @@ int main(int argc, char** argv) {
   problem.ref_image_idx = 0;
   for (int k = 1; k <= N; ++k) problem.src_image_idxs.push_back(k);
   problem.images = &images;
+  std::vector<Bitmap> masks(images.size());
+  std::ifstream mf(dir + "/mask.u8", std::ios::binary);
+  if (mf) {
+    std::vector<uint8_t> m((size_t)W * H);
+    mf.read((char*)m.data(), (std::streamsize)m.size());
+    masks[0] = Bitmap(W, H, std::move(m));
+    problem.mask_maps = &masks;
+  }
   try {
     PatchMatchACMMCuda engine(options, problem);
     engine.Run();

@@ -16,7 +16,8 @@ BIN = os.path.join(ROOT, "tests", "cpp", "bin", "shim_run")
 
 
 @pytest.mark.gpu
-def test_cpp_mirror_matches_python_path(tmp_path):
+@pytest.mark.parametrize("with_mask", [False, True])
+def test_cpp_mirror_matches_python_path(tmp_path, with_mask):
     assert os.path.exists(BIN), "tests/cpp/bin/shim_run is built by __graft_entry__.build()"
     sc = helpers.small_planes()
     opts = helpers.c1_options(num_iterations=3)
@@ -30,14 +31,21 @@ def test_cpp_mirror_matches_python_path(tmp_path):
         lines.append(" ".join(repr(float(np.float32(x))) for x in vals))
         v.image.tofile(str(tmp_path / f"img{k}.u8"))
     (tmp_path / "meta.txt").write_text("\n".join(lines) + "\n")
+    prob = scene.to_problem(sc)
+    if with_mask:
+        mask = np.full((120, 160), 255, np.uint8)
+        mask[20:70, 30:110] = 0
+        mask.tofile(str(tmp_path / "mask.u8"))
+        prob.mask_maps = [mask] + [None] * len(prob.src_image_idxs)
     subprocess.run([BIN, str(tmp_path)], check=True, timeout=120)
     d_cpp = np.fromfile(str(tmp_path / "depth.f32"), np.float32).reshape(120, 160)
     n_cpp = np.fromfile(str(tmp_path / "normal.f32"), np.float32).reshape(3, 120, 160)
-    prob = scene.to_problem(sc)
     e = pmm.PatchMatchACMMCuda(opts, prob)
     e.Run()
     assert np.array_equal(d_cpp.view(np.uint32), e.GetDepthMap().view(np.uint32))
     assert np.array_equal(n_cpp.view(np.uint32), e.GetNormalMap().view(np.uint32))
+    if with_mask:
+        assert np.all(d_cpp[20:70, 30:110] == 0)
 
 
 def test_cpp_mirror_header_compiles(tmp_path):
