@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 2
+#define PM_ABI_VERSION 3
 #define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
 #define PM_MAX_WINDOW_RADIUS 32
 #define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
@@ -79,6 +79,7 @@ typedef struct pm_options {
   int32_t init_depth_random;          /* = !geom_consistency */
   int32_t filter;                     /* 1 */
   int32_t conf_filter;                /* 0: FilterDepthMap runs when filter && conf_filter */
+  int32_t median_filter;              /* 0: ComputeGeomGradientMap + Black/RedPixelFilter after the iterations (:248) */
   float conf_threshold;               /* 0.5 */
   float depth_diff_threshold;         /* 0.02 */
   float filter_geom_consistency_max_cost;  /* 1.0: RunFilter's per-source threshold (patch_match_options.h:139) */
@@ -274,7 +275,9 @@ typedef enum pm_phase {
   PM_PHASE_LOCAL = 2,
   PM_PHASE_FILTER = 3,
   PM_PHASE_CURVATURE = 4,
-  PM_PHASE_MASK = 5
+  PM_PHASE_MASK = 5,
+  PM_PHASE_GEOM_GRADIENT = 6,
+  PM_PHASE_MEDIAN = 7
 } pm_phase;
 
 typedef struct pm_tuning {

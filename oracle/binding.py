@@ -75,6 +75,8 @@ def load():
         lib.pmo_model_depth_ranges.argtypes = [P(abi.pm_model), P(C.c_float)]
         lib.pmo_curvature_map.restype = C.c_int
         lib.pmo_curvature_map.argtypes = [P(C.c_uint8), C.c_int, C.c_int, P(C.c_float), P(C.c_uint16)]
+        lib.pmo_median_filter.restype = C.c_int
+        lib.pmo_median_filter.argtypes = [P(C.c_float), P(C.c_float), C.c_int, C.c_int, C.c_float]
         lib.pmo_mask_depth.restype = C.c_int
         lib.pmo_mask_depth.argtypes = [P(C.c_uint8), C.c_int, C.c_int, P(C.c_float)]
         lib.pmo_geom_cost.restype = C.c_float
@@ -139,6 +141,21 @@ def curvature_map(ref, normal):
                              out.ctypes.data_as(C.POINTER(C.c_uint16))) != 0:
         raise RuntimeError("pmo_curvature_map failed")
     return out
+
+
+def median_filter(depth, normal, diff_thres=0.02):
+    """Run()'s median filter on copies of an H x W depth map and its 3 x H x W
+    normal map; returns (depth, normal)."""
+    lib = load()
+    d = np.array(depth, np.float32, copy=True, order="C")
+    n = np.array(normal, np.float32, copy=True, order="C")
+    H, W = d.shape
+    if n.shape != (3, H, W):
+        raise ValueError("normal must be 3 x H x W")
+    if lib.pmo_median_filter(d.ctypes.data_as(C.POINTER(C.c_float)), n.ctypes.data_as(C.POINTER(C.c_float)), W, H,
+                             float(diff_thres)) != 0:
+        raise RuntimeError("pmo_median_filter failed")
+    return d, n
 
 
 def mask_depth(mask, depth):

@@ -1672,6 +1672,103 @@ int pmo_curvature_map(const uint8_t* ref, int W, int H, const float* normal, uin
   return 0;
 }
 
+/* ComputeGeomGradientMap: kernel_functions.cu:3095-3120.  Pixels with
+ * depth <= 0 are not written there (uninitialised memory); they read 0. */
+static void geom_gradient_map(const float* depth, int W, int H, float* grad) {
+#pragma omp parallel for schedule(static)
+  for (int row = 0; row < H; ++row) {
+    for (int col = 0; col < W; ++col) {
+      const float d = depth[(long)row * W + col];
+      if (d <= 0) {
+        grad[(long)row * W + col] = 0.0f;
+        continue;
+      }
+      float conv_x = 0, conv_y = 0;
+      for (int dr = -1; dr <= 1; ++dr) {
+        for (int dc = -1; dc <= 1; ++dc) {
+          const int nrow = row + dr, ncol = col + dc;
+          if (nrow < 0 || nrow >= H || ncol < 0 || ncol >= W) continue;
+          const float diff_depth = fabsf(depth[(long)nrow * W + ncol] - d) / d;
+          conv_x += diff_depth * kScharrX[dr + 1][dc + 1];
+          conv_y += diff_depth * kScharrY[dr + 1][dc + 1];
+        }
+      }
+      grad[(long)row * W + col] = sqrtf(conv_x * conv_x + conv_y * conv_y);
+    }
+  }
+}
+
+/* MedianFilter: kernel_functions.cu:3122-3244, for one pixel. */
+static void median_pixel(float* depth, float* normal, const float* grad_map, int row, int col, int W, int H,
+                         float diff_thres) {
+  const long plane = (long)W * H;
+  const float grad_thres = 1.4f * diff_thres;
+  float depths[17], nx[17], ny[17], nz[17];
+  int index = 0;
+  const long c = (long)row * W + col;
+  const float grad = grad_map[c];
+  const float cdepth = depth[c];
+  const float inv_cdepth = 1.0f / cdepth;
+  depths[index] = cdepth;
+  nx[index] = normal[c];
+  ny[index] = normal[plane + c];
+  nz[index] = normal[2 * plane + c];
+  index++;
+  /* the cross at distances 1 and 3: up, down, left, right; then the knight
+   * moves (dirs, :3214) */
+  int taps[16][2];
+  int nt = 0;
+  for (int i = 0; i < 2; ++i) {
+    const int k = (i << 1) + 1;
+    taps[nt][0] = -k, taps[nt][1] = 0, ++nt;
+    taps[nt][0] = k, taps[nt][1] = 0, ++nt;
+    taps[nt][0] = 0, taps[nt][1] = -k, ++nt;
+    taps[nt][0] = 0, taps[nt][1] = k, ++nt;
+  }
+  static const int dirs[8][2] = {{-2, 1}, {-1, 2}, {1, 2}, {2, 1}, {2, -1}, {1, -2}, {-1, -2}, {-2, -1}};
+  for (int i = 0; i < 8; ++i) taps[nt][0] = dirs[i][0], taps[nt][1] = dirs[i][1], ++nt;
+  for (int t = 0; t < nt; ++t) {
+    const int nrow = row + taps[t][0], ncol = col + taps[t][1];
+    if (nrow < 0 || nrow >= H || ncol < 0 || ncol >= W) continue;
+    const long q = (long)nrow * W + ncol;
+    depths[index] = depth[q];
+    const float diff_depth = fabsf(depths[index] - cdepth) * inv_cdepth;
+    if (grad < grad_thres || (grad >= grad_thres && diff_depth < diff_thres)) {
+      nx[index] = normal[q];
+      ny[index] = normal[plane + q];
+      nz[index] = normal[2 * plane + q];
+      index++;
+    }
+  }
+  bubble_sort(depths, index);
+  bubble_sort(nx, index);
+  bubble_sort(ny, index);
+  bubble_sort(nz, index);
+  const int nth = index >> 1;
+  const float inv_norm = 1.0f / sqrtf(nx[nth] * nx[nth] + ny[nth] * ny[nth] + nz[nth] * nz[nth]);
+  depth[c] = depths[nth];
+  normal[c] = nx[nth] * inv_norm;
+  normal[plane + c] = ny[nth] * inv_norm;
+  normal[2 * plane + c] = nz[nth] * inv_norm;
+}
+
+/* Run()'s median filter (patch_match_cuda.cu:991-1008): the gradient map,
+ * then BlackPixelFilter ((row + col) even) and RedPixelFilter (odd), in
+ * place.  Every tap of a pixel has the other colour. */
+int pmo_median_filter(float* depth, float* normal, int W, int H, float diff_thres) {
+  if (!depth || !normal || W <= 0 || H <= 0) return -1;
+  float* grad = (float*)malloc(sizeof(float) * (size_t)W * H);
+  if (!grad) return -1;
+  geom_gradient_map(depth, W, H, grad);
+  for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for schedule(static)
+    for (int row = 0; row < H; ++row)
+      for (int col = (row + color) & 1; col < W; col += 2) median_pixel(depth, normal, grad, row, col, W, H, diff_thres);
+  }
+  free(grad);
+  return 0;
+}
+
 /* MaskDepthMapKernel: kernel_functions.cu:3080-3093 (the last row and
  * column are left as they are) */
 int pmo_mask_depth(const uint8_t* mask, int W, int H, float* depth) {
@@ -1748,6 +1845,7 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   }
   if (opt->geom_consistency && opt->est_curvature && out->curvature)
     pmo_curvature_map(v.ref, v.W, v.H, st.normal, out->curvature);
+  if (opt->median_filter) pmo_median_filter(st.depth, st.normal, v.W, v.H, opt->depth_diff_threshold);
   if (opt->filter && opt->conf_filter) {
     for (int color = 0; color < 2; ++color) run_color(&x, filter_pixel, color, r0, r1);
   }

@@ -19,6 +19,8 @@
  *   src/mvs/kernel_functions.cu:2136-2560   SweepFromCheckerBoardOpt
  *   src/mvs/kernel_functions.cu:2736-2931   LocalOptimization
  *   src/mvs/kernel_functions.cu:3080-3093   MaskDepthMapKernel
+ *   src/mvs/kernel_functions.cu:3095-3280   ComputeGeomGradientMap, MedianFilter,
+ *                                           Black/RedPixelFilter
  *   src/mvs/kernel_functions.cu:3282-3354   FilterDepthMap
  *   src/mvs/patch_match_cuda.cu:493-580     InitTransforms (pose packing)
  *   src/mvs/patch_match_cuda.cu:582-698     InitWorkspace
@@ -165,6 +167,10 @@ float pmo_geom_cost(const pm_options* opt, const pm_view_desc* view, int row, in
  * planar {gradient, curvature}; ref is the W x H 8-bit reference image and
  * normal the 3 x W x H planar normal map. */
 int pmo_curvature_map(const uint8_t* ref, int W, int H, const float* normal, uint16_t* out);
+/* Run()'s median filter (patch_match_cuda.cu:991-1008: ComputeGeomGradientMap,
+ * BlackPixelFilter, RedPixelFilter; kernel_functions.cu:3095-3280) in place
+ * on a W x H depth map and its 3 x W x H planar normal map. */
+int pmo_median_filter(float* depth, float* normal, int W, int H, float diff_thres);
 /* MaskDepthMapKernel (kernel_functions.cu:3080-3093) on a W x H depth map. */
 int pmo_mask_depth(const uint8_t* mask, int W, int H, float* depth);
 float pmo_expf(float x);

@@ -51,6 +51,7 @@ class pm_options(C.Structure):
         ("init_depth_random", C.c_int32),
         ("filter", C.c_int32),
         ("conf_filter", C.c_int32),
+        ("median_filter", C.c_int32),
         ("conf_threshold", C.c_float),
         ("depth_diff_threshold", C.c_float),
         ("filter_geom_consistency_max_cost", C.c_float),
@@ -126,7 +127,8 @@ class pm_run_stats(C.Structure):
 
 
 # Every symbol the header declares, with its ctypes signature.
-PM_PHASE_INIT, PM_PHASE_SWEEP, PM_PHASE_LOCAL, PM_PHASE_FILTER, PM_PHASE_CURVATURE, PM_PHASE_MASK = 0, 1, 2, 3, 4, 5
+(PM_PHASE_INIT, PM_PHASE_SWEEP, PM_PHASE_LOCAL, PM_PHASE_FILTER, PM_PHASE_CURVATURE, PM_PHASE_MASK,
+ PM_PHASE_GEOM_GRADIENT, PM_PHASE_MEDIAN) = range(8)
 
 
 class pm_tuning(C.Structure):

@@ -65,6 +65,7 @@ struct ViewDev {
   uint32_t* view_sel;       // W*H
   uint16_t* curv;           // 2*W*H planar {gradient, curvature} or null (no curvature pass)
   const uint8_t* mask;      // W*H depth mask or null
+  float* geom_grad;         // W*H relative-depth gradient (median filter) or null
   unsigned long long* counters;  // [ncc_eval, ncc_samples, geo_eval] or null
   SrcDev src[PM_MAX_NUM_SRC];
 };
@@ -142,10 +143,12 @@ enum PhaseKind {
   kPhaseLocal = 2,
   kPhaseFilter = 3,     // FilterDepthMap, one colour per entry
   kPhaseCurvature = 4,  // ComputeCurvatureMap, one colour per entry
-  kPhaseMask = 5        // MaskDepthMapKernel, one colour per entry
+  kPhaseMask = 5,       // MaskDepthMapKernel, one colour per entry
+  kPhaseGeomGrad = 6,   // ComputeGeomGradientMap, one colour per entry
+  kPhaseMedian = 7      // Black/RedPixelFilter, one colour per entry
 };
 // Phases with no window, storage or reciprocal variants.
-inline bool plain_phase(PhaseKind k) { return k == kPhaseFilter || k == kPhaseCurvature || k == kPhaseMask; }
+inline bool plain_phase(PhaseKind k) { return k >= kPhaseFilter; }
 // Persistent grid for a launch of `tiles` tiles on the current device: the
 // number of CUs times the workgroups one CU holds (capped by blocks_per_cu
 // when > 0), and never more than the tiles.

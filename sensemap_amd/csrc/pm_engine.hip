@@ -123,6 +123,7 @@ PM_API pm_status pm_options_init(pm_options* o) {
   o->filter = 1;
   o->conf_filter = 0;
   o->est_curvature = 0;
+  o->median_filter = 0;
   o->conf_threshold = 0.5f;
   o->depth_diff_threshold = 0.02f;
   o->filter_geom_consistency_max_cost = 1.0f;
@@ -372,6 +373,7 @@ size_t job_block_bytes(const pm_options* o, const pm_view_desc* v, const bool* s
   add(6 * sizeof(unsigned long long));                // counters
   if (o->geom_consistency && o->est_curvature) add(2 * plane * sizeof(uint16_t));  // curvature map
   if (v->ref_mask.data) add(plane);                   // mask
+  if (o->median_filter) add(plane * sizeof(float));   // relative-depth gradient
   if (!o->init_depth_random) {
     add(plane * sizeof(float));
     add(3 * plane * sizeof(float));
@@ -516,6 +518,7 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
     if ((st = p->alloc(&h.curv, 2 * plane)) != PM_OK) return st;
     PM_HIP(hipMemsetAsync(h.curv, 0, 2 * plane * sizeof(uint16_t), s));
   }
+  if (o->median_filter && (st = p->alloc(&h.geom_grad, plane)) != PM_OK) return st;
   if (v->ref_mask.data) {
     uint8_t* m = nullptr;
     if ((st = p->alloc(&m, plane)) != PM_OK) return st;
@@ -622,10 +625,16 @@ std::vector<std::vector<PhaseRec>> view_ticks(const pm_problem& p) {
     if (o.local_optimization && iter > 2)
       for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseLocal, c, PM_PASS_LOCAL(iter, c) + goff, 0.0f}});
   }
-  // after PropagationAndOptim: curvature (:985-989), FilterDepthMap (:1015-1037),
-  // mask (:1040-1052); the curvature and mask passes touch only their own
-  // pixel's outputs, so both colours share a tick
+  // after PropagationAndOptim: curvature (:985-989), median filter
+  // (:991-1008), FilterDepthMap (:1015-1037), mask (:1040-1052); the
+  // curvature, gradient and mask passes write only their own pixel's outputs
+  // and read nothing another entry of the tick writes, so both colours share
+  // a tick
   if (p.host.curv) t.push_back({{pm::kPhaseCurvature, 0, 0u, 0.0f}, {pm::kPhaseCurvature, 1, 0u, 0.0f}});
+  if (p.host.geom_grad) {  // median filter (:991-1008): gradient of all pixels, then black, then red
+    t.push_back({{pm::kPhaseGeomGrad, 0, 0u, 0.0f}, {pm::kPhaseGeomGrad, 1, 0u, 0.0f}});
+    for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseMedian, c, 0u, 0.0f}});
+  }
   if (o.filter && o.conf_filter)
     for (int c = 0; c < 2; ++c) t.push_back({{pm::kPhaseFilter, c, 0u, 0.0f}});
   if (p.host.mask) t.push_back({{pm::kPhaseMask, 0, 0u, 0.0f}, {pm::kPhaseMask, 1, 0u, 0.0f}});
@@ -849,7 +858,7 @@ pm_status run_views(pm_engine* e, const std::vector<pm_problem*>& views, pm_sche
 namespace pmh {
 
 pm_status check_phase(int32_t phase) {
-  if (phase < PM_PHASE_INIT || phase > PM_PHASE_MASK) return fail(PM_ERR_INVALID_ARG, "unknown phase");
+  if (phase < PM_PHASE_INIT || phase > PM_PHASE_MEDIAN) return fail(PM_ERR_INVALID_ARG, "unknown phase");
   return PM_OK;
 }
 

@@ -26,6 +26,8 @@
 //   k_filter        <- FilterDepthMap (:3282-3354)
 //   k_curvature     <- ComputeCurvatureMap (:1603-1651)
 //   k_mask          <- MaskDepthMapKernel (:3080-3093)
+//   k_geom_grad     <- ComputeGeomGradientMap (:3095-3120)
+//   k_median        <- BlackPixelFilter / RedPixelFilter -> MedianFilter (:3122-3280)
 //   ncc_eval        <- ComputeNCCCost<R,S> (:1245-1340) with ComposeHomography (:431-503)
 //   geom_eval       <- ComputeGeomConsistencyCostShared (:505-568)
 #include <hip/hip_fp16.h>
@@ -1275,6 +1277,33 @@ __device__ __forceinline__ bool tile_pixel(const Ctx& cx, int& row, int& col) {
   return row < cx.v->H && col < W;
 }
 
+// Ascending bubble sort of the first n of K values (the reference's
+// BubbleSort, utils.h:206-221), unrolled over K slots so the array stays in
+// registers; the compare-swaps past n are switched off, which leaves the
+// same sequence of swaps as the loop over n.
+template <int K>
+__device__ __forceinline__ void bubble_sort_regs(float (&a)[K], int n) {
+#pragma unroll
+  for (int i = 0; i < K - 1; ++i) {
+#pragma unroll
+    for (int j = 0; j < K - 1 - i; ++j) {
+      if (j < n - 1 - i && a[j] > a[j + 1]) {
+        const float t = a[j];
+        a[j] = a[j + 1];
+        a[j + 1] = t;
+      }
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ float pick_reg(const float (&a)[K], int k) {
+  float r = 0.0f;
+#pragma unroll
+  for (int j = 0; j < K; ++j) r = j == k ? a[j] : r;
+  return r;
+}
+
 // float -> unsigned short as the reference's stores convert: toward zero,
 // NaN and negatives to 0, saturating at 65535.
 __device__ __forceinline__ uint16_t sat_u16(float x) {
@@ -1338,20 +1367,8 @@ __device__ __forceinline__ void curvature_tile(const Ctx& cx) {
       ++cnt;
     }
   }
-#pragma unroll
-  for (int i = 0; i < 23; ++i) {
-#pragma unroll
-    for (int j = 0; j < 23 - i; ++j) {
-      if (j < cnt - 1 - i && a[j] > a[j + 1]) {
-        const float t = a[j];
-        a[j] = a[j + 1];
-        a[j + 1] = t;
-      }
-    }
-  }
-  float med = 0.0f;
-#pragma unroll
-  for (int j = 0; j < 24; ++j) med = j == (cnt >> 1) ? a[j] : med;
+  bubble_sort_regs(a, cnt);
+  const float med = pick_reg(a, cnt >> 1);
   G(v->curv)[p] = sat_u16(fminf(grad * 10000.0f, 32767.0f));
   G(v->curv)[plane + p] = cnt > 0 ? sat_u16(med * 10000.0f) : (uint16_t)31415;  // M_PI * 10000
 }
@@ -1375,6 +1392,113 @@ __device__ __forceinline__ void mask_tile(const Ctx& cx) {
 __global__ __launch_bounds__(kSweepThreads) void k_mask(const LaunchTable* __restrict__ lt) {
   __shared__ int next;
   for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) mask_tile(resolve(lt, t));
+}
+
+// ComputeGeomGradientMap (kernel_functions.cu:3095-3120): Scharr gradient of
+// |d_q - d_p| / d_p over the in-image 3x3 neighbours (outside ones are
+// skipped, not zero).  The reference leaves pixels with d_p <= 0 unwritten
+// (uninitialised memory); here they read 0.
+__device__ __forceinline__ void geom_grad_tile(const Ctx& cx) {
+  int row, col;
+  if (!tile_pixel(cx, row, col)) return;
+  const VPtr v = cx.v;
+  const int W = v->W, H = v->H;
+  const int p = row * W + col;
+  const float depth = G(v->depth)[p];
+  if (depth <= 0.0f) {
+    G(v->geom_grad)[p] = 0.0f;
+    return;
+  }
+  constexpr float kx[3][3] = {{-3.0f, 0.0f, 3.0f}, {-10.0f, 0.0f, 10.0f}, {-3.0f, 0.0f, 3.0f}};
+  constexpr float ky[3][3] = {{-3.0f, -10.0f, -3.0f}, {0.0f, 0.0f, 0.0f}, {3.0f, 10.0f, 3.0f}};
+  float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+  for (int dr = -1; dr <= 1; ++dr) {
+#pragma unroll
+    for (int dc = -1; dc <= 1; ++dc) {
+      const int w = row + dr, u = col + dc;
+      if (w < 0 || w >= H || u < 0 || u >= W) continue;
+      const float diff = fabsf(G(v->depth)[w * W + u] - depth) / depth;
+      gx = gx + diff * kx[dr + 1][dc + 1];
+      gy = gy + diff * ky[dr + 1][dc + 1];
+    }
+  }
+  G(v->geom_grad)[p] = sqrtf(gx * gx + gy * gy);
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_geom_grad(const LaunchTable* __restrict__ lt) {
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) geom_grad_tile(resolve(lt, t));
+}
+
+// MedianFilter (kernel_functions.cu:3122-3244) for the pixels of one colour
+// (BlackPixelFilter: (row + col) even, then RedPixelFilter: odd).  Taps:
+// the pixel, the cross at distances 1 and 3 (up, down, left, right), then
+// the 8 knight moves, in the reference's order; a neighbour counts when the
+// pixel's gradient is below 1.4 x depth_diff_threshold or its relative depth
+// difference is below depth_diff_threshold.  Depth and each normal
+// component take their own median (element count / 2 after the sort); the
+// normal is then normalised.  Every tap of a colour-c pixel is of colour
+// 1 - c, so a pass reads nothing it writes.
+__device__ __forceinline__ void median_tile(const Ctx& cx) {
+  int row, col;
+  if (!tile_pixel(cx, row, col)) return;
+  const VPtr v = cx.v;
+  const int W = v->W, H = v->H;
+  const int plane = W * H;
+  const int p = row * W + col;
+  const float thr = v->depth_diff_threshold;
+  const float gthr = 1.4f * thr;
+  const float grad = G(v->geom_grad)[p];
+  const float cd = G(v->depth)[p];
+  const float inv = 1.0f / cd;
+  constexpr int kTaps = 17;
+  constexpr int offs[kTaps - 1][2] = {{-1, 0}, {1, 0},  {0, -1}, {0, 1},  {-3, 0}, {3, 0},   {0, -3},  {0, 3},
+                                      {-2, 1}, {-1, 2}, {1, 2},  {2, 1},  {2, -1}, {1, -2}, {-1, -2}, {-2, -1}};
+  float dd[kTaps], nx[kTaps], ny[kTaps], nz[kTaps];
+#pragma unroll
+  for (int k = 0; k < kTaps; ++k) dd[k] = nx[k] = ny[k] = nz[k] = 0.0f;
+  dd[0] = cd;
+  nx[0] = G(v->normal)[p];
+  ny[0] = G(v->normal)[plane + p];
+  nz[0] = G(v->normal)[2 * plane + p];
+  int cnt = 1;
+#pragma unroll
+  for (int k = 0; k < kTaps - 1; ++k) {
+    const int w = row + offs[k][0], u = col + offs[k][1];
+    if (w < 0 || w >= H || u < 0 || u >= W) continue;
+    const int q = w * W + u;
+    const float d = G(v->depth)[q];
+    const float diff = fabsf(d - cd) * inv;
+    if (grad < gthr || (grad >= gthr && diff < thr)) {
+      const float a = G(v->normal)[q], b = G(v->normal)[plane + q], c = G(v->normal)[2 * plane + q];
+#pragma unroll
+      for (int j = 1; j < kTaps; ++j) {
+        const bool at = j == cnt;
+        dd[j] = at ? d : dd[j];
+        nx[j] = at ? a : nx[j];
+        ny[j] = at ? b : ny[j];
+        nz[j] = at ? c : nz[j];
+      }
+      ++cnt;
+    }
+  }
+  bubble_sort_regs(dd, cnt);
+  bubble_sort_regs(nx, cnt);
+  bubble_sort_regs(ny, cnt);
+  bubble_sort_regs(nz, cnt);
+  const int nth = cnt >> 1;
+  const float mx = pick_reg(nx, nth), my = pick_reg(ny, nth), mz = pick_reg(nz, nth);
+  const float inv_norm = 1.0f / sqrtf(mx * mx + my * my + mz * mz);
+  G(v->depth)[p] = pick_reg(dd, nth);
+  G(v->normal)[p] = mx * inv_norm;
+  G(v->normal)[plane + p] = my * inv_norm;
+  G(v->normal)[2 * plane + p] = mz * inv_norm;
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_median(const LaunchTable* __restrict__ lt) {
+  __shared__ int next;
+  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) median_tile(resolve(lt, t));
 }
 
 // RunFilter (patch_match_cuda.cu:1074-1108): FilterDepthMapsKernel
@@ -1611,6 +1735,8 @@ void* phase_kernel(PhaseKind kind, const LaunchCfg& cfg) {
     case kPhaseFilter: return (void*)&k_filter;
     case kPhaseCurvature: return (void*)&k_curvature;
     case kPhaseMask: return (void*)&k_mask;
+    case kPhaseGeomGrad: return (void*)&k_geom_grad;
+    case kPhaseMedian: return (void*)&k_median;
   }
   return nullptr;
 }

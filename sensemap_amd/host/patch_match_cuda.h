@@ -68,6 +68,7 @@ struct PatchMatchOptions {
   bool init_depth_random = !geom_consistency;
   bool filter = true;
   bool conf_filter = false;
+  bool median_filter = false;
   float conf_threshold = 0.5;
   float depth_diff_threshold = 0.02;
   int filter_min_num_consistent = 1;
@@ -110,6 +111,7 @@ struct PatchMatchOptions {
     o.init_depth_random = init_depth_random;
     o.filter = filter;
     o.conf_filter = conf_filter;
+    o.median_filter = median_filter;
     o.conf_threshold = conf_threshold;
     o.depth_diff_threshold = depth_diff_threshold;
     o.filter_geom_consistency_max_cost = (float)filter_geom_consistency_max_cost;

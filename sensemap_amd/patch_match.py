@@ -47,6 +47,7 @@ class PatchMatchOptions:
     init_depth_random: Optional[bool] = None  # None = !geom_consistency, as the reference
     filter: bool = True
     conf_filter: bool = False
+    median_filter: bool = False
     conf_threshold: float = 0.5
     depth_diff_threshold: float = 0.02
     filter_geom_consistency_max_cost: float = 1.0

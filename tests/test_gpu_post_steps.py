@@ -1,7 +1,8 @@
-"""ComputeCurvatureMap (GetCurvatureMap) and MaskDepthMapKernel on the device,
-called through the C-ABI, against the CPU transcription: bitwise on the
-engine's own maps, on a given normal field whose float dot products round
-above 1 (NaN angles), in the batched form and in the serving loop."""
+"""The steps of Run() after the iterations on the device -- ComputeCurvatureMap
+(GetCurvatureMap), the median filter and MaskDepthMapKernel -- called through
+the C-ABI, against the CPU transcription: bitwise on the engine's own maps,
+on a given normal field whose float dot products round above 1 (NaN angles),
+in the batched form and in the serving loop."""
 import numpy as np
 import pytest
 
@@ -112,6 +113,28 @@ def test_mask_map_bitwise():
     assert np.all(gpu["depth"][:-1, :-1][mask[:-1, :-1] == 0] == 0)
 
 
+def test_median_filter_bitwise():
+    sc = helpers.small_planes(seed=8)
+    prob = scene.to_problem(sc)
+    opts = helpers.c1_options(num_iterations=3, median_filter=True)
+    print("[median] 160x120 N=4 it=3", flush=True)
+    e = pmm.PatchMatchACMMCuda(opts, prob)
+    e.Run()
+    gpu = maps_of(e)
+    cpu = binding.run_view(opts, e.view)
+    e.close()
+    diff = helpers.maps_equal(gpu, cpu)
+    print("[median] differing elements", diff, flush=True)
+    assert all(v == 0 for v in diff.values()), diff
+    plain_opts = helpers.c1_options(num_iterations=3)
+    e2 = pmm.PatchMatchACMMCuda(plain_opts, prob)
+    e2.Run()
+    d, n = binding.median_filter(e2.GetDepthMap(), e2.GetNormalMap(), opts.depth_diff_threshold)
+    e2.close()
+    assert np.array_equal(gpu["depth"].view(np.uint32), d.view(np.uint32))
+    assert np.array_equal(gpu["normal"].view(np.uint32), n.view(np.uint32))
+
+
 def test_curvature_getter_needs_the_option():
     sc = helpers.small_planes()
     e = pmm.PatchMatchACMMCuda(helpers.c1_options(num_iterations=0), scene.to_problem(sc))
@@ -123,7 +146,7 @@ def test_curvature_getter_needs_the_option():
 
 def test_batch_and_server_carry_curvature_and_mask():
     _, gprob = geom_problem()
-    gopts = helpers.c1_options(geom_consistency=True, num_iterations=3, est_curvature=True)
+    gopts = helpers.c1_options(geom_consistency=True, num_iterations=3, est_curvature=True, median_filter=True)
     # one scene for both jobs: the serving loop caches source images by their
     # index, and images under one key must be identical
     _, mprob, _ = masked_problem(seed=4)
