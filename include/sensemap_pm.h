@@ -87,7 +87,8 @@ typedef struct pm_options {
   /* ---- extensions (no counterpart in the reference) ---- */
   int32_t init_samples;               /* 1 = reference semantics; K > 1 keeps the best of K random inits */
   int32_t compat_lo_geo_bug;          /* 1 = reproduce kernel_functions.cu:2924 (ref_invK passed as K) */
-  int32_t image_storage;              /* PM_STORAGE_U8 or PM_STORAGE_F16 (source images in HBM) */
+  int32_t image_storage;              /* source images in HBM: PM_STORAGE_F16 (default; the 8-bit values exactly,
+                                         fewer instructions per sample) or PM_STORAGE_U8 (half the bytes); same maps */
   int32_t accum_mode;                 /* PM_ACCUM_PLAIN or PM_ACCUM_COMPENSATED (NCC moments) */
   int32_t rcp_mode;                   /* PM_RCP_EXACT (bitwise with the CPU transcription) or PM_RCP_FAST */
   int32_t reserved0;

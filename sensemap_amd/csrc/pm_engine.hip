@@ -90,6 +90,13 @@ size_t Arena::used() const {
   return used_;
 }
 
+size_t Arena::largest_hole() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  size_t m = 0;
+  for (const auto& h : holes_) m = std::max(m, h.second);
+  return m;
+}
+
 }  // namespace pmh
 
 // ---------------------------------------------------------------------------
@@ -130,7 +137,7 @@ PM_API pm_status pm_options_init(pm_options* o) {
   o->filter_min_num_consistent = 1;
   o->init_samples = 1;
   o->compat_lo_geo_bug = 1;
-  o->image_storage = PM_STORAGE_U8;
+  o->image_storage = PM_STORAGE_F16;  // the 8-bit values exactly, fewer instructions per sample
   o->accum_mode = PM_ACCUM_PLAIN;
   o->rcp_mode = PM_RCP_EXACT;
   o->seed = 0;

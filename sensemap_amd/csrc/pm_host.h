@@ -64,6 +64,7 @@ class Arena {
   uint8_t* base() const { return base_; }
   size_t capacity() const { return cap_; }
   size_t used() const;
+  size_t largest_hole() const;
   static size_t round(size_t bytes) { return (bytes + kAlign - 1) & ~(kAlign - 1); }
 
  private:

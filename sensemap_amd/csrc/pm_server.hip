@@ -89,6 +89,7 @@ struct pm_server {
   std::deque<Job*> pending;
   std::vector<Job*> active;
   std::deque<Job*> completing;
+  int copying = 0;  // jobs the completion thread took from `completing` and has not released yet
   std::map<uint64_t, Job*> jobs;  // submitted and not yet waited for
   std::map<CacheKey, CacheEntry> cache;
   uint64_t next_ticket = 1, use_clock = 0;
@@ -231,11 +232,10 @@ void release(pm_server* sv, Job* j, bool drop_fresh) {
 
 // Device bytes a job needs when none of its sources is cached: more than
 // the arena means it can never run.
-size_t worst_case_bytes(const pm_options* o, const pm_view_desc* v) {
-  size_t b = job_block_bytes(o, v, nullptr);
-  for (int i = 0; i < v->num_src; ++i) b += Arena::round(src_quad_bytes(v->src[i], o->image_storage));
-  return b;
-}
+// Arena bytes a job needs with nothing cached: its block with every source
+// in it (a source that goes to the cache instead takes the same rounded
+// bytes as a cache entry).
+size_t worst_case_bytes(const pm_options* o, const pm_view_desc* v) { return job_block_bytes(o, v, nullptr); }
 
 // ---- loop thread ---------------------------------------------------------------
 
@@ -387,11 +387,31 @@ void loop_main(pm_server* sv) {
             break;
           }
           if (sv->active.empty()) {
-            if (sv->completing.empty()) {
-              // everything is free and it still does not fit (cannot happen
-              // for jobs that passed the submit check); never wait forever
+            if (sv->completing.empty() && sv->copying == 0) {
+              // nothing runs, yet the job does not fit: the cached sources
+              // it would reuse, pinned where they lie, can split the free
+              // space into holes smaller than its block.  Drop every cached
+              // source (none is held now) so the arena is one hole; laid out
+              // from its start, the job's entries and block take exactly
+              // the bytes the submit check allowed.
+              while (evict_one(sv)) {
+              }
+              if (reserve(sv, j)) {
+                sv->pending.pop_front();
+                admit = j;
+                break;
+              }
+              // still no room (cannot happen for a job that passed the
+              // submit check): fail it rather than wait forever
               sv->pending.pop_front();
-              fail_job(sv, j, PM_ERR_OVER_BUDGET, "job does not fit in the budget");
+              int held = 0;
+              for (const auto& c : sv->cache) held += c.second.refs != 0;
+              fail_job(sv, j, PM_ERR_OVER_BUDGET,
+                       "job does not fit in the budget: needs " + std::to_string(worst_case_bytes(&j->opt, &j->view)) +
+                           " bytes; arena " + std::to_string(sv->arena.used()) + " of " +
+                           std::to_string(sv->arena.capacity()) + " used, largest hole " +
+                           std::to_string(sv->arena.largest_hole()) + ", " + std::to_string(sv->cache.size()) +
+                           " cached sources (" + std::to_string(held) + " held)");
               sv->cv.notify_all();
               continue;
             }
@@ -479,6 +499,7 @@ void completer_main(pm_server* sv) {
       if (sv->completing.empty()) break;
       j = sv->completing.front();
       sv->completing.pop_front();
+      sv->copying++;  // still holds its memory until released below
     }
     const pm_status rs = copy_outputs(sv, j);
     const std::string msg = rs == PM_OK ? std::string() : last_error();
@@ -494,6 +515,7 @@ void completer_main(pm_server* sv) {
       j->t_done = Clock::now();
       sv->st.completed++;
     }
+    sv->copying--;
     sv->cv.notify_all();
   }
 }

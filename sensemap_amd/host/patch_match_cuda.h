@@ -78,7 +78,7 @@ struct PatchMatchOptions {
   // extensions of this engine
   int init_samples = 1;
   bool compat_lo_geo_bug = true;
-  int image_storage = PM_STORAGE_U8;
+  int image_storage = PM_STORAGE_F16;
   int accum_mode = PM_ACCUM_PLAIN;
   int rcp_mode = PM_RCP_EXACT;
   uint64_t seed = 0;

@@ -55,7 +55,7 @@ class PatchMatchOptions:
     # extensions
     init_samples: int = 1
     compat_lo_geo_bug: bool = True
-    image_storage: int = abi.PM_STORAGE_U8
+    image_storage: int = abi.PM_STORAGE_F16
     accum_mode: int = abi.PM_ACCUM_PLAIN
     rcp_mode: int = abi.PM_RCP_EXACT
     seed: int = 0
@@ -423,6 +423,9 @@ class ServerJob:
         abi.check(server.lib.pm_server_submit(server.handle, C.byref(self._copt), C.byref(self.desc.desc),
                                               C.byref(self._out), C.byref(t)))
         self.ticket = t.value
+        # the server borrows this job's buffers until pm_server_wait returns:
+        # it holds the job until then, whatever the caller keeps
+        server._inflight[self.ticket] = self
         self.stats: Optional[abi.pm_job_stats] = None
 
     def done(self) -> bool:
@@ -436,6 +439,7 @@ class ServerJob:
             st = abi.pm_job_stats()
             rc = self.server.lib.pm_server_wait(self.server.handle, self.ticket, C.byref(st))
             self.stats = st
+            self.server._inflight.pop(self.ticket, None)
             abi.check(rc)
         return self.stats
 
@@ -457,6 +461,7 @@ class PatchMatchServer:
                  max_active_views: int = 64):
         self.engine = engine or default_engine()
         self.lib = self.engine.lib
+        self._inflight = {}  # ticket -> ServerJob not yet waited for
         cfg = abi.pm_server_config(int(hbm_budget_bytes), int(max_active_views), 0)
         h = C.c_void_p()
         abi.check(self.lib.pm_server_create(self.engine.handle, C.byref(cfg), C.byref(h)))
@@ -473,8 +478,9 @@ class PatchMatchServer:
     def close(self) -> None:
         """Finish every submitted job, then free the server."""
         if getattr(self, "handle", None):
-            self.lib.pm_server_destroy(self.handle)
+            self.lib.pm_server_destroy(self.handle)  # finishes every job first
             self.handle = None
+        self._inflight = {}
 
     def __del__(self):
         try:

@@ -107,6 +107,23 @@ def test_small_budget_admits_few_and_evicts():
     eng.close()
 
 
+def test_jobs_dropped_by_the_caller_still_complete():
+    # the server borrows a job's buffers until it is waited for; the Python
+    # server holds the jobs, so dropping every reference to them is safe
+    import gc
+    eng = pmm.Engine(0)
+    items = orbit_jobs()[:4]
+    sv = pmm.PatchMatchServer(eng, hbm_budget_bytes=8 << 30, max_active_views=2)
+    for o, p in items:
+        sv.submit(o, p)
+    gc.collect()
+    for _ in range(20):
+        bytearray(1 << 20)  # churn the host heap while the jobs run
+    sv.close()
+    assert not sv._inflight
+    eng.close()
+
+
 def test_one_slot_runs_jobs_one_by_one():
     eng = pmm.Engine(0)
     items = orbit_jobs()[:4]
