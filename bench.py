@@ -86,7 +86,7 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def c2_options(pmm, geom: bool, rcp: int, storage: int = 0):
+def c2_options(pmm, geom: bool, rcp: int, storage: int = 1):  # 1: PM_STORAGE_F16, the engine's default
     return pmm.PatchMatchOptions(window_radius=7, window_step=1, sigma_spatial=3.0, sigma_color=0.2,
                                  num_iterations=2 if geom else 8, geom_consistency=geom,
                                  init_depth_random=not geom, thk=2, seed=2, rcp_mode=rcp, image_storage=storage)
@@ -487,7 +487,7 @@ def run_filters_workload(args) -> dict:
     n = len(views)
     images = [pmm.Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
     engine = pmm.default_engine()
-    photo_opts = c2_options(pmm, False, abi.PM_RCP_EXACT, abi.PM_STORAGE_U8)
+    photo_opts = c2_options(pmm, False, abi.PM_RCP_EXACT)
     progress("photometric maps of the 11 views")
     depths, normals, photos = [], [], []
     for k in range(n):
@@ -504,7 +504,7 @@ def run_filters_workload(args) -> dict:
 
     # RunFilter (k_run_filter): a fresh problem per repetition (the filter
     # rewrites the depth map it is given)
-    fo = c2_options(pmm, True, abi.PM_RCP_EXACT, abi.PM_STORAGE_U8)
+    fo = c2_options(pmm, True, abi.PM_RCP_EXACT)
     fo.init_depth_random = False
     fprob = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths,
                         normal_maps=normals)
