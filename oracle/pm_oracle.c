@@ -1849,12 +1849,17 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   if (opt->filter && opt->conf_filter) {
     for (int color = 0; color < 2; ++color) run_color(&x, filter_pixel, color, r0, r1);
   }
+  int status = 0;
   if (view->ref_mask.data) {
     uint8_t* m = (uint8_t*)malloc(plane);
-    for (int r = 0; r < v.H; ++r)
-      memcpy(m + (size_t)r * v.W, view->ref_mask.data + (size_t)r * view->ref_mask.stride, v.W);
-    pmo_mask_depth(m, v.W, v.H, st.depth);
-    free(m);
+    if (m) {
+      for (int r = 0; r < v.H; ++r)
+        memcpy(m + (size_t)r * v.W, view->ref_mask.data + (size_t)r * view->ref_mask.stride, v.W);
+      pmo_mask_depth(m, v.W, v.H, st.depth);
+      free(m);
+    } else {
+      status = -1;
+    }
   }
   if (counters) *counters = local;
   if (out->depth) memcpy(out->depth, st.depth, sizeof(float) * plane);
@@ -1868,5 +1873,5 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   free(st.cost);
   free(st.view_sel);
   free_view(&v, rc, sc);
-  return 0;
+  return status;
 }
