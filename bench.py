@@ -566,14 +566,53 @@ def run_filters_workload(args) -> dict:
     enc = {"call_ms": round(float(np.median(times)), 3), "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1,
            "mpixel_per_s": round(px / 1e6 / (np.median(times) / 1000.0), 1),
            "bitwise_equal": bool(np.array_equal(codes, want_codes))}
+    # the steps after the iterations of the reference's geometric pass: the
+    # curvature map, the median filter and a mask (k_curvature, k_geom_grad,
+    # k_median, k_mask), timed on the device (pm_run_stats.post_ms) and
+    # checked against the transcription's chain applied to the same pass's
+    # maps without those steps
+    po = c2_options(pmm, True, abi.PM_RCP_EXACT)
+    rng = np.random.default_rng(3)
+    mask = (rng.random((H, W)) > 0.1).astype(np.uint8) * 255
+    gprob = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths,
+                        normal_maps=normals, mask_maps=[mask] + [None] * (n - 1))
+    plain_prob = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths,
+                             normal_maps=normals)
+    base = pmm.PatchMatchACMMCuda(po, plain_prob, engine)
+    base.Run()
+    d0, n0 = base.GetDepthMap(), base.GetNormalMap()
+    base.close()
+    po_post = c2_options(pmm, True, abi.PM_RCP_EXACT)
+    po_post.est_curvature = True
+    po_post.median_filter = True
+    e = pmm.PatchMatchACMMCuda(po_post, gprob, engine)
+    post_times = []
+    for _ in range(reps + 1):
+        e.Run()
+        post_times.append(e.GetStats().post_ms)
+    gd, gn, gc = e.GetDepthMap(), e.GetNormalMap(), e.GetCurvatureMap()
+    e.close()
+    t = time.perf_counter()
+    want_c = binding.curvature_map(views[0].image, n0)
+    want_d, want_n = binding.median_filter(d0, n0, po_post.depth_diff_threshold)
+    want_d = binding.mask_depth(mask, want_d)
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    post_ms = float(np.median(post_times[1:]))
+    post = {"post_ms": round(post_ms, 3), "mpixel_per_s": round(px / 1e6 / (post_ms / 1000.0), 1),
+            "algorithmic_bytes_per_px": 66,
+            "achieved_gb_s": round(66 * px / (post_ms / 1000.0) / 1e9, 1),
+            "cpu_ms": round(cpu_ms, 1), "cpu_cores": threads,
+            "bitwise_equal": bool(np.array_equal(gc, want_c) and np.array_equal(gd.view(np.uint32), want_d.view(np.uint32))
+                                  and np.array_equal(gn.view(np.uint32), want_n.view(np.uint32)))}
     for e in photos:
         e.close()
     return {"metric": "filter and encoding throughput (Mpixel/s) after the path", "value": cross["mpixel_per_s"],
             "unit": "Mpixel/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": cross["call_ms"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "1920x1080 heightfield scene, 11 views: RunFilter of the reference (10 sources), "
-                                   "cross-view filter of all 11 views (10 sources each), normal encoding"},
-            "run_filter": run_filter, "cross_filter": cross, "normal_encoding": enc}
+                                   "cross-view filter of all 11 views (10 sources each), normal encoding, "
+                                   "curvature map + median filter + mask after the geometric pass"},
+            "run_filter": run_filter, "cross_filter": cross, "normal_encoding": enc, "post_steps": post}
 
 
 def main() -> int:

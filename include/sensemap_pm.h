@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 3
+#define PM_ABI_VERSION 4
 #define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
 #define PM_MAX_WINDOW_RADIUS 32
 #define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
@@ -170,6 +170,9 @@ typedef struct pm_run_stats {
   uint64_t sweep_ncc_evaluations;
   uint64_t sweep_ncc_samples;
   uint64_t sweep_geo_evaluations;
+  /* the steps after the iterations: curvature map, median filter,
+   * FilterDepthMap, mask */
+  double post_ms;
 } pm_run_stats;
 
 typedef struct pm_engine pm_engine;

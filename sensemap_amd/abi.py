@@ -123,6 +123,7 @@ class pm_run_stats(C.Structure):
         ("sweep_ncc_evaluations", C.c_uint64),
         ("sweep_ncc_samples", C.c_uint64),
         ("sweep_geo_evaluations", C.c_uint64),
+        ("post_ms", C.c_double),
     ]
 
 

@@ -820,6 +820,7 @@ void collect_times(const pm_schedule& sch, const EventTimer& t, size_t first, pm
     if (l.kind == pm::kPhaseSweep) st->sweep_ms += ms;
     if (l.kind == pm::kPhaseLocal) st->local_opt_ms += ms;
     if (l.kind == pm::kPhaseInit) st->init_ms += ms;
+    if (pm::plain_phase(l.kind)) st->post_ms += ms;
     k += 2;
   }
 }

@@ -629,6 +629,11 @@ __device__ __forceinline__ int claim_tile(const LaunchTable* __restrict__ lt, in
       const int x = (xcd + k) & 7;
       const int lo = (int)(((long long)T * x) >> 3), hi = (int)(((long long)T * (x + 1)) >> 3);
       if (lo >= hi) continue;
+      // a range already used up is skipped without an atomic: at the end of
+      // a launch every workgroup scans all 8 ranges, and those atomics on 8
+      // addresses serialise (counters only grow, so a stale read costs one
+      // atomic, never a tile)
+      if (__hip_atomic_load(q + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)(hi - lo)) continue;
       const uint32_t c = atomicAdd(q + x, 1u);
       if (c < (uint32_t)(hi - lo)) tile = lo + (int)c;
     }
@@ -1211,6 +1216,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_local_opt(const LaunchTable* 
     local_opt_tile<R_, S_, STORE, RCP>(resolve(lt, t), tile, ct, spl);
 }
 
+// Tiles of a launch of an element-wise phase, assigned by workgroup index
+// (no LDS, no barriers).  The work per tile is a few loads, so the tile
+// queue's atomics (all on one cache line) would cost more than the tiles.
+template <typename F>
+__device__ __forceinline__ void for_each_tile(const LaunchTable* __restrict__ lt, F&& f) {
+  const int T = __builtin_amdgcn_readfirstlane(G(lt)->total_tiles);
+  for (int t = blockIdx.x; t < T; t += gridDim.x) f(t);
+}
+
 // ---------------------------------------------------------------------------
 // FilterDepthMap (kernel_functions.cu:3282-3354)
 
@@ -1260,8 +1274,7 @@ __device__ __forceinline__ void filter_tile(const Ctx& cx) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_filter(const LaunchTable* __restrict__ lt) {
-  __shared__ int next;
-  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) filter_tile(resolve(lt, t));
+  for_each_tile(lt, [&](int t) { filter_tile(resolve(lt, t)); });
 }
 
 // Pixel of a colour-pass tile (the mapping of filter_tile); false outside the view.
@@ -1374,8 +1387,7 @@ __device__ __forceinline__ void curvature_tile(const Ctx& cx) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_curvature(const LaunchTable* __restrict__ lt) {
-  __shared__ int next;
-  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) curvature_tile(resolve(lt, t));
+  for_each_tile(lt, [&](int t) { curvature_tile(resolve(lt, t)); });
 }
 
 // MaskDepthMapKernel (kernel_functions.cu:3080-3093): zero the depth where
@@ -1390,8 +1402,7 @@ __device__ __forceinline__ void mask_tile(const Ctx& cx) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_mask(const LaunchTable* __restrict__ lt) {
-  __shared__ int next;
-  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) mask_tile(resolve(lt, t));
+  for_each_tile(lt, [&](int t) { mask_tile(resolve(lt, t)); });
 }
 
 // ComputeGeomGradientMap (kernel_functions.cu:3095-3120): Scharr gradient of
@@ -1427,8 +1438,7 @@ __device__ __forceinline__ void geom_grad_tile(const Ctx& cx) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_geom_grad(const LaunchTable* __restrict__ lt) {
-  __shared__ int next;
-  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) geom_grad_tile(resolve(lt, t));
+  for_each_tile(lt, [&](int t) { geom_grad_tile(resolve(lt, t)); });
 }
 
 // MedianFilter (kernel_functions.cu:3122-3244) for the pixels of one colour
@@ -1497,8 +1507,7 @@ __device__ __forceinline__ void median_tile(const Ctx& cx) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_median(const LaunchTable* __restrict__ lt) {
-  __shared__ int next;
-  for (int t = claim_tile(lt, &next); t >= 0; t = claim_tile(lt, &next)) median_tile(resolve(lt, t));
+  for_each_tile(lt, [&](int t) { median_tile(resolve(lt, t)); });
 }
 
 // RunFilter (patch_match_cuda.cu:1074-1108): FilterDepthMapsKernel
