@@ -48,10 +48,14 @@
  * float64 brute-force median and a designed case of NaN angles; the mask
  * (pmo_mask_depth) by its skipped last row and column and its place after
  * FilterDepthMap in a run.
- * Parity unpinned: LocalOptimization's cost refresh for the sources it did
- * not select, and the order of the passes, are checked only end to end
- * (depth against rendered ground truth), by the device's bitwise agreement
- * and by the launch counts of a run.
+ * LocalOptimization's cost refresh for the sources it did not select is
+ * pinned in photometric runs: every stored cost equals a fresh FP32 NCC at
+ * the final maps (a transcription without the refresh fails that).
+ * Parity unpinned: that refresh's geometric term, which reproduces the
+ * reference passing ref_invK as K (kernel_functions.cu:2924), and the order
+ * of the passes, are checked only end to end (depth against rendered ground
+ * truth), by the device's bitwise agreement and by the launch counts of a
+ * run.
  */
 #ifndef SENSEMAP_PM_ORACLE_H_
 #define SENSEMAP_PM_ORACLE_H_
