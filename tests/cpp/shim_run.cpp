@@ -4,7 +4,11 @@
 //                      then per image (ref first): fx cx fy cy R[9] T[3]
 //   <dir>/img<k>.u8  : W*H bytes per image
 //   <dir>/mask.u8    : optional W*H mask of the reference image (Problem::mask_maps)
-// and writes <dir>/depth.f32 and <dir>/normal.f32.
+//   <dir>/geom.txt   : optional; present: a geometric pass with the curvature map and the
+//                      median filter, reading <dir>/depth<k>.f32 (W*H per image) and
+//                      <dir>/normal0.f32 (3*W*H, the reference's initial normals)
+// and writes <dir>/depth.f32 and <dir>/normal.f32 (and <dir>/curvature.u16 in the
+// geometric pass).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -42,8 +46,11 @@ int main(int argc, char** argv) {
   options.num_iterations = iters;
   options.thk = thk;
   options.seed = seed;
-  options.geom_consistency = false;
-  options.init_depth_random = true;
+  const bool geom = (bool)std::ifstream(dir + "/geom.txt");
+  options.geom_consistency = geom;
+  options.init_depth_random = !geom;
+  options.est_curvature = geom;
+  options.median_filter = geom;
   options.depth_min = dmin;
   options.depth_max = dmax;
   options.gpu_index = "0";
@@ -55,6 +62,23 @@ int main(int argc, char** argv) {
   problem.ref_image_idx = 0;
   for (int k = 1; k <= N; ++k) problem.src_image_idxs.push_back(k);
   problem.images = &images;
+  std::vector<DepthMap> depth_maps;
+  std::vector<NormalMap> normal_maps;
+  if (geom) {
+    for (int k = 0; k <= N; ++k) {
+      DepthMap d(W, H, (float)dmin, (float)dmax);
+      std::ifstream(dir + "/depth" + std::to_string(k) + ".f32", std::ios::binary)
+          .read((char*)d.GetPtr(), (std::streamsize)(sizeof(float) * W * H));
+      depth_maps.push_back(std::move(d));
+      NormalMap n(W, H);
+      if (k == 0)
+        std::ifstream(dir + "/normal0.f32", std::ios::binary)
+            .read((char*)n.GetPtr(), (std::streamsize)(sizeof(float) * 3 * W * H));
+      normal_maps.push_back(std::move(n));
+    }
+    problem.depth_maps = &depth_maps;
+    problem.normal_maps = &normal_maps;
+  }
   std::vector<Bitmap> masks(images.size());
   std::ifstream mf(dir + "/mask.u8", std::ios::binary);
   if (mf) {
@@ -72,6 +96,11 @@ int main(int argc, char** argv) {
         .write((const char*)depth.GetPtr(), (std::streamsize)(sizeof(float) * W * H));
     std::ofstream(dir + "/normal.f32", std::ios::binary)
         .write((const char*)normal.GetPtr(), (std::streamsize)(sizeof(float) * 3 * W * H));
+    if (geom) {
+      const Mat<unsigned short> curv = engine.GetCurvatureMap();
+      std::ofstream(dir + "/curvature.u16", std::ios::binary)
+          .write((const char*)curv.GetPtr(), (std::streamsize)(sizeof(unsigned short) * 2 * W * H));
+    }
   } catch (const std::exception& e) {
     std::cerr << e.what() << "\n";
     return 4;

@@ -16,11 +16,14 @@ BIN = os.path.join(ROOT, "tests", "cpp", "bin", "shim_run")
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("with_mask", [False, True])
-def test_cpp_mirror_matches_python_path(tmp_path, with_mask):
+@pytest.mark.parametrize("with_mask,geom", [(False, False), (True, False), (True, True)])
+def test_cpp_mirror_matches_python_path(tmp_path, with_mask, geom):
     assert os.path.exists(BIN), "tests/cpp/bin/shim_run is built by __graft_entry__.build()"
     sc = helpers.small_planes()
-    opts = helpers.c1_options(num_iterations=3)
+    opts = helpers.c1_options(num_iterations=2 if geom else 3)
+    if geom:
+        opts = helpers.c1_options(num_iterations=2, geom_consistency=True, init_depth_random=False,
+                                  est_curvature=True, median_filter=True)
     views = [sc.ref] + sc.srcs
     lines = [f"{160} {120} {len(sc.srcs)} {sc.depth_min!r} {sc.depth_max!r} {opts.window_radius} {opts.window_step} "
              f"{opts.sigma_spatial!r} {opts.sigma_color!r} {opts.num_iterations} {opts.thk} {opts.seed}"]
@@ -31,7 +34,12 @@ def test_cpp_mirror_matches_python_path(tmp_path, with_mask):
         lines.append(" ".join(repr(float(np.float32(x))) for x in vals))
         v.image.tofile(str(tmp_path / f"img{k}.u8"))
     (tmp_path / "meta.txt").write_text("\n".join(lines) + "\n")
-    prob = scene.to_problem(sc)
+    prob = scene.to_problem(sc, with_gt_maps=geom)
+    if geom:
+        (tmp_path / "geom.txt").write_text("1\n")
+        for k, v in enumerate(views):
+            np.ascontiguousarray(prob.depth_maps[k], np.float32).tofile(str(tmp_path / f"depth{k}.f32"))
+        np.ascontiguousarray(prob.normal_maps[0], np.float32).tofile(str(tmp_path / "normal0.f32"))
     if with_mask:
         mask = np.full((120, 160), 255, np.uint8)
         mask[20:70, 30:110] = 0
@@ -46,6 +54,9 @@ def test_cpp_mirror_matches_python_path(tmp_path, with_mask):
     assert np.array_equal(n_cpp.view(np.uint32), e.GetNormalMap().view(np.uint32))
     if with_mask:
         assert np.all(d_cpp[20:70, 30:110] == 0)
+    if geom:
+        c_cpp = np.fromfile(str(tmp_path / "curvature.u16"), np.uint16).reshape(2, 120, 160)
+        assert np.array_equal(c_cpp, e.GetCurvatureMap())
 
 
 def test_cpp_mirror_header_compiles(tmp_path):
