@@ -7,7 +7,9 @@ size:
   at the engine's own final hypotheses (the same bar as the precision test:
   costs of sources the local optimisation did not select may be older);
 - the geometric pass, fed the photometric result on the device, keeps the
-  depth error against the rendered ground truth small."""
+  depth error against the rendered ground truth small;
+- the curvature map, the median filter and a mask after the geometric pass
+  equal the transcription's chain applied to the pass's maps without them."""
 import functools
 
 import numpy as np
@@ -106,3 +108,34 @@ def test_full_size_geometric_pass_on_the_device():
     assert np.mean(err_g < 0.01) > 0.9
     for s in srcs + [photo, geo]:
         s.close()
+
+
+def test_full_size_post_run_steps_equal_the_transcription():
+    # the curvature map, the median filter and a mask after the geometric
+    # pass of the 1920x1080 view, against the transcription's chain applied
+    # to the same pass's maps without those steps
+    sc = view_scene()
+    views = [sc.ref] + sc.srcs
+    n = len(views)
+    images = [pmm.Image(v.image, v.camera.K, v.camera.R, v.camera.T) for v in views]
+    depths = [v.depth for v in views]
+    normals = [v.normal for v in views]
+    H, W = sc.ref.image.shape
+    mask = (np.random.default_rng(3).random((H, W)) > 0.1).astype(np.uint8) * 255
+    print("[full] geometric pass with and without the post-run steps", flush=True)
+    plain = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths, normal_maps=normals)
+    e, m = run(opts(geom=True), plain)
+    e.close()
+    post = pmm.Problem(0, list(range(1, n)), images, sc.depth_min, sc.depth_max, depth_maps=depths,
+                       normal_maps=normals, mask_maps=[mask] + [None] * (n - 1))
+    o = opts(geom=True, est_curvature=True, median_filter=True)
+    e, g = run(o, post)
+    curv = e.GetCurvatureMap()
+    e.close()
+    assert np.array_equal(curv, binding.curvature_map(sc.ref.image, m["normal"]))
+    d, nrm = binding.median_filter(m["depth"], m["normal"], o.depth_diff_threshold)
+    d = binding.mask_depth(mask, d)
+    assert np.array_equal(g["depth"].view(np.uint32), d.view(np.uint32))
+    assert np.array_equal(g["normal"].view(np.uint32), nrm.view(np.uint32))
+    for k in ("conf", "cost", "view_sel"):
+        assert np.array_equal(g[k], m[k])
