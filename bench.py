@@ -772,6 +772,36 @@ def main() -> int:
         full = {"workload": "10 source photometric passes + the reference's photometric and geometric passes",
                 "views": n_views, "mpixel_iter": round(full_mpix, 2), "seconds": round(full_s, 3),
                 "mpixel_iter_per_s": round(full_mpix / full_s, 3), "views_per_s": round(n_views / full_s, 4)}
+    # the reference's flow for the same step: each pass's problem created from
+    # host buffers (images; for the geometric pass the sources' depth maps and
+    # the photometric maps as its initial state), run, maps read back
+    host = None
+    if not args.only_photometric:
+        src_depths = [p.GetDepthMap() for p in src_photo]
+
+        def host_step():
+            t0 = time.perf_counter()
+            ph = pmm.PatchMatchACMMCuda(photo_opts, ref_prob, engine)
+            ph.Run()
+            d, nm = ph.GetDepthMap(), ph.GetNormalMap()
+            ph.close()
+            gprob = pmm.Problem(0, list(range(1, n_views)), images, sc.depth_min, sc.depth_max,
+                                depth_maps=[d] + src_depths, normal_maps=[nm] + [v.normal for v in views[1:]])
+            g = pmm.PatchMatchACMMCuda(c2_options(pmm, True, rcp, storage), gprob, engine)
+            g.Run()
+            gd = g.GetDepthMap()
+            g.GetNormalMap()
+            g.close()
+            return (time.perf_counter() - t0) * 1000.0, gd
+
+        host_step()
+        runs = [host_step() for _ in range(2)]
+        hms = float(np.median([r[0] for r in runs]))
+        host = {"workload": "the same step with each pass's problem created from host buffers and its depth and "
+                            "normal maps read back (images, source depth maps and initial maps cross PCIe)",
+                "ms_per_step": round(hms, 2), "value": round(mpix_iter / (hms / 1000.0), 3),
+                "maps_equal_to_device_flow": bool(np.array_equal(runs[-1][1].view(np.uint32),
+                                                                 geo.GetDepthMap().view(np.uint32)))}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -796,6 +826,7 @@ def main() -> int:
         "other_storage": other,
         "fast_reciprocal": fast,
         "full_pipeline": full,
+        "host_buffers": host,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "roofline": {
