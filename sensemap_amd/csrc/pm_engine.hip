@@ -85,6 +85,39 @@ void Arena::release(size_t off, size_t bytes) {
   }
 }
 
+hipError_t DevCache::get(size_t bytes, void** out) {
+  const size_t b = round(bytes);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = blocks_.find(b);
+    if (it != blocks_.end()) {
+      *out = it->second;
+      blocks_.erase(it);
+      bytes_ -= b;
+      return hipSuccess;
+    }
+  }
+  return hipMalloc(out, b);
+}
+
+void DevCache::put(void* p, size_t bytes) {
+  if (!p) return;
+  const size_t b = round(bytes);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (bytes_ + b <= kMaxBytes) {
+      blocks_.emplace(b, p);
+      bytes_ += b;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+DevCache::~DevCache() {
+  for (auto& kv : blocks_) (void)hipFree(kv.second);
+}
+
 size_t Arena::used() const {
   std::lock_guard<std::mutex> lk(mu_);
   return used_;
@@ -389,8 +422,18 @@ size_t job_block_bytes(const pm_options* o, const pm_view_desc* v, const bool* s
   return b;
 }
 
+// An 8-bit host image into a dense W x H device buffer.  A contiguous image
+// goes as one copy: from pageable memory a 2-D copy moves row by row, which
+// for 1080 rows costs about 10 ms per image.
+pm_status upload_image(uint8_t* dst, const uint8_t* src, int W, int H, int stride, hipStream_t s) {
+  if (stride == W) PM_HIP(hipMemcpyAsync(dst, src, (size_t)W * H, hipMemcpyHostToDevice, s));
+  else PM_HIP(hipMemcpy2DAsync(dst, W, src, stride, W, H, hipMemcpyHostToDevice, s));
+  return PM_OK;
+}
+
 pm_status upload_quads(const pm_image& im, int storage, void* dst, uint8_t* staging, hipStream_t s) {
-  PM_HIP(hipMemcpy2DAsync(staging, im.width, im.data, im.stride, im.width, im.height, hipMemcpyHostToDevice, s));
+  pm_status rs = upload_image(staging, im.data, im.width, im.height, im.stride, s);
+  if (rs != PM_OK) return rs;
   if (storage == PM_STORAGE_U8)
     PM_HIP(pm::launch_build_quads_u8(staging, im.width, im.height, im.width, (uint32_t*)dst, s));
   else
@@ -409,15 +452,22 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   PM_HIP(hipSetDevice(e->device));
   pm_problem* p = new (std::nothrow) pm_problem();
   if (!p) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  hipStream_t s = env.stream ? env.stream : e->stream;
+  // on an error path uploads may still read the staging buffer and write the
+  // problem's buffers: wait for the stream before handing them back
   struct Guard {
     pm_problem*& p;
+    pm_engine* e;
+    hipStream_t s;
     uint8_t* staging = nullptr;  // temporary staging (engine problems)
+    size_t staging_bytes = 0;
     bool keep = false;
     ~Guard() {
-      if (staging) (void)hipFree(staging);
+      if (staging || !keep) (void)hipStreamSynchronize(s);
+      if (staging) e->cache.put(staging, staging_bytes);
       if (!keep) delete p;
     }
-  } guard{p};
+  } guard{p, e, s};
   p->engine = e;
   p->opt = *o;
   p->W = v->ref.width;
@@ -434,14 +484,13 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   }
   const int W = p->W, H = p->H, N = p->N;
   const size_t plane = (size_t)W * H;
-  hipStream_t s = env.stream ? env.stream : e->stream;
   pm::ViewDev& h = p->host;
   std::memset(&h, 0, sizeof(h));
 
   // reference image and tables
   uint8_t* ref = nullptr;
   if ((st = p->alloc(&ref, plane)) != PM_OK) return st;
-  PM_HIP(hipMemcpy2DAsync(ref, W, v->ref.data, v->ref.stride, W, H, hipMemcpyHostToDevice, s));
+  if ((st = upload_image(ref, v->ref.data, W, H, v->ref.stride, s)) != PM_OK) return st;
   std::vector<float> sp, ct;
   int nwin = 0;
   weight_tables(o, sp, ct, &nwin);
@@ -464,7 +513,10 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   }
   uint8_t* staging = env.staging;
   if (!staging) {
-    PM_HIP(hipMalloc(&guard.staging, max_src));
+    void* q = nullptr;
+    PM_HIP(e->cache.get(max_src, &q));
+    guard.staging = (uint8_t*)q;
+    guard.staging_bytes = max_src;
     staging = guard.staging;
   } else if (env.staging_bytes < max_src) {
     return fail(PM_ERR_INTERNAL, "staging buffer too small");
@@ -529,7 +581,7 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
   if (v->ref_mask.data) {
     uint8_t* m = nullptr;
     if ((st = p->alloc(&m, plane)) != PM_OK) return st;
-    PM_HIP(hipMemcpy2DAsync(m, W, v->ref_mask.data, v->ref_mask.stride, W, H, hipMemcpyHostToDevice, s));
+    if ((st = upload_image(m, v->ref_mask.data, W, H, v->ref_mask.stride, s)) != PM_OK) return st;
     h.mask = m;
   }
   if (!o->init_depth_random) {
@@ -652,7 +704,7 @@ std::vector<std::vector<PhaseRec>> view_ticks(const pm_problem& p) {
 
 pm_problem::~pm_problem() {
   delete schedule;
-  for (void* q : allocs) (void)hipFree(q);
+  for (const auto& a : allocs) engine->cache.put(a.first, a.second);
 }
 
 PM_API void pm_problem_destroy(pm_problem* p) {
@@ -753,20 +805,22 @@ pm_status build_schedule(const std::vector<pm_problem*>& views, const Ticks& tic
   sch.tables = std::move(plan.tables);
   sch.launches = std::move(plan.launches);
   sch.ws_floats = plan.ws_floats;
-  if (sch.ws) PM_HIP(hipFree(sch.ws));
-  if (sch.queues) PM_HIP(hipFree(sch.queues));
-  if (sch.dev) PM_HIP(hipFree(sch.dev));
-  sch.ws = nullptr;
-  sch.queues = nullptr;
-  sch.dev = nullptr;
-  PM_HIP(hipMalloc(&sch.ws, std::max<size_t>(sch.ws_floats, 1) * sizeof(float)));
-  PM_HIP(hipMalloc(&sch.queues, std::max<size_t>(8 * sch.launches.size(), 1) * sizeof(uint32_t)));
+  sch.release();
+  auto get = [&](void** p, size_t bytes) -> hipError_t {
+    if (sch.cache) return sch.cache->get(bytes, p);
+    return hipMalloc(p, bytes);
+  };
+  sch.ws_bytes = std::max<size_t>(sch.ws_floats, 1) * sizeof(float);
+  sch.queue_bytes = std::max<size_t>(8 * sch.launches.size(), 1) * sizeof(uint32_t);
+  sch.dev_bytes = std::max<size_t>(sch.tables.size(), 16);
+  PM_HIP(get((void**)&sch.ws, sch.ws_bytes));
+  PM_HIP(get((void**)&sch.queues, sch.queue_bytes));
   for (size_t k = 0; k < sch.launches.size(); ++k) {
     pm::LaunchTable* lt = (pm::LaunchTable*)(sch.tables.data() + sch.launches[k].offset);
     lt->ws = sch.ws;
     lt->queue = sch.queues + 8 * k;
   }
-  PM_HIP(hipMalloc(&sch.dev, std::max<size_t>(sch.tables.size(), 16)));
+  PM_HIP(get(&sch.dev, sch.dev_bytes));
   PM_HIP(hipMemcpyAsync(sch.dev, sch.tables.data(), sch.tables.size(), hipMemcpyHostToDevice, s));
   return PM_OK;
 }
@@ -1133,10 +1187,8 @@ PM_API pm_status pm_batch_run(pm_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
-    if (b->schedule.dev) {
-      PM_HIP(hipFree(b->schedule.dev));
-      b->schedule.dev = nullptr;
-    }
+    PM_HIP(hipStreamSynchronize(e->stream));
+    b->schedule.release();
   }
   if (!b->use_graph || e->counting) return run_views(e, b->views, b->schedule, &b->stats);
   PM_HIP(hipSetDevice(e->device));
@@ -1196,8 +1248,11 @@ PM_API void pm_batch_destroy(pm_batch* b) {
 
 PM_API pm_status pm_problem_run(pm_problem* p) {
   if (!p) return fail(PM_ERR_INVALID_ARG, "null problem");
-  if (!p->schedule) p->schedule = new (std::nothrow) pm_schedule();
-  if (!p->schedule) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  if (!p->schedule) {
+    p->schedule = new (std::nothrow) pm_schedule();
+    if (!p->schedule) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+    p->schedule->cache = &p->engine->cache;
+  }
   return run_views(p->engine, {p}, *p->schedule, &p->stats);
 }
 

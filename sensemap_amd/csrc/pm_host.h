@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -86,8 +87,32 @@ struct CreateEnv {
 
 }  // namespace pmh
 
+namespace pmh {
+
+// Device blocks an engine keeps for reuse.  Problems and their schedules
+// hand their buffers back here instead of freeing them, so creating a
+// problem of a size seen before allocates nothing (the reference's caller
+// builds a new engine per problem, paying every allocation).  Blocks are
+// matched by size rounded to 64 KiB; at most kMaxBytes are kept, the rest is
+// freed.  Everything is freed with the engine.
+struct DevCache {
+  static constexpr size_t kMaxBytes = (size_t)16 << 30;
+  static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 0xffff) & ~(size_t)0xffff; }
+  hipError_t get(size_t bytes, void** out);
+  void put(void* p, size_t bytes);
+  ~DevCache();
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> blocks_;
+  size_t bytes_ = 0;
+};
+
+}  // namespace pmh
+
 struct pm_engine {
   int device = 0;
+  pmh::DevCache cache;
   hipStream_t stream = nullptr;
   int counting = 0;
   std::mutex tune_mu;
@@ -100,7 +125,7 @@ struct pm_problem {
   int W = 0, H = 0, N = 0;
   uint32_t view_id = 0;
   float depth_min = 0.f, depth_max = 0.f;
-  std::vector<void*> allocs;           // hipMalloc'd (engine problems)
+  std::vector<std::pair<void*, size_t>> allocs;  // from the engine's block cache (engine problems)
   pmh::Arena* arena = nullptr;         // arena-backed (server jobs): one range, carved in order
   size_t block_off = 0, block_bytes = 0, block_used = 0;
   pm::ViewDev host{};
@@ -128,12 +153,12 @@ struct pm_problem {
       return PM_OK;
     }
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 1);
+    hipError_t e = engine->cache.get(bytes, &q);
     if (e != hipSuccess) {
       return pmh::fail(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
                        "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
     }
-    allocs.push_back(q);
+    allocs.emplace_back(q, bytes);
     device_bytes += bytes;
     *p = (T*)q;
     return PM_OK;
@@ -213,11 +238,21 @@ struct pm_schedule {
   uint32_t* queues = nullptr;  // 8 tile counters per launch
   float* ws = nullptr;         // workspace sized for the largest launch
   size_t ws_floats = 0;
-  ~pm_schedule() {
-    if (dev) (void)hipFree(dev);
-    if (queues) (void)hipFree(queues);
-    if (ws) (void)hipFree(ws);
+  pmh::DevCache* cache = nullptr;  // non-null: buffers come from and go back to this cache
+  size_t dev_bytes = 0, queue_bytes = 0, ws_bytes = 0;
+  void release() {
+    void* p[3] = {dev, queues, ws};
+    const size_t b[3] = {dev_bytes, queue_bytes, ws_bytes};
+    for (int k = 0; k < 3; ++k) {
+      if (!p[k]) continue;
+      if (cache) cache->put(p[k], b[k]);
+      else (void)hipFree(p[k]);
+    }
+    dev = nullptr;
+    queues = nullptr;
+    ws = nullptr;
   }
+  ~pm_schedule() { release(); }
 };
 
 namespace pmh {
