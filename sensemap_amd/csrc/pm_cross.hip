@@ -160,10 +160,24 @@ __global__ __launch_bounds__(256) void k_cross_filter(const CrossViewDev* __rest
 // to the float argument of cosf (kernel_functions.h:39).
 float cos_deg(double deg) { return (float)std::cos((double)(float)(deg * 0.0174532925199432)); }
 
+// A device buffer of one call, borrowed from the engine's block cache; the
+// stream is synchronised before the block goes back (a call that fails may
+// leave work on it).
 struct DevBuf {
   void* p = nullptr;
+  pmh::DevCache* cache = nullptr;
+  hipStream_t s = nullptr;
+  size_t bytes = 0;
+  hipError_t get(pm_engine* e, hipStream_t st, size_t b) {
+    cache = &e->cache;
+    s = st;
+    bytes = b;
+    return cache->get(b, &p);
+  }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    (void)hipStreamSynchronize(s);
+    cache->put(p, bytes);
   }
 };
 
@@ -260,10 +274,10 @@ pm_status run_cross(pm_engine* e, const pm_cross_options* o, const std::vector<C
     tiles += (int64_t)((ref.width + kCrossTile - 1) / kCrossTile) * ((ref.height + kCrossTile - 1) / kCrossTile);
     if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "cross filter: too many pixels in one call");
   }
-  PM_HIP(hipMalloc(&d_poses.p, std::max<size_t>(poses.size(), 1) * sizeof(float)));
+  PM_HIP(d_poses.get(e, st, std::max<size_t>(poses.size(), 1) * sizeof(float)));
   PM_HIP(hipMemcpyAsync(d_poses.p, poses.data(), poses.size() * sizeof(float), hipMemcpyHostToDevice, st));
   for (int32_t k = 0; k < num_views; ++k) tab[k].poses = (const float*)d_poses.p + pose_off[k];
-  PM_HIP(hipMalloc(&d_tab.p, tab.size() * sizeof(CrossViewDev)));
+  PM_HIP(d_tab.get(e, st, tab.size() * sizeof(CrossViewDev)));
   PM_HIP(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(CrossViewDev), hipMemcpyHostToDevice, st));
 
   CrossParams prm;
@@ -322,8 +336,8 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
     for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
   }
   DevBuf d_depth, d_codes, d_out, d_poses, d_tab;
-  PM_HIP(hipMalloc(&d_depth.p, (size_t)img_total * sizeof(float)));
-  PM_HIP(hipMalloc(&d_codes.p, (size_t)img_total * sizeof(uint32_t)));
+  PM_HIP(d_depth.get(e, st, (size_t)img_total * sizeof(float)));
+  PM_HIP(d_codes.get(e, st, (size_t)img_total * sizeof(uint32_t)));
   std::vector<CrossImgDev> imgs(num_images);
   for (int32_t i = 0; i < num_images; ++i) {
     imgs[i].width = images[i].width;
@@ -340,7 +354,7 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
   }
   int64_t out_total = 0;
   const std::vector<int64_t> off = out_offsets(imgs, num_views, views, &out_total);
-  PM_HIP(hipMalloc(&d_out.p, (size_t)out_total * sizeof(float)));
+  PM_HIP(d_out.get(e, st, (size_t)out_total * sizeof(float)));
   if (pm_status s = run_cross(e, o, imgs, num_views, views, (float*)d_out.p, off, d_poses, d_tab)) return s;
   for (int32_t k = 0; k < num_views; ++k) {
     const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
@@ -382,7 +396,7 @@ PM_API pm_status pm_cross_filter_problems(pm_engine* e, const pm_cross_options* 
     for (int32_t s = 0; s < views[k].num_src; ++s) use(views[k].src[s]);
   }
   DevBuf d_codes, d_out, d_poses, d_tab;
-  PM_HIP(hipMalloc(&d_codes.p, (size_t)std::max<int64_t>(code_total, 1) * sizeof(uint32_t)));
+  PM_HIP(d_codes.get(e, st, (size_t)std::max<int64_t>(code_total, 1) * sizeof(uint32_t)));
   std::vector<CrossImgDev> imgs(num_problems);
   for (int32_t i = 0; i < num_problems; ++i) {
     const pm_problem* p = problems[i];
@@ -397,7 +411,7 @@ PM_API pm_status pm_cross_filter_problems(pm_engine* e, const pm_cross_options* 
   }
   int64_t out_total = 0;
   const std::vector<int64_t> off = out_offsets(imgs, num_views, views, &out_total);
-  PM_HIP(hipMalloc(&d_out.p, (size_t)std::max<int64_t>(out_total, 1) * sizeof(float)));
+  PM_HIP(d_out.get(e, st, (size_t)std::max<int64_t>(out_total, 1) * sizeof(float)));
   if (pm_status s = run_cross(e, o, imgs, num_views, views, (float*)d_out.p, off, d_poses, d_tab)) return s;
   // after the launch (every view has read its sources): results out
   for (int32_t k = 0; k < num_views; ++k) {
