@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--jobs", type=int, default=24, help="serve workload: jobs per arrival rate")
     ap.add_argument("--rates", default="5,20,50,200", help="serve workload: arrival rates, requests/s")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve workload: HBM admission budget")
-    ap.add_argument("--max-active", type=int, default=16, help="serve workload: jobs on the device at once")
+    ap.add_argument("--max-active", type=int, default=4, help="serve workload: jobs on the device at once")
     return ap.parse_args()
 
 

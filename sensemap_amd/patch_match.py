@@ -458,7 +458,7 @@ class PatchMatchServer:
     shared launches.  Uses the engine's launch shapes."""
 
     def __init__(self, engine: Optional[Engine] = None, hbm_budget_bytes: int = 16 << 30,
-                 max_active_views: int = 64):
+                 max_active_views: int = 4):
         self.engine = engine or default_engine()
         self.lib = self.engine.lib
         self._inflight = {}  # ticket -> ServerJob not yet waited for
