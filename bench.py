@@ -275,8 +275,9 @@ def run_serve_workload(args) -> dict:
     over a 500-image orbit; job sizes from the 64-view batch distribution
     (640x480..3000x2000, 5-20 sources, window 11x11 s2 or 15x15 s1, 6
     iterations, f16 storage).  Per arrival rate: views/s, p50/p99 latency
-    (submit -> maps in host memory) and the cache hit rate.  The plain path
-    serves the same jobs one at a time (create + run + read back)."""
+    (submit -> depth map in host memory) and the cache hit rate.  The plain
+    path serves the same jobs one at a time (create + run + read back the
+    depth map)."""
     from sensemap_amd import abi, scene
     from sensemap_amd import patch_match as pmm
 
@@ -334,7 +335,7 @@ def run_serve_workload(args) -> dict:
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            handles[k] = server.submit(o, p)
+            handles[k] = server.submit(o, p, maps=("depth",))  # what the plain path reads back too
         lat = []
         for k, h in enumerate(handles):
             st = h.wait()

@@ -400,24 +400,33 @@ class ServerJob:
     """One depth-map job of a PatchMatchServer: owns the view's input buffers
     and the output maps the server writes."""
 
-    def __init__(self, server: "PatchMatchServer", options: PatchMatchOptions, problem: Problem):
+    MAPS = ("depth", "normal", "conf", "cost", "view_sel", "curvature")
+
+    def __init__(self, server: "PatchMatchServer", options: PatchMatchOptions, problem: Problem, maps=None):
+        """maps: the names of the maps to return (default all); the others
+        are neither allocated nor copied back."""
         self.server = server
         self.options = options
         self.desc = ViewDesc(problem, options)
         W, H, N = self.desc.width, self.desc.height, self.desc.num_src
-        self.depth = np.empty((H, W), np.float32)
-        self.normal = np.empty((3, H, W), np.float32)
-        self.conf = np.empty((H, W), np.float32)
-        self.cost = np.empty((N, H, W), np.float32)
-        self.view_sel = np.empty((H, W), np.uint32)
-        self.curvature = (np.empty((2, H, W), np.uint16)
-                          if options.geom_consistency and options.est_curvature else None)
-        f = C.POINTER(C.c_float)
-        self._out = abi.pm_outputs(self.depth.ctypes.data_as(f), self.normal.ctypes.data_as(f),
-                                   self.conf.ctypes.data_as(f), self.cost.ctypes.data_as(f),
-                                   self.view_sel.ctypes.data_as(C.POINTER(C.c_uint32)))
-        if self.curvature is not None:
-            self._out.curvature = self.curvature.ctypes.data_as(C.POINTER(C.c_uint16))
+        want = set(self.MAPS if maps is None else maps)
+        unknown = want - set(self.MAPS)
+        if unknown:
+            raise ValueError(f"unknown maps {sorted(unknown)}")
+        if not (options.geom_consistency and options.est_curvature):
+            want.discard("curvature")
+        shapes = {"depth": ((H, W), np.float32), "normal": ((3, H, W), np.float32), "conf": ((H, W), np.float32),
+                  "cost": ((N, H, W), np.float32), "view_sel": ((H, W), np.uint32),
+                  "curvature": ((2, H, W), np.uint16)}
+        ctypes_of = {np.float32: C.c_float, np.uint32: C.c_uint32, np.uint16: C.c_uint16}
+        self._out = abi.pm_outputs()
+        for name in self.MAPS:
+            arr = None
+            if name in want:
+                shape, dt = shapes[name]
+                arr = np.empty(shape, dt)
+                setattr(self._out, name, arr.ctypes.data_as(C.POINTER(ctypes_of[dt])))
+            setattr(self, name, arr)
         self._copt = options.to_c()
         t = C.c_uint64()
         abi.check(server.lib.pm_server_submit(server.handle, C.byref(self._copt), C.byref(self.desc.desc),
@@ -445,11 +454,7 @@ class ServerJob:
 
     def maps(self) -> dict:
         self.wait()
-        m = {"depth": self.depth, "normal": self.normal, "conf": self.conf, "cost": self.cost,
-             "view_sel": self.view_sel}
-        if self.curvature is not None:
-            m["curvature"] = self.curvature
-        return m
+        return {name: getattr(self, name) for name in self.MAPS if getattr(self, name) is not None}
 
 
 class PatchMatchServer:
@@ -467,8 +472,9 @@ class PatchMatchServer:
         abi.check(self.lib.pm_server_create(self.engine.handle, C.byref(cfg), C.byref(h)))
         self.handle = h
 
-    def submit(self, options: PatchMatchOptions, problem: Problem) -> ServerJob:
-        return ServerJob(self, options, problem)
+    def submit(self, options: PatchMatchOptions, problem: Problem, maps=None) -> ServerJob:
+        """Queue a job; maps: the names of the maps to return (default all)."""
+        return ServerJob(self, options, problem, maps)
 
     def stats(self) -> abi.pm_server_stats:
         s = abi.pm_server_stats()
