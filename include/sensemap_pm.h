@@ -190,6 +190,10 @@ int32_t pm_abi_version(void);
 /* Select device `device` and create a HIP stream owned by the engine
  * (SetBestCudaDevice, src/util/cuda.cc:36-61). */
 pm_status pm_engine_create(int32_t device, pm_engine** out);
+/* Drop the caller's handle.  Problems, batches and servers created on the
+ * engine each keep it alive, so they may be destroyed after it (in any
+ * order, as a garbage collector finalises); the stream and the cached device
+ * blocks are released with the last of them. */
 void pm_engine_destroy(pm_engine* engine);
 /* Turn on per-run counting of NCC evaluations and window samples
  * (calibration runs only; off by default). */

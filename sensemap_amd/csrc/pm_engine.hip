@@ -230,11 +230,21 @@ PM_API pm_status pm_engine_create(int32_t device, pm_engine** out) {
   return PM_OK;
 }
 
-PM_API void pm_engine_destroy(pm_engine* e) {
-  if (!e) return;
+namespace pmh {
+
+void engine_retain(pm_engine* e) { e->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void engine_release(pm_engine* e) {
+  if (e->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
+}
+
+}  // namespace pmh
+
+PM_API void pm_engine_destroy(pm_engine* e) {
+  if (e) pmh::engine_release(e);
 }
 
 PM_API pm_status pm_engine_set_counting(pm_engine* e, int32_t enable) {
@@ -469,6 +479,7 @@ pm_status create_problem(pm_engine* e, const pm_options* o, const pm_view_desc* 
     }
   } guard{p, e, s};
   p->engine = e;
+  pmh::engine_retain(e);
   p->opt = *o;
   p->W = v->ref.width;
   p->H = v->ref.height;
@@ -704,7 +715,9 @@ std::vector<std::vector<PhaseRec>> view_ticks(const pm_problem& p) {
 
 pm_problem::~pm_problem() {
   delete schedule;
+  if (!engine) return;
   for (const auto& a : allocs) engine->cache.put(a.first, a.second);
+  pmh::engine_release(engine);
 }
 
 PM_API void pm_problem_destroy(pm_problem* p) {
@@ -1126,6 +1139,7 @@ PM_API pm_status pm_engine_load_tuning(pm_engine* e, const char* path) {
 // batch
 
 struct pm_batch {
+  pmh::EngineRef engine_ref;  // first member: released after the others
   pm_engine* engine = nullptr;
   std::vector<pm_problem*> views;
   pm_schedule schedule;
@@ -1148,6 +1162,7 @@ PM_API pm_status pm_batch_create(pm_engine* e, int32_t num_views, const pm_optio
   *out = nullptr;
   pm_batch* b = new (std::nothrow) pm_batch();
   if (!b) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  b->engine_ref.hold(e);
   b->engine = e;
   for (int32_t i = 0; i < num_views; ++i) {
     pm_problem* p = nullptr;

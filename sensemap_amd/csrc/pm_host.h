@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -108,10 +109,33 @@ struct DevCache {
   size_t bytes_ = 0;
 };
 
+// An engine is reference counted: the caller's handle holds one reference
+// and every problem, batch and server created on it holds one more, so
+// pm_engine_destroy may come before the objects made from it (a garbage
+// collector finalises in any order) and the stream and cached blocks they
+// still use stay valid until the last of them is destroyed.
+void engine_retain(pm_engine* e);
+void engine_release(pm_engine* e);
+
+// An owner's reference on its engine.  Declared as the owner's first member,
+// it is released after every other member (members are destroyed in reverse
+// order of declaration), so their destructors may still use the engine.
+struct EngineRef {
+  pm_engine* e = nullptr;
+  void hold(pm_engine* engine) {
+    engine_retain(engine);
+    e = engine;
+  }
+  ~EngineRef() {
+    if (e) engine_release(e);
+  }
+};
+
 }  // namespace pmh
 
 struct pm_engine {
   int device = 0;
+  std::atomic<int> refs{1};
   pmh::DevCache cache;
   hipStream_t stream = nullptr;
   int counting = 0;

@@ -65,6 +65,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }  // namespace
 
 struct pm_server {
+  pmh::EngineRef engine_ref;  // first member: released after the others
   pm_engine* engine = nullptr;
   pm_server_config cfg{};
   hipStream_t compute = nullptr, upload = nullptr, copy = nullptr;
@@ -541,6 +542,7 @@ PM_API pm_status pm_server_create(pm_engine* e, const pm_server_config* cfg, pm_
     return fail(PM_ERR_OVER_BUDGET, "budget below the workspace reserve (" + std::to_string(ws_bytes) + " bytes)");
   std::unique_ptr<pm_server> sv(new (std::nothrow) pm_server());
   if (!sv) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
+  sv->engine_ref.hold(e);
   sv->engine = e;
   sv->cfg = *cfg;
   PM_HIP(hipStreamCreateWithFlags(&sv->compute, hipStreamNonBlocking));

@@ -1,0 +1,68 @@
+"""Engine lifetime: problems, batches and servers keep their engine alive, so
+the engine's handle may be dropped first, in any order, as a garbage
+collector finalises (Python finalises the objects held in a module's
+globals at exit in no fixed order)."""
+import os
+import subprocess
+import sys
+import textwrap
+
+import numpy as np
+import pytest
+
+from sensemap_amd import patch_match as pmm
+from tests import helpers
+from tests.golden import make_golden as mg
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def fixture():
+    return np.load(os.path.join(os.path.dirname(mg.__file__), "scene_planes_n2.npz"))
+
+
+def maps(e):
+    return {"depth": e.GetDepthMap(), "normal": e.GetNormalMap(), "conf": e.GetConfMap(), "cost": e.GetCostMap(),
+            "view_sel": e.GetViewSelMap()}
+
+
+def test_problem_runs_after_its_engine_is_closed():
+    f = fixture()
+    o = mg.options(1)
+    eng = pmm.Engine(0)
+    e = pmm.PatchMatchACMMCuda(o, mg.problem_from(f), eng)
+    print("[lifetime] run, close the engine, run again", flush=True)
+    e.Run()
+    first = maps(e)
+    eng.close()
+    e.Run()
+    diff = helpers.maps_equal(maps(e), first)
+    e.close()
+    assert all(v == 0 for v in diff.values()), diff
+    diff = helpers.maps_equal(first, {k: f[f"photo1_{k}"] for k in mg.MAPS})
+    assert all(v == 0 for v in diff.values()), diff
+
+
+def test_module_globals_finalised_at_exit():
+    # engine and problems held only in the script's globals: at exit the
+    # engine's finaliser may run before the problems'
+    script = textwrap.dedent("""
+        import numpy as np
+        from sensemap_amd import patch_match as pmm
+        from tests.golden import make_golden as mg
+        f = np.load("tests/golden/scene_planes_n2.npz")
+        eng = pmm.Engine(0)
+        probs = [pmm.PatchMatchACMMCuda(mg.options(1), mg.problem_from(f), eng) for _ in range(3)]
+        def step():
+            for p in probs:
+                p.Run()
+        step()
+        print("ran", flush=True)
+    """)
+    print("[lifetime] child process with the engine in its globals", flush=True)
+    r = subprocess.run([sys.executable, "-c", script], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "ran" in r.stdout
