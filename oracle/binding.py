@@ -33,6 +33,14 @@ def load():
         lib.pmo_run_view.restype = C.c_int
         lib.pmo_run_view.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(abi.pm_outputs), C.c_int, C.c_int,
                                      C.c_int, P(pmo_counters)]
+        lib.pmo_run_phases.restype = C.c_int
+        lib.pmo_run_phases.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(abi.pm_outputs), P(C.c_int32),
+                                       C.c_int, C.c_int]
+        lib.pmo_schedule.restype = C.c_int
+        lib.pmo_schedule.argtypes = [P(abi.pm_options), P(C.c_int32), C.c_int]
+        lib.pmo_geom_cost_maps.restype = C.c_int
+        lib.pmo_geom_cost_maps.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(C.c_float), C.c_int,
+                                           P(C.c_float)]
         lib.pmo_pack_poses.restype = C.c_int
         lib.pmo_pack_poses.argtypes = [P(abi.pm_view_desc), P(C.c_float), P(C.c_float)]
         lib.pmo_ref_moments.restype = C.c_int
@@ -94,11 +102,51 @@ def load():
     return _lib
 
 
-def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):
-    """Run the transcription on a sensemap_amd.patch_match.ViewDesc; returns a
-    dict of depth, normal, conf, cost, view_sel (and counters if count; and
-    curvature, 2 x H x W, when the options ask for the curvature map)."""
+PHASE_INIT, PHASE_SWEEP, PHASE_LOCAL = 0, 1, 2
+
+
+def phase(kind, it=0, color=0):
+    """A phase of Run(): kind (PHASE_*), iteration, colour (0 = black = (row + col) even)."""
+    return (kind << 16) | (it << 1) | color
+
+
+def schedule(options):
+    """The phase list the transcription's Run() follows (before the steps after the iterations)."""
+    opt = options.to_c()
+    buf = (C.c_int32 * (1 + 4 * max(opt.num_iterations, 0)))()
+    n = load().pmo_schedule(C.byref(opt), buf, len(buf))
+    if n < 0:
+        raise RuntimeError("pmo_schedule failed")
+    return [int(buf[k]) for k in range(n)]
+
+
+def run_phases(options, view, phases, threads=0):
+    """Run exactly the given phases (see phase()) on a ViewDesc, from the
+    state InitWorkspace sets up; no steps after them.  Same dict as run_view."""
     lib = load()
+    out, o = _outputs(view, options, curvature=False)
+    ph = np.ascontiguousarray(phases, np.int32)
+    copt = options.to_c() if hasattr(options, "to_c") else options
+    if lib.pmo_run_phases(C.byref(copt), C.byref(view.desc), C.byref(o), ph.ctypes.data_as(C.POINTER(C.c_int32)),
+                          len(ph), int(threads)) != 0:
+        raise RuntimeError("pmo_run_phases failed")
+    return out
+
+
+def geom_cost_maps(options, view, depth, threads=0):
+    """Geometric term (reference K) of every pixel's depth against every
+    source: N x H x W float32, NaN where depth <= 0."""
+    lib = load()
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.empty((view.num_src, view.height, view.width), np.float32)
+    opt = options.to_c()
+    if lib.pmo_geom_cost_maps(C.byref(opt), C.byref(view.desc), d.ctypes.data_as(C.POINTER(C.c_float)), int(threads),
+                              out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+        raise RuntimeError("pmo_geom_cost_maps failed")
+    return out
+
+
+def _outputs(view, options, curvature=True):
     W, H, N = view.width, view.height, view.num_src
     out = {
         "depth": np.zeros((H, W), np.float32),
@@ -114,9 +162,19 @@ def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):
     o.cost = out["cost"].ctypes.data_as(C.POINTER(C.c_float))
     o.view_sel = out["view_sel"].ctypes.data_as(C.POINTER(C.c_uint32))
     copt = options.to_c() if hasattr(options, "to_c") else options
-    if copt.geom_consistency and copt.est_curvature:
+    if curvature and copt.geom_consistency and copt.est_curvature:
         out["curvature"] = np.zeros((2, H, W), np.uint16)
         o.curvature = out["curvature"].ctypes.data_as(C.POINTER(C.c_uint16))
+    return out, o
+
+
+def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):
+    """Run the transcription on a sensemap_amd.patch_match.ViewDesc; returns a
+    dict of depth, normal, conf, cost, view_sel (and counters if count; and
+    curvature, 2 x H x W, when the options ask for the curvature map)."""
+    lib = load()
+    out, o = _outputs(view, options)
+    copt = options.to_c() if hasattr(options, "to_c") else options
     cnt = pmo_counters()
     rc = lib.pmo_run_view(C.byref(copt), C.byref(view.desc), C.byref(o), int(row_begin), int(row_end), int(threads),
                           C.byref(cnt) if count else None)

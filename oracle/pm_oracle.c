@@ -1781,8 +1781,54 @@ int pmo_mask_depth(const uint8_t* mask, int W, int H, float* depth) {
 
 /* ------------------------------------------------------------------------ */
 /* Run(): patch_match_cuda.cu:748-1057 */
-int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, int row_begin,
-                 int row_end, int num_threads, pmo_counters* counters) {
+
+/* One phase of Run() over rows [r0, r1) (the init over [i0, i1)).  The
+ * threshold and the random-stream pass follow the phase's iteration. */
+static void run_phase(pass_ctx* x, int32_t phase, int r0, int r1, int i0, int i1) {
+  const int kind = PMO_PHASE_KIND(phase), iter = PMO_PHASE_ITER(phase), color = PMO_PHASE_COLOR(phase);
+  const uint32_t goff = x->opt->geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
+  if (kind == PMO_PHASE_INIT) {
+    x->th_mc = x->opt->init_ncc_matching_cost;
+    x->pass = PM_PASS_INIT + goff;
+    run_all(x, init_pixel, i0, i1);
+  } else if (kind == PMO_PHASE_SWEEP) {
+    x->th_mc = pmo_th_mc(x->opt, iter);
+    x->pass = PM_PASS_SWEEP(iter, color) + goff;
+    run_color(x, sweep_pixel, color, r0, r1);
+  } else {
+    x->th_mc = pmo_th_mc(x->opt, iter);
+    x->pass = PM_PASS_LOCAL(iter, color) + goff;
+    run_color(x, local_opt_pixel, color, r0, r1);
+  }
+}
+
+/* Run()'s phase list (patch_match_cuda.cu:748-880): InitWorkspace +
+ * InitConfMapKernel, then per iteration BlackPixelUpdateOpt,
+ * RedPixelUpdateOpt and, from the fourth iteration (iter > 2) with
+ * local_optimization, BlackPixelLocalOptimization and
+ * RedPixelLocalOptimization.  Black pixels are those with (row + col) even:
+ * BlackPixelUpdateOpt (kernel_functions.cu:2657-2662) maps thread x of a
+ * block of even width (THREADS_PER_BLOCK = 32) to row 2y when x is even and
+ * 2y + 1 when x is odd; black is colour 0 here.  Returns the count. */
+static int run_schedule(const pm_options* opt, int32_t* phases) {
+  int n = 0;
+  phases[n++] = PMO_PHASE(PMO_PHASE_INIT, 0, 0);
+  for (int iter = 0; iter < opt->num_iterations; ++iter) {
+    phases[n++] = PMO_PHASE(PMO_PHASE_SWEEP, iter, 0);
+    phases[n++] = PMO_PHASE(PMO_PHASE_SWEEP, iter, 1);
+    if (opt->local_optimization && iter > 2) {
+      phases[n++] = PMO_PHASE(PMO_PHASE_LOCAL, iter, 0);
+      phases[n++] = PMO_PHASE(PMO_PHASE_LOCAL, iter, 1);
+    }
+  }
+  return n;
+}
+
+/* The body shared by pmo_run_view and pmo_run_phases: state set up as
+ * InitWorkspace does, the given phases, then (post != 0) the steps after the
+ * iterations in Run()'s order. */
+static int run_core(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, const int32_t* phases,
+                    int num_phases, int post, int row_begin, int row_end, int num_threads, pmo_counters* counters) {
 #ifdef _OPENMP
   if (num_threads > 0) omp_set_num_threads(num_threads);
 #else
@@ -1792,6 +1838,10 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   uint8_t* rc;
   uint8_t* sc[PM_MAX_NUM_SRC];
   if (opt->thk < 1 || opt->thk > 4) return -1;
+  for (int k = 0; k < num_phases; ++k) {
+    const int kind = PMO_PHASE_KIND(phases[k]);
+    if (kind < PMO_PHASE_INIT || kind > PMO_PHASE_LOCAL || PMO_PHASE_ITER(phases[k]) >= 8192) return -1;
+  }
   if (prepare_view(opt, view, &v, &rc, sc) != 0) return -1;
   if (opt->geom_consistency && !view->src_depth_maps) {
     free_view(&v, rc, sc);
@@ -1821,44 +1871,32 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   x.v = &v;
   x.st = &st;
   x.th_mc = opt->init_ncc_matching_cost;
+  x.pass = 0;
   x.cnt = counters ? &local : NULL;
-  const uint32_t goff = opt->geom_consistency ? PM_PASS_GEOM_OFFSET : 0u;
   /* A band run initialises the band plus a margin that covers every
    * propagation offset (21 rows) and the window radius. */
   const int margin = 21 + opt->window_radius;
   const int i0 = (r0 > 0 || r1 < v.H) ? (r0 - margin < 0 ? 0 : r0 - margin) : 0;
   const int i1 = (r0 > 0 || r1 < v.H) ? (r1 + margin > v.H ? v.H : r1 + margin) : v.H;
-  x.pass = PM_PASS_INIT + goff;
-  run_all(&x, init_pixel, i0, i1);
-  for (int iter = 0; iter < opt->num_iterations; ++iter) {
-    x.th_mc = pmo_th_mc(opt, iter);
-    for (int color = 0; color < 2; ++color) {
-      x.pass = PM_PASS_SWEEP(iter, color) + goff;
-      run_color(&x, sweep_pixel, color, r0, r1);
-    }
-    if (opt->local_optimization && iter > 2) {
-      for (int color = 0; color < 2; ++color) {
-        x.pass = PM_PASS_LOCAL(iter, color) + goff;
-        run_color(&x, local_opt_pixel, color, r0, r1);
-      }
-    }
-  }
-  if (opt->geom_consistency && opt->est_curvature && out->curvature)
-    pmo_curvature_map(v.ref, v.W, v.H, st.normal, out->curvature);
-  if (opt->median_filter) pmo_median_filter(st.depth, st.normal, v.W, v.H, opt->depth_diff_threshold);
-  if (opt->filter && opt->conf_filter) {
-    for (int color = 0; color < 2; ++color) run_color(&x, filter_pixel, color, r0, r1);
-  }
+  for (int k = 0; k < num_phases; ++k) run_phase(&x, phases[k], r0, r1, i0, i1);
   int status = 0;
-  if (view->ref_mask.data) {
-    uint8_t* m = (uint8_t*)malloc(plane);
-    if (m) {
-      for (int r = 0; r < v.H; ++r)
-        memcpy(m + (size_t)r * v.W, view->ref_mask.data + (size_t)r * view->ref_mask.stride, v.W);
-      pmo_mask_depth(m, v.W, v.H, st.depth);
-      free(m);
-    } else {
-      status = -1;
+  if (post) {
+    if (opt->geom_consistency && opt->est_curvature && out->curvature)
+      pmo_curvature_map(v.ref, v.W, v.H, st.normal, out->curvature);
+    if (opt->median_filter) pmo_median_filter(st.depth, st.normal, v.W, v.H, opt->depth_diff_threshold);
+    if (opt->filter && opt->conf_filter) {
+      for (int color = 0; color < 2; ++color) run_color(&x, filter_pixel, color, r0, r1);
+    }
+    if (view->ref_mask.data) {
+      uint8_t* m = (uint8_t*)malloc(plane);
+      if (m) {
+        for (int r = 0; r < v.H; ++r)
+          memcpy(m + (size_t)r * v.W, view->ref_mask.data + (size_t)r * view->ref_mask.stride, v.W);
+        pmo_mask_depth(m, v.W, v.H, st.depth);
+        free(m);
+      } else {
+        status = -1;
+      }
     }
   }
   if (counters) *counters = local;
@@ -1874,4 +1912,54 @@ int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outpu
   free(st.view_sel);
   free_view(&v, rc, sc);
   return status;
+}
+
+int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, int row_begin,
+                 int row_end, int num_threads, pmo_counters* counters) {
+  if (opt->num_iterations < 0 || opt->num_iterations > 4096) return -1;
+  int32_t* phases = (int32_t*)malloc(sizeof(int32_t) * (size_t)(1 + 4 * opt->num_iterations));
+  if (!phases) return -1;
+  const int n = run_schedule(opt, phases);
+  const int rc = run_core(opt, view, out, phases, n, 1, row_begin, row_end, num_threads, counters);
+  free(phases);
+  return rc;
+}
+
+int pmo_run_phases(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, const int32_t* phases,
+                   int num_phases, int num_threads) {
+  if (num_phases < 0 || (num_phases > 0 && !phases)) return -1;
+  return run_core(opt, view, out, phases, num_phases, 0, 0, 0, num_threads, NULL);
+}
+
+int pmo_schedule(const pm_options* opt, int32_t* phases, int capacity) {
+  if (opt->num_iterations < 0 || capacity < 1 + 4 * opt->num_iterations) return -1;
+  return run_schedule(opt, phases);
+}
+
+/* Geometric term of every pixel's depth against every source with the
+ * reference K (N planar W*H, NaN where depth <= 0). */
+int pmo_geom_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, int num_threads,
+                       float* out) {
+#ifdef _OPENMP
+  if (num_threads > 0) omp_set_num_threads(num_threads);
+#else
+  (void)num_threads;
+#endif
+  if (!view->src_depth_maps) return -1;
+  pmo_view v;
+  uint8_t* rc;
+  uint8_t* sc[PM_MAX_NUM_SRC];
+  if (prepare_view(opt, view, &v, &rc, sc) != 0) return -1;
+  const int W = v.W, N = v.N;
+  const long plane = (long)W * v.H;
+  long p;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (p = 0; p < plane; ++p) {
+    const int row = (int)(p / W), col = (int)(p % W);
+    const float np[3] = {v.refInvK[0] * (float)col + v.refInvK[1], v.refInvK[2] * (float)row + v.refInvK[3], 1.0f};
+    for (int i = 0; i < N; ++i)
+      out[i * plane + p] = depth[p] > 0.0f ? geom_cost(opt, &v, row, col, depth[p], np, i, v.refK, NULL) : NAN;
+  }
+  free_view(&v, rc, sc);
+  return 0;
 }

@@ -50,12 +50,16 @@
  * FilterDepthMap in a run.
  * LocalOptimization's cost refresh for the sources it did not select is
  * pinned in photometric runs: every stored cost equals a fresh FP32 NCC at
- * the final maps (a transcription without the refresh fails that).
- * Parity unpinned: that refresh's geometric term, which reproduces the
- * reference passing ref_invK as K (kernel_functions.cu:2924), and the order
- * of the passes, are checked only end to end (depth against rendered ground
- * truth), by the device's bitwise agreement and by the launch counts of a
- * run.
+ * the final maps (a transcription without the refresh fails that).  Its
+ * geometric term, which reproduces the reference passing ref_invK as K
+ * (kernel_functions.cu:2924), is pinned by the closed form that K gives:
+ * the back-projection lands near the image origin, so away from it the
+ * term saturates at geom_consistency_max_cost and every stored cost is
+ * either NCC + lambda * (true-K term) or NCC + lambda * max_cost.  The
+ * order of the phases (pmo_schedule) is pinned against the reference's
+ * order written down in the tests, composed phase by phase with
+ * pmo_run_phases; red-first, early or reordered local-optimisation
+ * schedules give other maps there.
  */
 #ifndef SENSEMAP_PM_ORACLE_H_
 #define SENSEMAP_PM_ORACLE_H_
@@ -111,6 +115,28 @@ typedef struct pmo_counters {
  * Returns 0 on success. */
 int pmo_run_view(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out,
                  int row_begin, int row_end, int num_threads, pmo_counters* counters);
+
+/* Phases of Run(), for checking its order: kind (init, sweep, local
+ * optimisation), iteration and colour (0 = black = (row + col) even). */
+#define PMO_PHASE_INIT 0
+#define PMO_PHASE_SWEEP 1
+#define PMO_PHASE_LOCAL 2
+#define PMO_PHASE(kind, iter, color) ((int32_t)(((kind) << 16) | ((iter) << 1) | (color)))
+#define PMO_PHASE_KIND(p) ((int)((p) >> 16))
+#define PMO_PHASE_ITER(p) ((int)(((p) & 0xffff) >> 1))
+#define PMO_PHASE_COLOR(p) ((int)((p) & 1))
+/* The phase list pmo_run_view runs (before the steps after the
+ * iterations); returns its length, or -1 when capacity is short. */
+int pmo_schedule(const pm_options* opt, int32_t* phases, int capacity);
+/* Runs exactly the given phases on the whole view, from the state
+ * InitWorkspace sets up (zeros, or the given init maps), with no steps after
+ * them.  Returns 0 on success. */
+int pmo_run_phases(const pm_options* opt, const pm_view_desc* view, const pm_outputs* out, const int32_t* phases,
+                   int num_phases, int num_threads);
+/* Geometric term (reference K) of every pixel's depth against every source:
+ * N planar W*H, NaN where depth <= 0. */
+int pmo_geom_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* depth, int num_threads,
+                       float* out);
 
 /* Pose packing exactly as the engine does it (see pm_problem_get_poses). */
 int pmo_pack_poses(const pm_view_desc* view, float* poses, float ref_k8[8]);
