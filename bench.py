@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--storage", choices=["u8", "f16"], default="f16",
                     help="source image storage in HBM (f16 holds the 8-bit values exactly: same maps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-forms", action="store_true",
+                    help="view workload: leave out the batched and served forms' summaries")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration")
     ap.add_argument("--only-photometric", action="store_true",
                     help="profiling mode: the step is the photometric pass alone, no source passes")
@@ -119,7 +121,7 @@ def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
     prob = scene_mod.to_problem(sc)
     opts = c2_options(pmm, False, 0)
     view = pmm.ViewDesc(prob, opts)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    threads, cores = cpu_threads()
     W, H = view.width, view.height
     mid = H // 2
     # size the band for ~target_s: the cost of one iteration over 2 rows is
@@ -149,6 +151,7 @@ def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
     binding.run_view(one, view, mid, mid + 2, threads)
     dn = time.perf_counter() - t
     return {"value": round(mpix_iter / dt, 4), "unit": "Mpixel·iter/s", "cores": threads, "kind": "port",
+            "host_cpus": cores,
             "sample": f"C/OpenMP transcription, photometric pass of the 1920x1080 view, rows [{r0},{r1}) "
                       f"x {W} px, {opts.num_iterations} iterations, 10 sources, 15x15 step 1; {dt:.1f} s",
             "cpu_model": cpu_model(),
@@ -156,6 +159,36 @@ def cpu_baseline(pmm, scene_mod, sc, target_s: float) -> dict:
                               "same_sample_all_threads": round(2 * W / 1e6 / dn, 5),
                               "sample": f"rows [{mid},{mid + 2}) x {W} px, 1 iteration (with the init); "
                                         f"{d1:.1f} s on 1 thread"}}
+
+
+def cpu_threads():
+    """(threads the CPU baseline uses, what the host shows).  The threads are
+    the CPUs this process may run on (its affinity mask), capped by
+    OMP_NUM_THREADS where set: a GPU's share of a shared host is given that
+    way, and the rest of the machine belongs to other GPUs' jobs."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(allowed, env) if env > 0 else allowed
+    phys = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":", 1)[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":", 1)[1].strip()
+                elif not line.strip():
+                    if pid is not None and cid is not None:
+                        phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    return threads, {"affinity": allowed, "logical": os.cpu_count(), "physical_cores": len(phys) or None,
+                     "omp_num_threads": env or None}
 
 
 def cpu_model() -> str:
@@ -410,7 +443,7 @@ def run_plane_view(pmm, scene_mod, steps: int, with_cpu: bool) -> dict:
            "median_rel_depth_error": float(np.median(rel_depth_errors(v.GetDepthMap(), sc.ref.depth)))}
     if with_cpu:
         from oracle import binding
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()[0]
         t = time.perf_counter()
         cpu = binding.run_view(opts, v.view, 0, 0, threads)
         cs = time.perf_counter() - t
@@ -482,7 +515,7 @@ def run_filters_workload(args) -> dict:
     from sensemap_amd import patch_match as pmm
     from sensemap_amd.cross_filter import CrossImage, cross_filter
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()[0]
     sc = scene.heightfield_scene(seed=2)
     views = [sc.ref] + sc.srcs
     n = len(views)
@@ -701,7 +734,8 @@ def main() -> int:
     mpix_iter = W * H * iters / 1e6
     value = mpix_iter / (ms_per_step / 1000.0)
     final = photo if args.only_photometric else geo
-    err = np.abs(final.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
+    step_depth = final.GetDepthMap()
+    err = np.abs(step_depth - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
     err_photo = np.abs(photo.GetDepthMap() - sc.ref.depth)[sc.ref.depth > 0] / sc.ref.depth[sc.ref.depth > 0]
     avg_launch_ms = sweep_ms / (args.steps * launches_per_step)
     achieved = (flops_sweep / launches_per_step) / (avg_launch_ms / 1000.0) / 1e12
@@ -762,17 +796,23 @@ def main() -> int:
     # geometric passes (11 views, 8 photometric iterations each, plus 2)
     full = None
     if not args.only_photometric:
-        t = time.perf_counter()
-        for p in src_photo:
-            p.Run()
-        for i, p in enumerate(src_photo):
-            geo.SetSrcDepthFrom(i, p)
-        step()
-        full_s = time.perf_counter() - t
+        full_runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            for p in src_photo:
+                p.Run()
+            for i, p in enumerate(src_photo):
+                geo.SetSrcDepthFrom(i, p)
+            step()
+            full_runs.append(time.perf_counter() - t)
+        full_s = float(np.median(full_runs))
         full_mpix = (n_views * photo_opts.num_iterations + 2) * W * H / 1e6
         full = {"workload": "10 source photometric passes + the reference's photometric and geometric passes",
                 "views": n_views, "mpixel_iter": round(full_mpix, 2), "seconds": round(full_s, 3),
-                "mpixel_iter_per_s": round(full_mpix / full_s, 3), "views_per_s": round(n_views / full_s, 4)}
+                "runs_s": [round(x, 3) for x in full_runs],
+                "mpixel_iter_per_s": round(full_mpix / full_s, 3), "views_per_s": round(n_views / full_s, 4),
+                "depth_equal_to_step": bool(np.array_equal(geo.GetDepthMap().view(np.uint32),
+                                                           step_depth.view(np.uint32)))}
     # the reference's flow for the same step: each pass's problem created from
     # host buffers (images; for the geometric pass the sources' depth maps and
     # the photometric maps as its initial state), run, maps read back
@@ -848,6 +888,23 @@ def main() -> int:
         out["plane_view"] = run_plane_view(pmm, scene, args.steps, not args.no_cpu_baseline)
         progress("source selection")
         out["source_selection"] = run_source_selection(scene, args.steps, not args.no_cpu_baseline)
+    if not args.only_photometric and not args.skip_forms:
+        # the batched (64 orbit views) and served (Poisson arrivals) forms
+        # against the plain path, at reduced repetitions so the default run
+        # stays within minutes; --workload batch / serve run them in full
+        progress("batched form")
+        b = run_batch_workload(argparse.Namespace(views=64, steps=1))
+        out["batch"] = {k: b[k] for k in ("config", "batched_views_per_s", "plain_views_per_s", "speedup_vs_plain",
+                                          "batched_mpixel_iter_per_s", "plain_mpixel_iter_per_s",
+                                          "median_rel_depth_error_plain", "median_rel_depth_error_batched",
+                                          "median_error_ratio", "depth_maps_identical", "plain_view_p50_ms",
+                                          "plain_view_p99_ms", "batch_warmup_s")}
+        progress("serving loop")
+        s = run_serve_workload(argparse.Namespace(rates="20,200", jobs=16, budget_gb=200.0, max_active=4))
+        out["serve"] = {k: s[k] for k in ("config", "per_rate", "plain_views_per_s", "plain_job_p50_ms",
+                                          "plain_job_p99_ms", "speedup_vs_plain", "median_rel_depth_error_plain",
+                                          "median_rel_depth_error_served", "median_error_ratio", "cache_hits",
+                                          "cache_misses", "cache_evictions")}
     print(json.dumps(out))
     return 0
 

@@ -84,6 +84,12 @@ typedef struct pm_options {
   float depth_diff_threshold;         /* 0.02 */
   float filter_geom_consistency_max_cost;  /* 1.0: RunFilter's per-source threshold (patch_match_options.h:139) */
   int32_t filter_min_num_consistent;  /* 1: RunFilter keeps a depth with this many consistent sources (:135) */
+  /* Priors (patch_match_options.h; read at patch_match_cuda.cu:377, 603, 943-947): 0 only.  has_prior_depth
+   * belongs to the RGB-D path; propagate_depth makes the reference's sweep read prior normal and weight maps
+   * it allocates as 0x0 (patch_match_cuda.cu:589-591, kernel_functions.cu:2191-2199), so it defines no result.
+   * pm_options_check rejects either with PM_ERR_INVALID_ARG rather than ignore it. */
+  int32_t propagate_depth;            /* 0 */
+  int32_t has_prior_depth;            /* 0 */
   /* ---- extensions (no counterpart in the reference) ---- */
   int32_t init_samples;               /* 1 = reference semantics; K > 1 keeps the best of K random inits */
   int32_t compat_lo_geo_bug;          /* 1 = reproduce kernel_functions.cu:2924 (ref_invK passed as K) */
@@ -111,8 +117,9 @@ typedef struct pm_image {
   int32_t height;
   int32_t stride;  /* bytes between rows, >= width */
   /* > 0: a serving loop may keep this image on the device under this key
-   * (images given the same key must be identical); <= 0: never cached.
-   * Ignored outside pm_server_*. */
+   * (images given the same key must be identical; a key the server holds
+   * for an image of another size fails the job with PM_ERR_INVALID_ARG);
+   * <= 0: never cached.  Ignored outside pm_server_*. */
   int32_t cache_id;
 } pm_image;
 
@@ -444,6 +451,43 @@ pm_status pm_cross_filter(pm_engine* engine, const pm_cross_options* options, in
  * every view has read its sources (two such views may not share a problem). */
 pm_status pm_cross_filter_problems(pm_engine* engine, const pm_cross_options* options, int32_t num_problems,
                                    pm_problem* const* problems, int32_t num_views, const pm_cross_view* views);
+
+/* ---- deduplication after the cross filter ----
+ * CrossFilterCuda::DedupRun (patch_match_cuda.cu:1561-1612) with FillGpuMat
+ * and DeduplicFilterKernel (kernel_functions.cu:3780-3916, 3930-3941), and
+ * the getters GetDeduplicDepthMap, GetDeltDepthMap and GetVisibilyMap
+ * (patch_match_cuda.cu:1629-1703).  Every image starts with a visibility
+ * word of 0x80000000 per pixel.  The views run in the given order, as the
+ * reference's problems do: a view's pixel keeps its word (and gains bit
+ * 30 - k for each consistent source k) when at least
+ * min(filter_min_num_consistent, num_src) sources reproject it within
+ * filter_geom_consistency_max_cost pixels and depth_diff_threshold, and no
+ * consistent source that is itself a filtered reference (ref_flag) still
+ * shows that point as visible; otherwise its word becomes 0.  So a point
+ * seen by several filtered views is kept by the last view that sees it.
+ * max_normal_error is not used here. */
+typedef struct pm_dedup_image {
+  pm_camera cam;
+  int32_t width;
+  int32_t height;
+  const float* depth;    /* width*height: the map DedupRun reads (the cross filter's refined map for a
+                            filtered reference, the image's own map otherwise; ResetDepthMaps, :1260-1295) */
+  int32_t ref_flag;      /* ref_flags_ (CrossFilterCuda ctor): the image is a reference of the cross filter */
+  int32_t reserved0;
+  uint32_t* visibility;  /* out, may be NULL: width*height visibility words (GetVisibilyMap) */
+  float* dedup_depth;    /* out, may be NULL: depth where bit 31 of the word is set, else 0 (GetDeduplicDepthMap) */
+} pm_dedup_image;
+
+typedef struct pm_dedup_view {
+  int32_t ref;           /* index into the images */
+  int32_t num_src;       /* 0..31; a view with no source is skipped (:1586-1588) */
+  const int32_t* src;    /* indices into the images, none equal to ref */
+  float* delt;           /* out, may be NULL: the ref's width*height GetDeltDepthMap of this view (0 where the
+                            pixel is dropped) */
+} pm_dedup_view;
+
+pm_status pm_cross_dedup(pm_engine* engine, const pm_cross_options* options, int32_t num_images,
+                         const pm_dedup_image* images, int32_t num_views, const pm_dedup_view* views);
 
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,

@@ -59,6 +59,9 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_cross_dedup.restype = C.c_int
+        lib.pmo_cross_dedup.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_dedup_image), C.c_int,
+                                        P(abi.pm_dedup_view), C.c_int]
         lib.pmo_th_mc.restype = C.c_float
         lib.pmo_th_mc.argtypes = [P(abi.pm_options), C.c_int]
         lib.pmo_lo_decide.restype = C.c_int
@@ -258,6 +261,18 @@ def cross_filter(images, views, options=None, threads=0):
     if lib.pmo_cross_filter(C.byref(opt), len(images), inp.images, len(views), inp.views, int(threads)) != 0:
         raise RuntimeError("pmo_cross_filter failed")
     return inp.outputs
+
+
+def cross_dedup(images, views, options=None, threads=0):
+    """The transcription of CrossFilterCuda::DedupRun, same inputs and outputs
+    as sensemap_amd.cross_filter.dedup."""
+    from sensemap_amd.cross_filter import CrossFilterOptions, DedupInputs
+    lib = load()
+    opt = (options or CrossFilterOptions()).to_c()
+    inp = DedupInputs(images, views)
+    if lib.pmo_cross_dedup(C.byref(opt), len(images), inp.images, inp.num_views, inp.views, int(threads)) != 0:
+        raise RuntimeError("pmo_cross_dedup failed")
+    return {"visibility": inp.visibility, "dedup_depth": inp.dedup_depth, "delt": inp.delt}
 
 
 def run_filter(options, view, dedup=False, threads=0):

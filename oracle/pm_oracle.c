@@ -1598,6 +1598,169 @@ int pmo_cross_filter(const pm_cross_options* o, int num_images, const pm_cross_i
 }
 
 /* ------------------------------------------------------------------------ */
+/* CrossFilterCuda::DedupRun (patch_match_cuda.cu:1561-1612) */
+
+/* DeduplicFilterKernel (kernel_functions.cu:3780-3916) at one pixel of view
+ * cv: updates the ref's visibility word and, when kept, writes *delt. */
+static void dedup_pixel(const pm_cross_options* o, const pm_dedup_image* images, const pm_dedup_view* cv,
+                        uint32_t* const* vis, const float* poses, const float k8[8], int min_consistent, int row,
+                        int col, float* delt) {
+  const pm_dedup_image* ref = &images[cv->ref];
+  const int W = ref->width, H = ref->height;
+  const size_t p = (size_t)row * W + col;
+  uint32_t* rv = vis[cv->ref];
+  const float depth = ref->depth[p];
+  if ((double)depth < 1e-6 || col < 3 || col > W - 3 || row < 3 || row > H - 3) {
+    rv[p] = 0u;
+    return;
+  }
+  /* visbility_value >> (sizeof(uint32_t) - 1) shifts by 3 (:3803); every
+   * word the views write is 0 or has bit 31 set, so this equals a test of
+   * bit 31 on every reachable state */
+  if (!(rv[p] >> 3)) {
+    rv[p] = 0u;
+    return;
+  }
+  const float* iK = k8 + 4;
+  const float fcol = (float)col, frow = (float)row;
+  /* ComputePointAtDepth (kernel_functions.cu:308-315) */
+  const float fp[3] = {depth * (iK[0] * fcol + iK[1]), depth * (iK[2] * frow + iK[3]), depth};
+  const float sqr_thr = o->depth_diff_threshold * o->depth_diff_threshold;
+  float deltas[32];
+  int good_src[32];
+  int ng = 0, nvis = 0;
+  for (int i = 0; i < cv->num_src; ++i) {
+    const float* pose = poses + (size_t)i * PM_POSE_STRIDE;
+    const float* P = pose + 16;
+    const float* iP = pose + 28;
+    const float inv_fz = 1.0f / (P[8] * fp[0] + P[9] * fp[1] + P[10] * fp[2] + P[11]);
+    float sc = inv_fz * (P[0] * fp[0] + P[1] * fp[1] + P[2] * fp[2] + P[3]);
+    float sr = inv_fz * (P[4] * fp[0] + P[5] * fp[1] + P[6] * fp[2] + P[7]);
+    const pm_dedup_image* si = &images[cv->src[i]];
+    /* point-sampled layered texture with a zero border (InitDepthMaps,
+     * :1139-1207): texel (floor(x + 0.5), floor(y + 0.5)) */
+    const float xf = sc + 0.5f, yf = sr + 0.5f;
+    if (!(xf >= 0.0f && xf < (float)si->width && yf >= 0.0f && yf < (float)si->height)) continue;
+    const size_t q = (size_t)(int)floorf(yf) * si->width + (size_t)(int)floorf(xf);
+    const float sd = si->depth[q];
+    if (sd == 0.0f) continue;
+    sc = sc * sd;
+    sr = sr * sd;
+    const float bx = iP[0] * sc + iP[1] * sr + iP[2] * sd + iP[3];
+    const float by = iP[4] * sc + iP[5] * sr + iP[6] * sd + iP[7];
+    const float bz = iP[8] * sc + iP[9] * sr + iP[10] * sd + iP[11];
+    const float inv_bz = 1.0f / bz;
+    const float bcol = inv_bz * (k8[0] * bx + k8[1] * bz);
+    const float brow = inv_bz * (k8[2] * by + k8[3] * bz);
+    const float dc = fcol - bcol, dr = frow - brow;
+    const float reproj = sqrtf(dc * dc + dr * dr);
+    if (reproj > o->filter_geom_consistency_max_cost || (depth - bz) * (depth - bz) > depth * depth * sqr_thr)
+      continue;
+    deltas[ng] = fabsf(bz - depth);
+    good_src[ng] = i;
+    ng++;
+    /* the source's word at the sampled texel: (size_t)(x + 0.5) equals
+     * floor(x + 0.5) here, x + 0.5 being >= 0 (:3893) */
+    if ((vis[cv->src[i]][q] & 0x80000000u) && si->ref_flag) nvis++;
+  }
+  if (ng < min_consistent || nvis > 0) {
+    rv[p] = 0u;
+    return;
+  }
+  uint32_t bits = rv[p];
+  for (int k = 0; k < ng; ++k) bits |= 1u << (31 - 1 - good_src[k]); /* MAX_NUM_CROSS_SRC - 1 - idx (:3907) */
+  rv[p] = bits;
+  if (ng == 0) return; /* the reference reads an unset entry; this build leaves 0 */
+  /* BubbleSort(delt_depths, num_good_src_depth - 1) sorts all but the last
+   * entry; the median index is int((num_good_src_depth - 1) * 0.5) (:3912-3914) */
+  const int len = ng - 1;
+  for (int a = 0; a < len - 1; ++a)
+    for (int b = 0; b < len - 1 - a; ++b)
+      if (deltas[b] > deltas[b + 1]) {
+        const float t = deltas[b];
+        deltas[b] = deltas[b + 1];
+        deltas[b + 1] = t;
+      }
+  const int nth = (int)((float)(ng - 1) * 0.5f);
+  if (delt) *delt = deltas[nth] / (o->depth_diff_threshold * depth);
+}
+
+int pmo_cross_dedup(const pm_cross_options* o, int num_images, const pm_dedup_image* images, int num_views,
+                    const pm_dedup_view* views, int num_threads) {
+#ifdef _OPENMP
+  if (num_threads > 0) omp_set_num_threads(num_threads);
+#else
+  (void)num_threads;
+#endif
+  if (num_images < 1 || num_views < 0) return -1;
+  uint32_t** vis = (uint32_t**)calloc((size_t)num_images, sizeof(uint32_t*));
+  if (!vis) return -1;
+  int status = 0;
+  /* FillGpuMat (kernel_functions.cu:3930-3941) */
+  for (int i = 0; i < num_images && status == 0; ++i) {
+    const size_t n = (size_t)images[i].width * images[i].height;
+    vis[i] = (uint32_t*)malloc(n * sizeof(uint32_t) + 4);
+    if (!vis[i]) {
+      status = -1;
+      break;
+    }
+    for (size_t k = 0; k < n; ++k) vis[i][k] = 0x80000000u;
+  }
+  for (int k = 0; k < num_views && status == 0; ++k) {
+    const pm_dedup_view* cv = &views[k];
+    if (cv->ref < 0 || cv->ref >= num_images || cv->num_src < 0 || cv->num_src > 31) {
+      status = -1;
+      break;
+    }
+    const pm_dedup_image* ref = &images[cv->ref];
+    const long plane = (long)ref->width * ref->height;
+    if (cv->delt)
+      for (long p = 0; p < plane; ++p) cv->delt[p] = 0.0f; /* FillWithScalar(0.0f) (:1595) */
+    if (cv->num_src < 1) continue; /* FindSrcImageLists gave < 2 entries (:1586-1588) */
+    pm_image srcs[31];
+    pm_camera cams[31];
+    memset(srcs, 0, sizeof(srcs));
+    for (int s = 0; s < cv->num_src; ++s) {
+      if (cv->src[s] < 0 || cv->src[s] >= num_images || cv->src[s] == cv->ref) status = -1;
+      if (status) break;
+      srcs[s].width = images[cv->src[s]].width;
+      srcs[s].height = images[cv->src[s]].height;
+      cams[s] = images[cv->src[s]].cam;
+    }
+    if (status) break;
+    pm_view_desc vd;
+    memset(&vd, 0, sizeof(vd));
+    vd.num_src = cv->num_src;
+    vd.ref_cam = ref->cam;
+    vd.src = srcs;
+    vd.src_cams = cams;
+    float poses[31 * PM_POSE_STRIDE];
+    float k8[8];
+    pmo_pack_poses(&vd, poses, k8);
+    const int minc = o->filter_min_num_consistent < cv->num_src ? o->filter_min_num_consistent : cv->num_src;
+    /* a view writes only its ref's words and reads its sources' (none is the
+     * ref), so its pixels are independent */
+    long p;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (p = 0; p < plane; ++p) {
+      const int row = (int)(p / ref->width), col = (int)(p % ref->width);
+      dedup_pixel(o, images, cv, vis, poses, k8, minc, row, col, cv->delt ? &cv->delt[p] : NULL);
+    }
+  }
+  if (status == 0) {
+    for (int i = 0; i < num_images; ++i) {
+      const size_t n = (size_t)images[i].width * images[i].height;
+      if (images[i].visibility) memcpy(images[i].visibility, vis[i], n * sizeof(uint32_t));
+      if (images[i].dedup_depth) /* GetDeduplicDepthMapsLayer (kernel_functions.cu:3943-3959) */
+        for (size_t k = 0; k < n; ++k) images[i].dedup_depth[k] = (vis[i][k] & 0x80000000u) ? images[i].depth[k] : 0.0f;
+    }
+  }
+  for (int i = 0; i < num_images; ++i) free(vis[i]);
+  free(vis);
+  return status;
+}
+
+/* ------------------------------------------------------------------------ */
 /* ComputeCurvatureMap: kernel_functions.cu:1603-1651 */
 
 /* float -> unsigned short of the reference's GpuMat<unsigned short>::Set:

@@ -159,6 +159,10 @@ int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* 
 /* RunFilter on view->init_depth into depth_out (W*H). */
 int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, float* depth_out, int num_threads);
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
+/* CrossFilterCuda::DedupRun and its getters over the views in order, same
+ * arguments and outputs as the C-ABI's pm_cross_dedup. */
+int pmo_cross_dedup(const pm_cross_options* options, int num_images, const pm_dedup_image* images, int num_views,
+                    const pm_dedup_view* views, int num_threads);
 int pmo_cross_filter(const pm_cross_options* options, int num_images, const pm_cross_image* images,
                      int num_views, const pm_cross_view* views, int num_threads);
 /* Iteration iter's good-matching threshold, and LocalOptimization's choice

@@ -15,6 +15,8 @@ PM_MAX_NUM_SRC = 32
 PM_POSE_STRIDE = 44
 
 PM_OK = 0
+(PM_ERR_INVALID_ARG, PM_ERR_OUT_OF_MEMORY, PM_ERR_DEVICE, PM_ERR_TOO_MANY_SOURCES, PM_ERR_NOT_READY,
+ PM_ERR_OVER_BUDGET, PM_ERR_INTERNAL) = range(1, 8)
 STATUS_NAMES = {
     0: "PM_OK", 1: "PM_ERR_INVALID_ARG", 2: "PM_ERR_OUT_OF_MEMORY", 3: "PM_ERR_DEVICE",
     4: "PM_ERR_TOO_MANY_SOURCES", 5: "PM_ERR_NOT_READY", 6: "PM_ERR_OVER_BUDGET", 7: "PM_ERR_INTERNAL",
@@ -56,6 +58,8 @@ class pm_options(C.Structure):
         ("depth_diff_threshold", C.c_float),
         ("filter_geom_consistency_max_cost", C.c_float),
         ("filter_min_num_consistent", C.c_int32),
+        ("propagate_depth", C.c_int32),
+        ("has_prior_depth", C.c_int32),
         ("init_samples", C.c_int32),
         ("compat_lo_geo_bug", C.c_int32),
         ("image_storage", C.c_int32),
@@ -183,6 +187,17 @@ class pm_cross_view(C.Structure):
                 ("refined", C.POINTER(C.c_float))]
 
 
+class pm_dedup_image(C.Structure):
+    _fields_ = [("cam", pm_camera), ("width", C.c_int32), ("height", C.c_int32), ("depth", C.POINTER(C.c_float)),
+                ("ref_flag", C.c_int32), ("reserved0", C.c_int32), ("visibility", C.POINTER(C.c_uint32)),
+                ("dedup_depth", C.POINTER(C.c_float))]
+
+
+class pm_dedup_view(C.Structure):
+    _fields_ = [("ref", C.c_int32), ("num_src", C.c_int32), ("src", C.POINTER(C.c_int32)),
+                ("delt", C.POINTER(C.c_float))]
+
+
 class pm_model(C.Structure):
     _fields_ = [("num_images", C.c_int32), ("R", C.POINTER(C.c_float)), ("T", C.POINTER(C.c_float)),
                 ("is_rig", C.POINTER(C.c_uint8)), ("num_points", C.c_int64), ("xyz", C.POINTER(C.c_float)),
@@ -256,6 +271,8 @@ _SIGNATURES = {
                                   C.c_int32, C.POINTER(pm_cross_view)]),
     "pm_cross_filter_problems": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32,
                                            C.POINTER(C.c_void_p), C.c_int32, C.POINTER(pm_cross_view)]),
+    "pm_cross_dedup": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_dedup_image),
+                                 C.c_int32, C.POINTER(pm_dedup_view)]),
     "pm_model_max_overlapping_images": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.c_int32, C.c_double,
                                                   C.POINTER(C.c_uint8), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                                   C.POINTER(pm_overlap_stats)]),

@@ -78,6 +78,70 @@ class CrossInputs:
             self.views[k].refined = out.ctypes.data_as(C.POINTER(C.c_float))
 
 
+class DedupInputs:
+    """Owns the buffers behind one deduplication call: images
+    (CrossImage, ref_flag) and views (ref, sources); outputs per image
+    (visibility words, deduplicated depth) and per view (delt map)."""
+
+    def __init__(self, images: List[Tuple[CrossImage, bool]], views: List[Tuple[int, Sequence[int]]]):
+        self._keep = []
+        n = len(images)
+        self.images = (abi.pm_dedup_image * n)()
+        self.visibility, self.dedup_depth, self.delt = [], [], []
+        for i, (im, flag) in enumerate(images):
+            d = np.ascontiguousarray(im.depth, np.float32)
+            vis = np.zeros(d.shape, np.uint32)
+            dd = np.zeros(d.shape, np.float32)
+            self._keep += [d]
+            self.visibility.append(vis)
+            self.dedup_depth.append(dd)
+            img = self.images[i]
+            img.cam = _camera(im)
+            img.height, img.width = d.shape
+            img.depth = d.ctypes.data_as(C.POINTER(C.c_float))
+            img.ref_flag = int(bool(flag))
+            img.visibility = vis.ctypes.data_as(C.POINTER(C.c_uint32))
+            img.dedup_depth = dd.ctypes.data_as(C.POINTER(C.c_float))
+        self.views = (abi.pm_dedup_view * max(len(views), 1))()
+        self.num_views = len(views)
+        for k, (ref, srcs) in enumerate(views):
+            s = (C.c_int32 * max(len(srcs), 1))(*[int(x) for x in srcs])
+            out = np.zeros(np.asarray(images[ref][0].depth).shape, np.float32)
+            self._keep.append(s)
+            self.delt.append(out)
+            self.views[k].ref = int(ref)
+            self.views[k].num_src = len(srcs)
+            self.views[k].src = C.cast(s, C.POINTER(C.c_int32))
+            self.views[k].delt = out.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _camera(im: CrossImage) -> abi.pm_camera:
+    c = abi.pm_camera()
+    for k in range(4):
+        c.K[k] = float(im.K[k])
+    R = np.asarray(im.R, np.float64).reshape(9)
+    T = np.asarray(im.T, np.float64).reshape(3)
+    for k in range(9):
+        c.R[k] = float(R[k])
+    for k in range(3):
+        c.T[k] = float(T[k])
+    return c
+
+
+def dedup(images: List[Tuple[CrossImage, bool]], views: List[Tuple[int, Sequence[int]]],
+          options: Optional[CrossFilterOptions] = None, engine=None) -> dict:
+    """CrossFilterCuda::DedupRun over views (ref index, source indices) in
+    order; images are (CrossImage, ref_flag) with the depth map DedupRun
+    reads (normals are not used).  Returns {"visibility": [per image (H, W)
+    uint32], "dedup_depth": [per image (H, W)], "delt": [per view (H, W)]}."""
+    from .patch_match import default_engine
+    eng = engine or default_engine()
+    opt = (options or CrossFilterOptions()).to_c()
+    inp = DedupInputs(images, views)
+    abi.check(eng.lib.pm_cross_dedup(eng.handle, C.byref(opt), len(images), inp.images, inp.num_views, inp.views))
+    return {"visibility": inp.visibility, "dedup_depth": inp.dedup_depth, "delt": inp.delt}
+
+
 def cross_filter(images: List[CrossImage], views: List[Tuple[int, Sequence[int]]],
                  options: Optional[CrossFilterOptions] = None, engine=None) -> List[np.ndarray]:
     """Refined depth map of every view (ref index, source indices), in one

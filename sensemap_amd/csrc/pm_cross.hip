@@ -425,3 +425,316 @@ PM_API pm_status pm_cross_filter_problems(pm_engine* e, const pm_cross_options* 
   PM_HIP(hipStreamSynchronize(st));
   return PM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// CrossFilterCuda::DedupRun (patch_match_cuda.cu:1561-1612): visibility words
+// per image, DeduplicFilterKernel (kernel_functions.cu:3780-3916) per view in
+// the given order.  Consecutive views that touch disjoint images (no view's
+// reference is another's reference or source) share one launch; that is the
+// same as running them one after another, since a view writes only its
+// reference's words and reads only its sources'.
+
+namespace {
+
+struct DedupSrc {
+  const float* depth;
+  const uint32_t* vis;
+  int32_t width, height;
+  int32_t flag, pad;
+};
+
+struct DedupViewDev {
+  int32_t W, H, N, tile_start;
+  int32_t min_consistent, pad0;
+  float K[4], iK[4];
+  const float* depth;
+  uint32_t* vis;
+  float* delt;         // may be null
+  const float* poses;  // N * PM_POSE_STRIDE
+  DedupSrc src[kCrossMaxSrc];
+};
+
+struct DedupParams {
+  float max_reproj;     // filter_geom_consistency_max_cost
+  float depth_thr;      // depth_diff_threshold
+  float sqr_depth_thr;  // depth_diff_threshold^2
+  int32_t first, count; // the group's views in the table
+};
+
+using DV = const __attribute__((address_space(4))) DedupViewDev;
+using gfloat = const __attribute__((address_space(1))) float;
+using guint = __attribute__((address_space(1))) uint32_t;
+
+__device__ void dedup_pixel(DV& v, const DedupParams& prm, int row, int col) {
+  const int p = row * v.W + col;
+  guint* rv = (guint*)v.vis;
+  __attribute__((address_space(1))) float* delt = (__attribute__((address_space(1))) float*)v.delt;
+  if (delt) delt[p] = 0.0f;
+  const float depth = ((gfloat*)v.depth)[p];
+  if ((double)depth < 1e-6 || col < 3 || col > v.W - 3 || row < 3 || row > v.H - 3) {
+    rv[p] = 0u;
+    return;
+  }
+  const uint32_t word = rv[p];
+  if (!(word >> 3)) {  // visbility_value >> (sizeof(uint32_t) - 1) (:3803)
+    rv[p] = 0u;
+    return;
+  }
+  const float fcol = (float)col, frow = (float)row;
+  const float fx = depth * (v.iK[0] * fcol + v.iK[1]);
+  const float fy = depth * (v.iK[2] * frow + v.iK[3]);
+  const float fz = depth;
+  float deltas[kCrossMaxSrc];
+  uint32_t bits = word;
+  int ng = 0, nvis = 0;
+  for (int i = 0; i < v.N; ++i) {
+    gfloat* pose = (gfloat*)v.poses + i * PM_POSE_STRIDE;
+    gfloat* P = pose + 16;
+    gfloat* iP = pose + 28;
+    const float inv_fz = 1.0f / (P[8] * fx + P[9] * fy + P[10] * fz + P[11]);
+    float sc = inv_fz * (P[0] * fx + P[1] * fy + P[2] * fz + P[3]);
+    float sr = inv_fz * (P[4] * fx + P[5] * fy + P[6] * fz + P[7]);
+    // point sampling at (sc + 0.5, sr + 0.5) with a zero border
+    const float xf = sc + 0.5f, yf = sr + 0.5f;
+    const int sw = v.src[i].width, sh = v.src[i].height;
+    if (!(xf >= 0.0f && xf < (float)sw && yf >= 0.0f && yf < (float)sh)) continue;
+    const int q = (int)floorf(yf) * sw + (int)floorf(xf);
+    const float sd = ((gfloat*)v.src[i].depth)[q];
+    if (sd == 0.0f) continue;
+    sc = sc * sd;
+    sr = sr * sd;
+    const float bx = iP[0] * sc + iP[1] * sr + iP[2] * sd + iP[3];
+    const float by = iP[4] * sc + iP[5] * sr + iP[6] * sd + iP[7];
+    const float bz = iP[8] * sc + iP[9] * sr + iP[10] * sd + iP[11];
+    const float inv_bz = 1.0f / bz;
+    const float bcol = inv_bz * (v.K[0] * bx + v.K[1] * bz);
+    const float brow = inv_bz * (v.K[2] * by + v.K[3] * bz);
+    const float dc = fcol - bcol, dr = frow - brow;
+    const float reproj = sqrtf(dc * dc + dr * dr);
+    if (reproj > prm.max_reproj || (depth - bz) * (depth - bz) > depth * depth * prm.sqr_depth_thr) continue;
+    deltas[ng++] = fabsf(bz - depth);
+    bits |= 1u << (kCrossMaxSrc - 1 - i);  // MAX_NUM_CROSS_SRC - 1 - idx (:3907)
+    // the source's word at the sampled texel ((size_t)(x + 0.5), x + 0.5 >= 0)
+    if ((((const __attribute__((address_space(1))) uint32_t*)v.src[i].vis)[q] & 0x80000000u) && v.src[i].flag) nvis++;
+  }
+  if (ng < v.min_consistent || nvis > 0) {
+    rv[p] = 0u;
+    return;
+  }
+  rv[p] = bits;
+  if (ng == 0 || !delt) return;
+  // BubbleSort(delt_depths, num_good_src_depth - 1): all but the last entry;
+  // median index int((num_good_src_depth - 1) * 0.5) (:3912-3914)
+  const int len = ng - 1;
+  for (int a = 0; a < len - 1; ++a)
+    for (int b = 0; b < len - 1 - a; ++b)
+      if (deltas[b] > deltas[b + 1]) {
+        const float t = deltas[b];
+        deltas[b] = deltas[b + 1];
+        deltas[b + 1] = t;
+      }
+  delt[p] = deltas[(ng - 1) >> 1] / (prm.depth_thr * depth);
+}
+
+__global__ __launch_bounds__(256) void k_dedup(const DedupViewDev* __restrict__ views, DedupParams prm) {
+  const int t = blockIdx.x;
+  const __attribute__((address_space(4))) DedupViewDev* tab =
+      (const __attribute__((address_space(4))) DedupViewDev*)views;
+  int lo = prm.first, hi = prm.first + prm.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].tile_start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  DV& v = tab[lo];
+  const int lt = t - v.tile_start;
+  const int tiles_x = (v.W + kCrossTile - 1) / kCrossTile;
+  const int ty = lt / tiles_x, tx = lt - ty * tiles_x;
+  const int col = tx * kCrossTile + (int)(threadIdx.x & (kCrossTile - 1));
+  const int row = ty * kCrossTile + (int)(threadIdx.x / kCrossTile);
+  if (col >= v.W || row >= v.H) return;
+  dedup_pixel(v, prm, row, col);
+}
+
+// FillGpuMat (kernel_functions.cu:3930-3941): every word 0x80000000.
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// GetDeduplicDepthMapsLayer (kernel_functions.cu:3943-3959) over all images.
+__global__ __launch_bounds__(256) void k_dedup_depth(const float* __restrict__ depth, const uint32_t* __restrict__ vis,
+                                                     float* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (vis[i] & 0x80000000u) ? depth[i] : 0.0f;
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
+
+}  // namespace
+
+PM_API pm_status pm_cross_dedup(pm_engine* e, const pm_cross_options* o, int32_t num_images,
+                                const pm_dedup_image* images, int32_t num_views, const pm_dedup_view* views) {
+  if (!e || !o || !images || num_images <= 0 || num_views < 0 || (num_views > 0 && !views))
+    return fail(PM_ERR_INVALID_ARG, "invalid dedup arguments");
+  if (!(o->filter_geom_consistency_max_cost >= 0.0f) || !(o->depth_diff_threshold >= 0.0f) ||
+      o->filter_min_num_consistent < 0)
+    return fail(PM_ERR_INVALID_ARG, "invalid dedup options");
+  for (int32_t i = 0; i < num_images; ++i) {
+    const pm_dedup_image& im = images[i];
+    if (im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > ((int64_t)1 << 28) || !im.depth ||
+        !finite_camera(im.cam))
+      return fail(PM_ERR_INVALID_ARG, "invalid dedup image " + std::to_string(i));
+  }
+  for (int32_t k = 0; k < num_views; ++k) {
+    const pm_dedup_view& v = views[k];
+    if (v.ref < 0 || v.ref >= num_images || v.num_src < 0 || (v.num_src > 0 && !v.src))
+      return fail(PM_ERR_INVALID_ARG, "invalid dedup view " + std::to_string(k));
+    if (v.num_src > kCrossMaxSrc) return fail(PM_ERR_TOO_MANY_SOURCES, "dedup views take 0..31 sources");
+    for (int32_t s = 0; s < v.num_src; ++s)
+      if (v.src[s] < 0 || v.src[s] >= num_images || v.src[s] == v.ref)
+        return fail(PM_ERR_INVALID_ARG, "dedup source index out of range or equal to the reference in view " +
+                                            std::to_string(k));
+  }
+  PM_HIP(hipSetDevice(e->device));
+  hipStream_t st = e->stream;
+
+  // every image's depth and visibility words on the device
+  std::vector<int64_t> off(num_images);
+  int64_t total = 0;
+  for (int32_t i = 0; i < num_images; ++i) {
+    off[i] = total;
+    total += (int64_t)images[i].width * images[i].height;
+  }
+  int64_t delt_total = 0;
+  std::vector<int64_t> delt_off(num_views, -1);
+  for (int32_t k = 0; k < num_views; ++k) {
+    if (!views[k].delt || views[k].num_src < 1) continue;
+    delt_off[k] = delt_total;
+    delt_total += (int64_t)images[views[k].ref].width * images[views[k].ref].height;
+  }
+  DevBuf d_depth, d_vis, d_out, d_delt, d_poses, d_tab;
+  PM_HIP(d_depth.get(e, st, (size_t)total * sizeof(float)));
+  PM_HIP(d_vis.get(e, st, (size_t)total * sizeof(uint32_t)));
+  PM_HIP(d_out.get(e, st, (size_t)total * sizeof(float)));
+  PM_HIP(d_delt.get(e, st, (size_t)std::max<int64_t>(delt_total, 1) * sizeof(float)));
+  float* depth = (float*)d_depth.p;
+  uint32_t* vis = (uint32_t*)d_vis.p;
+  for (int32_t i = 0; i < num_images; ++i)
+    PM_HIP(hipMemcpyAsync(depth + off[i], images[i].depth, (size_t)images[i].width * images[i].height * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(total)), dim3(256), 0, st, vis, total, 0x80000000u);
+  PM_HIP(hipGetLastError());
+
+  // view table and poses; groups of consecutive independent views
+  std::vector<DedupViewDev> tab;
+  std::vector<float> poses;
+  std::vector<size_t> pose_off;
+  std::vector<std::pair<int, int>> groups;  // (first, count) in tab
+  std::vector<int32_t> touched(num_images, -1);  // group that reads or writes the image
+  std::vector<uint8_t> written(num_images, 0);   // the image is a reference in the current group
+  int64_t tiles = 0;
+  for (int32_t k = 0; k < num_views; ++k) {
+    const pm_dedup_view& cv = views[k];
+    if (cv.num_src < 1) continue;  // FindSrcImageLists gave < 2 entries (:1586-1588)
+    // conflicts with the current group: its references vs this view's
+    // images, this view's reference vs every image the group touches
+    const int g = (int)groups.size() - 1;
+    bool conflict = g < 0 || touched[cv.ref] == g;
+    for (int32_t s = 0; s < cv.num_src && !conflict; ++s) conflict = touched[cv.src[s]] == g && written[cv.src[s]];
+    if (conflict) {
+      groups.push_back({(int)tab.size(), 0});
+      std::fill(written.begin(), written.end(), 0);
+      tiles = 0;
+    }
+    const int cur = (int)groups.size() - 1;
+    groups.back().second++;
+    touched[cv.ref] = cur;
+    written[cv.ref] = 1;
+    for (int32_t s = 0; s < cv.num_src; ++s) touched[cv.src[s]] = cur;
+
+    const pm_dedup_image& ref = images[cv.ref];
+    pm_view_desc vd{};
+    std::vector<pm_image> srcs(cv.num_src);
+    std::vector<pm_camera> cams(cv.num_src);
+    for (int32_t s = 0; s < cv.num_src; ++s) {
+      srcs[s].width = images[cv.src[s]].width;
+      srcs[s].height = images[cv.src[s]].height;
+      cams[s] = images[cv.src[s]].cam;
+    }
+    vd.num_src = cv.num_src;
+    vd.ref_cam = ref.cam;
+    vd.src = srcs.data();
+    vd.src_cams = cams.data();
+    float k8[8];
+    pose_off.push_back(poses.size());
+    poses.resize(poses.size() + (size_t)cv.num_src * PM_POSE_STRIDE);
+    pack_poses(&vd, poses.data() + pose_off.back(), k8);
+    DedupViewDev t;
+    std::memset(&t, 0, sizeof(t));
+    t.W = ref.width;
+    t.H = ref.height;
+    t.N = cv.num_src;
+    t.min_consistent = std::min(o->filter_min_num_consistent, cv.num_src);
+    std::memcpy(t.K, k8, sizeof(t.K));
+    std::memcpy(t.iK, k8 + 4, sizeof(t.iK));
+    t.depth = depth + off[cv.ref];
+    t.vis = vis + off[cv.ref];
+    t.delt = delt_off[k] >= 0 ? (float*)d_delt.p + delt_off[k] : nullptr;
+    for (int32_t s = 0; s < cv.num_src; ++s) {
+      const int32_t si = cv.src[s];
+      t.src[s].depth = depth + off[si];
+      t.src[s].vis = vis + off[si];
+      t.src[s].width = images[si].width;
+      t.src[s].height = images[si].height;
+      t.src[s].flag = images[si].ref_flag ? 1 : 0;
+    }
+    t.tile_start = (int32_t)tiles;
+    tiles += (int64_t)((ref.width + kCrossTile - 1) / kCrossTile) * ((ref.height + kCrossTile - 1) / kCrossTile);
+    if (tiles > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "dedup: too many pixels in one group");
+    tab.push_back(t);
+  }
+  if (!tab.empty()) {
+    PM_HIP(d_poses.get(e, st, std::max<size_t>(poses.size(), 1) * sizeof(float)));
+    PM_HIP(hipMemcpyAsync(d_poses.p, poses.data(), poses.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    for (size_t k = 0; k < tab.size(); ++k) tab[k].poses = (const float*)d_poses.p + pose_off[k];
+    PM_HIP(d_tab.get(e, st, tab.size() * sizeof(DedupViewDev)));
+    PM_HIP(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(DedupViewDev), hipMemcpyHostToDevice, st));
+    DedupParams prm;
+    prm.max_reproj = o->filter_geom_consistency_max_cost;
+    prm.depth_thr = o->depth_diff_threshold;
+    prm.sqr_depth_thr = o->depth_diff_threshold * o->depth_diff_threshold;
+    for (const auto& g : groups) {
+      const DedupViewDev& last = tab[g.first + g.second - 1];
+      const int64_t n_tiles = (int64_t)last.tile_start +
+                              (int64_t)((last.W + kCrossTile - 1) / kCrossTile) * ((last.H + kCrossTile - 1) / kCrossTile);
+      prm.first = g.first;
+      prm.count = g.second;
+      hipLaunchKernelGGL(k_dedup, dim3((unsigned)n_tiles), dim3(kCrossTile * kCrossTile), 0, st,
+                         (const DedupViewDev*)d_tab.p, prm);
+      PM_HIP(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_dedup_depth, dim3(grid_for(total)), dim3(256), 0, st, (const float*)depth, (const uint32_t*)vis,
+                     (float*)d_out.p, total);
+  PM_HIP(hipGetLastError());
+  for (int32_t i = 0; i < num_images; ++i) {
+    const size_t n = (size_t)images[i].width * images[i].height;
+    if (images[i].visibility)
+      PM_HIP(hipMemcpyAsync(images[i].visibility, vis + off[i], n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (images[i].dedup_depth)
+      PM_HIP(hipMemcpyAsync(images[i].dedup_depth, (const float*)d_out.p + off[i], n * sizeof(float),
+                            hipMemcpyDeviceToHost, st));
+  }
+  for (int32_t k = 0; k < num_views; ++k) {
+    if (!views[k].delt) continue;
+    const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
+    if (delt_off[k] >= 0)
+      PM_HIP(hipMemcpyAsync(views[k].delt, (const float*)d_delt.p + delt_off[k], n * sizeof(float),
+                            hipMemcpyDeviceToHost, st));
+    else
+      std::memset(views[k].delt, 0, n * sizeof(float));
+  }
+  PM_HIP(hipStreamSynchronize(st));
+  return PM_OK;
+}

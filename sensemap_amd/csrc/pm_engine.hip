@@ -190,6 +190,13 @@ PM_API pm_status pm_options_check(const pm_options* o) {
   if (!(o->geom_consistency_regularizer >= 0.0f))
     return fail(PM_ERR_INVALID_ARG, "geom_consistency_regularizer must be >= 0");
   if (!(o->geom_consistency_max_cost >= 0.0f)) return fail(PM_ERR_INVALID_ARG, "geom_consistency_max_cost must be >= 0");
+  // the reference CHECKs that they are not both set (patch_match_cuda.cu:950); this engine builds neither
+  if (o->has_prior_depth)
+    return fail(PM_ERR_INVALID_ARG, "has_prior_depth (RGB-D depth priors) is not supported by this engine");
+  if (o->propagate_depth)
+    return fail(PM_ERR_INVALID_ARG,
+                "propagate_depth is not supported: the reference's sweep reads prior normal and weight maps it never "
+                "allocates for it (patch_match_cuda.cu:589-591, kernel_functions.cu:2191-2199)");
   // limits of this engine
   if (o->thk < 1 || o->thk > 4) return fail(PM_ERR_INVALID_ARG, "thk must be in [1, 4] (mweights has 5 entries)");
   if (!(o->beta > 0.0f)) return fail(PM_ERR_INVALID_ARG, "beta must be > 0");

@@ -18,7 +18,9 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <cstdlib>
 #include <new>
+#include <set>
 #include <thread>
 
 using namespace pmh;
@@ -32,6 +34,7 @@ constexpr int kRing = 4;            // step table buffers
 
 struct CacheEntry {
   size_t off = 0, bytes = 0;
+  int32_t width = 0, height = 0;  // the image the entry holds: a key names one image
   int refs = 0;
   uint64_t last_use = 0;
 };
@@ -97,6 +100,7 @@ struct pm_server {
   pm_server_stats st{};
   size_t cache_bytes = 0;
   bool stop = false, loop_done = false;
+  bool abandon = false;  // process exit: stop after the step in progress, leave jobs unfinished
   pm_status fatal = PM_OK;
   std::string fatal_msg;
   std::thread loop, completer;
@@ -128,14 +132,41 @@ bool alloc_evicting(pm_server* sv, size_t bytes, size_t* off) {
   }
 }
 
-// Reserve cache entries and the job's memory block.  Returns false (and
-// holds nothing) when the job does not fit now.
-bool reserve(pm_server* sv, Job* j) {
+// A cache key names one image: a job whose source has a cached key but
+// other dimensions (or that lists one key with two sizes) is an error, not
+// a hit.  Empty when there is no conflict (under mu).
+std::string key_conflict(const pm_server* sv, const pm_view_desc& v) {
+  for (int i = 0; i < v.num_src; ++i) {
+    const pm_image& a = v.src[i];
+    if (a.cache_id <= 0) continue;
+    for (int k = 0; k < i; ++k) {
+      const pm_image& b = v.src[k];
+      if (b.cache_id == a.cache_id && (b.width != a.width || b.height != a.height))
+        return "sources " + std::to_string(k) + " and " + std::to_string(i) + " share cache_id " +
+               std::to_string(a.cache_id) + " but differ in size";
+    }
+    for (const auto& kv : sv->cache) {
+      if (kv.first.first != a.cache_id) continue;
+      if (kv.second.width != a.width || kv.second.height != a.height)
+        return "source " + std::to_string(i) + " (" + std::to_string(a.width) + "x" + std::to_string(a.height) +
+               ") has cache_id " + std::to_string(a.cache_id) + ", which the server holds for a " +
+               std::to_string(kv.second.width) + "x" + std::to_string(kv.second.height) + " image";
+    }
+  }
+  return std::string();
+}
+
+// Reserve cache entries and the job's memory block.  Returns 1 when
+// reserved, 0 (holding nothing) when the job does not fit now, -1 (holding
+// nothing, *why set) when a source's key names another image.
+int reserve(pm_server* sv, Job* j, std::string* why) {
   const int N = j->view.num_src;
   const int storage = j->opt.image_storage;
   j->keys.clear();
   j->fresh.clear();
   j->src_tex.assign(N, nullptr);
+  *why = key_conflict(sv, j->view);
+  if (!why->empty()) return -1;
   std::vector<bool> cached(N, false);
   // hits first, so eviction below never takes an entry this job needs
   for (int i = 0; i < N; ++i) {
@@ -164,6 +195,8 @@ bool reserve(pm_server* sv, Job* j) {
     }
     CacheEntry e;
     e.bytes = src_quad_bytes(j->view.src[i], storage);
+    e.width = j->view.src[i].width;
+    e.height = j->view.src[i].height;
     if (!alloc_evicting(sv, e.bytes, &e.off)) {
       ok = false;
       break;
@@ -197,7 +230,7 @@ bool reserve(pm_server* sv, Job* j) {
     }
     j->keys.clear();
     j->fresh.clear();
-    return false;
+    return 0;
   }
   const uint64_t now = ++sv->use_clock;
   for (const CacheKey& k : j->keys) sv->cache[k].last_use = now;
@@ -206,7 +239,7 @@ bool reserve(pm_server* sv, Job* j) {
   for (int i = 0; i < N; ++i)
     if (j->view.src[i].cache_id > 0) hits++;
   sv->st.cache_hits += (uint64_t)std::max(0, hits - (int)j->fresh.size());
-  return true;
+  return 1;
 }
 
 // Give back everything job j holds (under mu).  drop_fresh: the job's own
@@ -376,16 +409,24 @@ void loop_main(pm_server* sv) {
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       for (;;) {
-        if (sv->stop && sv->pending.empty() && sv->active.empty()) {
+        if (sv->abandon || (sv->stop && sv->pending.empty() && sv->active.empty())) {
           quit = true;
           break;
         }
         if (!sv->pending.empty() && (int)sv->active.size() < sv->cfg.max_active_views) {
           Job* j = sv->pending.front();
-          if (reserve(sv, j)) {
+          std::string why;
+          const int r = reserve(sv, j, &why);
+          if (r > 0) {
             sv->pending.pop_front();
             admit = j;
             break;
+          }
+          if (r < 0) {
+            sv->pending.pop_front();
+            fail_job(sv, j, PM_ERR_INVALID_ARG, why);
+            sv->cv.notify_all();
+            continue;
           }
           if (sv->active.empty()) {
             if (sv->completing.empty() && sv->copying == 0) {
@@ -397,7 +438,7 @@ void loop_main(pm_server* sv) {
               // the bytes the submit check allowed.
               while (evict_one(sv)) {
               }
-              if (reserve(sv, j)) {
+              if (reserve(sv, j, &why) > 0) {
                 sv->pending.pop_front();
                 admit = j;
                 break;
@@ -497,7 +538,7 @@ void completer_main(pm_server* sv) {
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv.wait(lk, [&] { return !sv->completing.empty() || sv->loop_done; });
-      if (sv->completing.empty()) break;
+      if (sv->completing.empty() || (sv->abandon && sv->loop_done)) break;
       j = sv->completing.front();
       sv->completing.pop_front();
       sv->copying++;  // still holds its memory until released below
@@ -525,6 +566,53 @@ void destroy_job(Job* j) {
   if (j->done_ev) (void)hipEventDestroy(j->done_ev);
   delete j->prob;
   delete j;
+}
+
+// Servers alive at process exit.  A caller may exit without destroying a
+// server (CPython, for one, does not finalise every object still alive at
+// exit).  Its loop thread would then go on issuing launches and its
+// completion thread waiting on events while exit() runs the HIP runtime's
+// static destructors, which tear down the streams and events those calls
+// use.  The first server registers stop_live_servers with atexit after the
+// runtime is initialised (pm_server_create has called into it), and atexit
+// handlers run in reverse order of registration, so the handler runs before
+// the runtime's teardown.  It stops each live server's threads after the
+// step in progress and waits for the device work already issued; the jobs
+// left are never finished, and nothing is freed (the process is ending).
+// A pm_server_destroy after that makes no HIP call.
+struct LiveServers {
+  std::mutex mu;
+  std::set<pm_server*> servers;
+  bool registered = false, exiting = false;
+};
+
+LiveServers& live_servers() {
+  static LiveServers* l = new LiveServers();  // never destroyed: used from atexit
+  return *l;
+}
+
+void stop_live_servers() {
+  LiveServers& l = live_servers();
+  std::set<pm_server*> servers;
+  {
+    std::lock_guard<std::mutex> lk(l.mu);
+    l.exiting = true;
+    servers = l.servers;
+  }
+  for (pm_server* sv : servers) {
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->abandon = true;
+      sv->stop = true;
+      sv->cv.notify_all();
+    }
+    if (sv->loop.joinable()) sv->loop.join();
+    if (sv->completer.joinable()) sv->completer.join();
+    (void)hipSetDevice(sv->engine->device);
+    (void)hipStreamSynchronize(sv->compute);
+    (void)hipStreamSynchronize(sv->upload);
+    (void)hipStreamSynchronize(sv->copy);
+  }
 }
 
 }  // namespace
@@ -556,6 +644,15 @@ PM_API pm_status pm_server_create(pm_engine* e, const pm_server_config* cfg, pm_
   if (rs != PM_OK) return rs;
   pm_server* p = sv.release();
   p->st.arena_capacity = p->arena.capacity();
+  {
+    LiveServers& l = live_servers();
+    std::lock_guard<std::mutex> lk(l.mu);
+    if (!l.registered) {
+      l.registered = true;
+      std::atexit(stop_live_servers);
+    }
+    l.servers.insert(p);
+  }
   p->loop = std::thread(loop_main, p);
   p->completer = std::thread(completer_main, p);
   *out = p;
@@ -586,6 +683,11 @@ PM_API pm_status pm_server_submit(pm_server* sv, const pm_options* o, const pm_v
   if (sv->fatal != PM_OK) {
     delete j;
     return fail(sv->fatal, "server stopped after a device error: " + sv->fatal_msg);
+  }
+  const std::string conflict = key_conflict(sv, *v);
+  if (!conflict.empty()) {
+    delete j;
+    return fail(PM_ERR_INVALID_ARG, conflict);
   }
   j->ticket = sv->next_ticket++;
   sv->jobs[j->ticket] = j;
@@ -641,6 +743,12 @@ PM_API pm_status pm_server_get_stats(pm_server* sv, pm_server_stats* out) {
 
 PM_API void pm_server_destroy(pm_server* sv) {
   if (!sv) return;
+  {
+    LiveServers& l = live_servers();
+    std::lock_guard<std::mutex> lk(l.mu);
+    if (l.exiting) return;  // stopped by stop_live_servers; the runtime may be gone
+    l.servers.erase(sv);
+  }
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->stop = true;

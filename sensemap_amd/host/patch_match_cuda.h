@@ -72,6 +72,8 @@ struct PatchMatchOptions {
   float conf_threshold = 0.5;
   float depth_diff_threshold = 0.02;
   int filter_min_num_consistent = 1;
+  bool propagate_depth = false;  // rejected by Check() and the engine
+  bool has_prior_depth = false;  // rejected by Check() and the engine
   double filter_geom_consistency_max_cost = 1.0f;
   double max_normal_error = 15.0f;
   std::string gpu_index = "-1";
@@ -116,6 +118,8 @@ struct PatchMatchOptions {
     o.depth_diff_threshold = depth_diff_threshold;
     o.filter_geom_consistency_max_cost = (float)filter_geom_consistency_max_cost;
     o.filter_min_num_consistent = filter_min_num_consistent;
+    o.propagate_depth = propagate_depth;
+    o.has_prior_depth = has_prior_depth;
     o.init_samples = init_samples;
     o.compat_lo_geo_bug = compat_lo_geo_bug;
     o.image_storage = image_storage;
@@ -170,6 +174,8 @@ class DepthMap : public Mat<float> {
   DepthMap() = default;
   DepthMap(size_t width, size_t height, float depth_min, float depth_max)
       : Mat<float>(width, height, 1), depth_min_(depth_min), depth_max_(depth_max) {}
+  DepthMap(const Mat<float>& m, float depth_min, float depth_max)
+      : Mat<float>(m), depth_min_(depth_min), depth_max_(depth_max) {}
   float GetDepthMin() const { return depth_min_; }
   float GetDepthMax() const { return depth_max_; }
 
@@ -469,6 +475,92 @@ class CrossFilterCuda {
     return refined_[k];
   }
 
+  // DedupRun (patch_match_cuda.cu:1561-1612): after Run(), each problem in
+  // order over the refined maps of the flagged images (ResetDepthMaps,
+  // :1260-1295) and the given maps of the others.
+  bool DedupRun() {
+    if (problems_.empty()) return true;
+    const Problem& p0 = problems_[0];
+    const size_t n = ids_.size();
+    std::vector<pm_dedup_image> images(n);
+    dedup_depth_.assign(n, DepthMap());
+    visibility_.assign(n, Mat<uint32_t>());
+    delt_.assign(n, DepthMap());
+    for (size_t k = 0; k < n; ++k) {
+      const int id = ids_[k];
+      const Image& im = p0.images->at(id);
+      const bool flag = k < ref_flags_.size() && ref_flags_[k];
+      const bool refined = flag && k < refined_.size() && refined_[k].GetWidth() > 0;
+      const DepthMap& d = refined ? refined_[k] : p0.depth_maps->at(id);
+      pm_dedup_image& c = images[k];
+      c.cam = ToCamera(im);
+      c.width = (int32_t)im.GetWidth();
+      c.height = (int32_t)im.GetHeight();
+      c.depth = d.GetPtr();
+      c.ref_flag = flag ? 1 : 0;
+      dedup_depth_[k] = DepthMap(c.width, c.height, d.GetDepthMin(), d.GetDepthMax());
+      visibility_[k] = Mat<uint32_t>(c.width, c.height, 1);
+      c.dedup_depth = dedup_depth_[k].GetPtr();
+      c.visibility = visibility_[k].GetPtr();
+    }
+    std::vector<pm_dedup_view> views;
+    std::vector<std::vector<int32_t>> srcs;
+    std::vector<int> refs;
+    for (const Problem& p : problems_) {
+      const int r = Position(p.ref_image_idx);
+      if (r < 0) continue;
+      std::vector<int32_t> s;
+      for (int id : p.src_image_extend_idxs) {
+        const int k = Position(id);
+        if (k >= 0 && k != r && s.size() < 31) s.push_back(k);
+      }
+      if (s.empty()) continue;  // FindSrcImageLists gave < 2 entries (:1586-1588)
+      srcs.push_back(std::move(s));
+      refs.push_back(r);
+    }
+    for (size_t v = 0; v < refs.size(); ++v) {
+      const int r = refs[v];
+      delt_[r] = DepthMap(images[r].width, images[r].height, 0.0f, 1.0f);  // the last problem on r wins
+      pm_dedup_view cv{};
+      cv.ref = r;
+      cv.num_src = (int32_t)srcs[v].size();
+      cv.src = srcs[v].data();
+      cv.delt = delt_[r].GetPtr();
+      views.push_back(cv);
+    }
+    pm_cross_options o;
+    PmCheck(pm_cross_options_init(&o));
+    o.filter_geom_consistency_max_cost = (float)options_.filter_geom_consistency_max_cost;
+    o.depth_diff_threshold = options_.depth_diff_threshold;
+    o.filter_min_num_consistent = options_.filter_min_num_consistent;
+    PmCheck(pm_cross_dedup(engine_, &o, (int32_t)n, images.data(), (int32_t)views.size(), views.data()));
+    deduped_ = true;
+    return true;
+  }
+
+  // GetDeduplicDepthMap (patch_match_cuda.cu:1641-1663): an empty map for an
+  // image not taking part.
+  DepthMap GetDeduplicDepthMap(int image_id, DepthMapInfo info) const {
+    const int k = Position(image_id);
+    if (k < 0 || !deduped_) return DepthMap();
+    DepthMap m = dedup_depth_[k];
+    return DepthMap(m, info.depth_min, info.depth_max);
+  }
+  // GetDeltDepthMap (patch_match_cuda.cu:1665-1675): a 1x1 zero map for an
+  // image that was no problem's reference (InitDeltDepthMaps, :1420-1425).
+  DepthMap GetDeltDepthMap(int image_id) const {
+    const int k = Position(image_id);
+    if (k < 0 || !deduped_) return DepthMap();
+    if (delt_[k].GetWidth() == 0) return DepthMap(1, 1, 0.0f, 1.0f);
+    return delt_[k];
+  }
+  // GetVisibilyMap (patch_match_cuda.cu:1677-1703).
+  Mat<uint32_t> GetVisibilyMap(int image_id) const {
+    const int k = Position(image_id);
+    if (k < 0 || !deduped_) return Mat<uint32_t>();
+    return visibility_[k];
+  }
+
  private:
   int Position(int image_id) const {
     for (size_t k = 0; k < ids_.size(); ++k)
@@ -492,6 +584,9 @@ class CrossFilterCuda {
   std::vector<int> ids_;
   std::vector<bool> ref_flags_;
   std::vector<DepthMap> refined_;
+  std::vector<DepthMap> dedup_depth_, delt_;
+  std::vector<Mat<uint32_t>> visibility_;
+  bool deduped_ = false;
   pm_engine* engine_ = nullptr;
 };
 

@@ -8,9 +8,11 @@ library; this module only marshals numpy arrays into the C structs.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
 import os
+import weakref
 from dataclasses import dataclass, field, fields
 from typing import List, Optional, Sequence
 
@@ -52,6 +54,8 @@ class PatchMatchOptions:
     depth_diff_threshold: float = 0.02
     filter_geom_consistency_max_cost: float = 1.0
     filter_min_num_consistent: int = 1
+    propagate_depth: bool = False  # not supported: rejected by Check() / the engine
+    has_prior_depth: bool = False  # not supported: rejected by Check() / the engine
     # extensions
     init_samples: int = 1
     compat_lo_geo_bug: bool = True
@@ -336,10 +340,12 @@ class PatchMatchACMMCuda:
 
 
 class _ProblemView:
-    """Read-only accessors on a problem handle owned by a batch."""
+    """Read-only accessors on a problem handle owned by a batch.  The view
+    keeps its batch alive: the handle is freed with the batch."""
 
-    def __init__(self, lib, handle, W, H, N):
-        self.lib, self.handle, self.W, self.H, self.N = lib, handle, W, H, N
+    def __init__(self, batch, handle, W, H, N):
+        self._batch = batch
+        self.lib, self.handle, self.W, self.H, self.N = batch.lib, handle, W, H, N
 
     _get = PatchMatchACMMCuda._get
     GetDepthMap = PatchMatchACMMCuda.GetDepthMap
@@ -373,7 +379,7 @@ class PatchMatchBatch:
         for i, d in enumerate(self.descs):
             vh = C.c_void_p()
             abi.check(self.lib.pm_batch_get_view(self.handle, i, C.byref(vh)))
-            self.views.append(_ProblemView(self.lib, vh, d.width, d.height, d.num_src))
+            self.views.append(_ProblemView(self, vh, d.width, d.height, d.num_src))
 
     def Run(self) -> bool:
         abi.check(self.lib.pm_batch_run(self.handle))
@@ -402,12 +408,20 @@ class ServerJob:
 
     MAPS = ("depth", "normal", "conf", "cost", "view_sel", "curvature")
 
-    def __init__(self, server: "PatchMatchServer", options: PatchMatchOptions, problem: Problem, maps=None):
+    def __init__(self, server: "PatchMatchServer", options: PatchMatchOptions, problem: Problem, maps=None,
+                 cache_ids=None):
         """maps: the names of the maps to return (default all); the others
-        are neither allocated nor copied back."""
+        are neither allocated nor copied back.  cache_ids: per source, the
+        key the server caches it under (> 0; <= 0 never cached); by default
+        the Problem's image index + 1, so one server serves one image set."""
         self.server = server
         self.options = options
         self.desc = ViewDesc(problem, options)
+        if cache_ids is not None:
+            if len(cache_ids) != self.desc.num_src:
+                raise ValueError("need one cache id per source")
+            for k, cid in enumerate(cache_ids):
+                self.desc.desc.src[k].cache_id = int(cid)
         W, H, N = self.desc.width, self.desc.height, self.desc.num_src
         want = set(self.MAPS if maps is None else maps)
         unknown = want - set(self.MAPS)
@@ -436,25 +450,52 @@ class ServerJob:
         # it holds the job until then, whatever the caller keeps
         server._inflight[self.ticket] = self
         self.stats: Optional[abi.pm_job_stats] = None
+        self._error: Optional[abi.PMError] = None
 
     def done(self) -> bool:
+        if self.stats is not None:
+            return True
         d = C.c_int32()
         abi.check(self.server.lib.pm_server_poll(self.server.handle, self.ticket, C.byref(d)))
         return bool(d.value)
 
+    def _finish(self) -> None:
+        """pm_server_wait once; keeps the stats and the job's error."""
+        st = abi.pm_job_stats()
+        rc = self.server.lib.pm_server_wait(self.server.handle, self.ticket, C.byref(st))
+        self.stats = st
+        if rc != abi.PM_OK:
+            self._error = abi.PMError(rc, self.server.lib.pm_last_error().decode(errors="replace"))
+        self.server._inflight.pop(self.ticket, None)
+
     def wait(self) -> abi.pm_job_stats:
-        """Block until the maps are written; raises if the job failed."""
+        """Block until the maps are written; raises if the job failed (on
+        every call, also after the server is closed)."""
         if self.stats is None:
-            st = abi.pm_job_stats()
-            rc = self.server.lib.pm_server_wait(self.server.handle, self.ticket, C.byref(st))
-            self.stats = st
-            self.server._inflight.pop(self.ticket, None)
-            abi.check(rc)
+            self._finish()
+        if self._error is not None:
+            raise self._error
         return self.stats
 
     def maps(self) -> dict:
         self.wait()
         return {name: getattr(self, name) for name in self.MAPS if getattr(self, name) is not None}
+
+
+_live_servers: "weakref.WeakSet[PatchMatchServer]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_servers() -> None:
+    """At interpreter exit, before module teardown: close every open server,
+    so each finishes its jobs and joins its threads while the jobs' buffers
+    and the HIP runtime are still there (CPython does not finalise every
+    object alive at exit)."""
+    for sv in list(_live_servers):
+        try:
+            sv.close()
+        except Exception:
+            pass
 
 
 class PatchMatchServer:
@@ -471,10 +512,12 @@ class PatchMatchServer:
         h = C.c_void_p()
         abi.check(self.lib.pm_server_create(self.engine.handle, C.byref(cfg), C.byref(h)))
         self.handle = h
+        _live_servers.add(self)
 
-    def submit(self, options: PatchMatchOptions, problem: Problem, maps=None) -> ServerJob:
-        """Queue a job; maps: the names of the maps to return (default all)."""
-        return ServerJob(self, options, problem, maps)
+    def submit(self, options: PatchMatchOptions, problem: Problem, maps=None, cache_ids=None) -> ServerJob:
+        """Queue a job; maps: the names of the maps to return (default all);
+        cache_ids: the sources' cache keys (see ServerJob)."""
+        return ServerJob(self, options, problem, maps, cache_ids)
 
     def stats(self) -> abi.pm_server_stats:
         s = abi.pm_server_stats()
@@ -482,9 +525,12 @@ class PatchMatchServer:
         return s
 
     def close(self) -> None:
-        """Finish every submitted job, then free the server."""
+        """Finish every submitted job (each keeps its maps and status for
+        its wait()), then free the server."""
         if getattr(self, "handle", None):
-            self.lib.pm_server_destroy(self.handle)  # finishes every job first
+            for job in list(self._inflight.values()):
+                job._finish()
+            self.lib.pm_server_destroy(self.handle)
             self.handle = None
         self._inflight = {}
 

@@ -4,7 +4,9 @@
 //                    n_views; per view ref num_src src...;
 //                    filter_geom_consistency_max_cost max_normal_error depth_diff_threshold min_consistent
 //   <dir>/depth<k>.bin, <dir>/normal<k>.bin : the image's maps as DepthMap / NormalMap files
-// and writes <dir>/refined<ref>.bin (DepthMap::Write) for every view.
+// and writes <dir>/refined<ref>.bin (DepthMap::Write) for every view; then
+// runs DedupRun (the views' references flagged) and writes, per image k,
+// <dir>/dedup<k>.bin, <dir>/vis<k>.bin (Mat<uint32_t>) and <dir>/delt<k>.bin.
 #include <fstream>
 #include <iostream>
 #include <vector>
@@ -53,13 +55,24 @@ int main(int argc, char** argv) {
   std::vector<int> ids(n);
   for (int k = 0; k < n; ++k) ids[k] = k;
   try {
-    CrossFilterCuda filter(options, problems, ids, std::vector<bool>(n, true));
+    std::vector<bool> ref_flags(n, false);
+    for (const Problem& p : problems) ref_flags[p.ref_image_idx] = true;
+    CrossFilterCuda filter(options, problems, ids, ref_flags);
     filter.Run();
     for (const Problem& p : problems) {
       DepthMapInfo info;
       info.width = (int)images[p.ref_image_idx].GetWidth();
       info.height = (int)images[p.ref_image_idx].GetHeight();
       filter.GetRefineDepthMap(p.ref_image_idx, info).Write(dir + "/refined" + std::to_string(p.ref_image_idx) + ".bin");
+    }
+    filter.DedupRun();
+    for (int k = 0; k < n; ++k) {
+      DepthMapInfo info;
+      info.width = (int)images[k].GetWidth();
+      info.height = (int)images[k].GetHeight();
+      filter.GetDeduplicDepthMap(k, info).Write(dir + "/dedup" + std::to_string(k) + ".bin");
+      filter.GetVisibilyMap(k).Write(dir + "/vis" + std::to_string(k) + ".bin");
+      filter.GetDeltDepthMap(k).Write(dir + "/delt" + std::to_string(k) + ".bin");
     }
   } catch (const std::exception& e) {
     std::cerr << e.what() << "\n";

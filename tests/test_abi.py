@@ -69,12 +69,14 @@ def test_option_defaults_follow_patch_match_options():
     assert (o.alpha, o.thk, o.num_good_hypothesis, o.num_bad_hypothesis) == (90.0, 4, 2, 3)
     assert abs(o.beta - 0.3) < 1e-7
     assert o.init_samples == 1 and o.compat_lo_geo_bug == 1
+    assert o.propagate_depth == 0 and o.has_prior_depth == 0
 
 
 @pytest.mark.parametrize("field,value", [
     ("window_radius", 0), ("window_radius", 33), ("window_step", 3), ("window_step", 0),
     ("sigma_color", 0.0), ("num_samples", 0), ("thk", 0), ("thk", 5), ("geom_consistency_max_cost", -1.0),
     ("image_storage", 7), ("accum_mode", 3), ("init_samples", 0),
+    ("propagate_depth", 1), ("has_prior_depth", 1),
 ])
 def test_option_check_rejects(field, value):
     lib = abi.load_engine()

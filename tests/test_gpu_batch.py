@@ -168,3 +168,21 @@ def test_batch_autotune_equals_views_alone():
             assert all(d == 0 for d in diff.values()), (rep, k, diff)
     b.close()
     eng.close()
+
+
+def test_view_keeps_its_batch_alive():
+    import gc
+    items = mixed_items()[:2]
+    alone = []
+    for opt, prob in items:
+        e = pmm.PatchMatchACMMCuda(opt, prob)
+        e.Run()
+        alone.append(grab(e))
+        e.close()
+    b = pmm.PatchMatchBatch(items)
+    b.Run()
+    v = b.views[1]
+    del b
+    gc.collect()
+    diff = helpers.maps_equal(grab(v), alone[1], MAPS)
+    assert all(x == 0 for x in diff.values()), diff

@@ -99,6 +99,23 @@ def test_cpp_cross_filter_matches_python_path(tmp_path):
         got = pmio.read_depth_map(str(tmp_path / f"refined{r}.bin"))
         assert (got > 0).sum() > 0
         assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), r
+    # DedupRun after Run: the references' refined maps, the others' own maps
+    from sensemap_amd.cross_filter import dedup
+    refined = {r: w for (r, _), w in zip(views, want)}
+    dims = [(CrossImage(im.K, im.R, im.T, refined.get(k, im.depth), im.normal), k in refined)
+            for k, im in enumerate(ims)]
+    d = dedup(dims, views)
+    for k in range(len(ims)):
+        got_dd = pmio.read_depth_map(str(tmp_path / f"dedup{k}.bin"))
+        got_vis = pmio.read_mat(str(tmp_path / f"vis{k}.bin"), np.uint32)
+        got_delt = pmio.read_depth_map(str(tmp_path / f"delt{k}.bin"))
+        assert np.array_equal(got_dd.view(np.uint32), d["dedup_depth"][k].view(np.uint32)), k
+        assert np.array_equal(got_vis.reshape(d["visibility"][k].shape), d["visibility"][k]), k
+        if k in refined:
+            want_delt = d["delt"][[r for r, _ in views].index(k)]
+            assert np.array_equal(got_delt.view(np.uint32), want_delt.view(np.uint32)), k
+        else:
+            assert got_delt.shape == (1, 1)
 
 
 @pytest.mark.gpu

@@ -66,3 +66,58 @@ def test_module_globals_finalised_at_exit():
     print(r.stdout, r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     assert "ran" in r.stdout
+
+
+def _child(script):
+    r = subprocess.run([sys.executable, "-c", textwrap.dedent(script)], cwd=ROOT, capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    return r
+
+
+def test_server_in_module_globals_at_exit():
+    # a server with jobs still in flight, held only in the script's globals:
+    # the module's exit hook closes it (jobs finish, threads joined) before
+    # the interpreter tears its globals down
+    script = """
+        import numpy as np
+        from sensemap_amd import patch_match as pmm
+        from tests.golden import make_golden as mg
+        f = np.load("tests/golden/scene_planes_n2.npz")
+        eng = pmm.Engine(0)
+        sv = pmm.PatchMatchServer(eng, hbm_budget_bytes=4 << 30, max_active_views=2)
+        jobs = [sv.submit(mg.options(2), mg.problem_from(f)) for _ in range(6)]
+        jobs[0].wait()
+        print("ran", flush=True)
+    """
+    print("[lifetime] child process with a server and jobs in flight in its globals", flush=True)
+    r = _child(script)
+    assert r.returncode == 0
+    assert "ran" in r.stdout
+    assert "terminate" not in r.stderr
+
+
+def test_server_never_destroyed_is_stopped_at_exit():
+    # the server and its jobs are leaked (never finalised, not closed by the
+    # Python hook), so only the library's own exit handler stops its threads
+    # before the HIP runtime's teardown
+    script = """
+        import ctypes
+        import numpy as np
+        from sensemap_amd import patch_match as pmm
+        from tests.golden import make_golden as mg
+        f = np.load("tests/golden/scene_planes_n2.npz")
+        eng = pmm.Engine(0)
+        sv = pmm.PatchMatchServer(eng, hbm_budget_bytes=4 << 30, max_active_views=2)
+        pmm._live_servers.discard(sv)
+        jobs = [sv.submit(mg.options(2), mg.problem_from(f)) for _ in range(6)]
+        jobs[0].wait()
+        for obj in [sv, eng] + jobs:
+            ctypes.pythonapi.Py_IncRef(ctypes.py_object(obj))
+        print("ran", flush=True)
+    """
+    print("[lifetime] child process exiting with a leaked server", flush=True)
+    r = _child(script)
+    assert r.returncode == 0
+    assert "ran" in r.stdout
+    assert "terminate" not in r.stderr

@@ -167,3 +167,47 @@ def test_geometric_job_through_the_server():
     assert all(x == 0 for x in diff.values()), diff
     sv.close()
     eng.close()
+
+
+def test_cache_key_of_another_size_is_rejected():
+    eng = pmm.Engine(0)
+    items = orbit_jobs()
+    sv = pmm.PatchMatchServer(eng, hbm_budget_bytes=4 << 30, max_active_views=2)
+    o, p = items[0]
+    a = sv.submit(o, p, maps=("depth",), cache_ids=[1000 + k for k in range(len(p.src_image_idxs))])
+    a.wait()
+    # the same key for a source of another size: an error, not a cache hit
+    o2, p2 = next((o2, p2) for o2, p2 in items[1:]
+                  if p2.images[p2.src_image_idxs[0]].bitmap.shape != p.images[p.src_image_idxs[0]].bitmap.shape)
+    with pytest.raises(abi.PMError) as ei:
+        sv.submit(o2, p2, cache_ids=[1000] + [0] * (len(p2.src_image_idxs) - 1))
+    assert ei.value.status == abi.PM_ERR_INVALID_ARG
+    print("[server] rejected:", ei.value, flush=True)
+    # one key for two sources of different sizes in one job
+    srcs = [(k, p2.images[i].bitmap.shape) for k, i in enumerate(p2.src_image_idxs)]
+    pair = next(((k1, k2) for k1, s1 in srcs for k2, s2 in srcs if k1 < k2 and s1 != s2), None)
+    if pair is not None:
+        ids = [0] * len(srcs)
+        ids[pair[0]] = ids[pair[1]] = 2000
+        with pytest.raises(abi.PMError) as ei:
+            sv.submit(o2, p2, cache_ids=ids)
+        assert ei.value.status == abi.PM_ERR_INVALID_ARG
+    # the server still serves
+    b = sv.submit(o2, p2, maps=("depth",))
+    b.wait()
+    sv.close()
+
+
+def test_jobs_keep_their_results_after_close():
+    eng = pmm.Engine(0)
+    items = orbit_jobs()[:4]
+    want = alone_maps(items, eng)
+    sv = pmm.PatchMatchServer(eng, hbm_budget_bytes=4 << 30, max_active_views=2)
+    jobs = [sv.submit(o, p) for o, p in items]
+    sv.close()  # finishes every job; each keeps its maps and status
+    for k, j in enumerate(jobs):
+        assert j.done()
+        st = j.wait()
+        assert st.status == abi.PM_OK and j.wait() is st
+        diff = helpers.maps_equal(j.maps(), want[k], MAPS)
+        assert all(d == 0 for d in diff.values()), (k, diff)
