@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 4
+#define PM_ABI_VERSION 5
 #define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
 #define PM_MAX_WINDOW_RADIUS 32
 #define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
@@ -488,6 +488,66 @@ typedef struct pm_dedup_view {
 
 pm_status pm_cross_dedup(pm_engine* engine, const pm_cross_options* options, int32_t num_images,
                          const pm_dedup_image* images, int32_t num_views, const pm_dedup_view* views);
+
+/* ---- depth-map fusion ----
+ * StereoFusion's fusion of the depth and normal maps into points
+ * (fusion.cc:1730-2183, FuseImage and Fuse), over the images in the order
+ * StereoFusion::Run takes them (the first non-rig image, then FindNextImage,
+ * fusion.cc:697-727, 1150-1215), with one fusing thread per image: seeds
+ * row by row at step_size from the 3-pixel margin, each seed walking the
+ * consistency graph depth first through the images' overlapping lists.  A
+ * pixel belongs to the first walk that reaches it.  The result is the
+ * reference's with num_eff_threads = 1 (several threads share the fused
+ * pixel masks, so their result depends on timing).  Semantic maps, the ROI
+ * box and the steps after the walks (duplicate and outlier removal, plane
+ * completion) are not part of this call. */
+typedef struct pm_fusion_options {
+  int32_t step_size;               /* 1 (fusion.h:57) */
+  int32_t min_num_pixels;          /* 5, >= 2 */
+  int32_t max_num_pixels;          /* 10000 */
+  int32_t min_num_visible_images;  /* 3 */
+  int32_t max_traversal_depth;     /* 100 */
+  int32_t with_normal;             /* 1 */
+  double max_reproj_error;         /* 2.0 px */
+  double max_depth_error;          /* 0.01f, relative */
+  double max_normal_error;         /* 10.0f degrees */
+} pm_fusion_options;
+
+typedef struct pm_fusion_image {
+  pm_camera cam;
+  int32_t width;
+  int32_t height;
+  const float* depth;            /* width*height, <= 0 = no depth */
+  const float* normal;           /* 3 planar width*height slices in the camera frame; may be NULL without normals */
+  const uint8_t* rgb;            /* width*height*3, interleaved r, g, b (Bitmap::GetPixel) */
+  const int32_t* overlapping;    /* overlapping_images_[i]: GetMaxOverlappingImages(check_num_images, 0) */
+  int32_t num_overlapping;
+  int32_t is_rig;                /* Image::IsRig: FindNextImage takes non-rig images first */
+  int32_t used;                  /* used_images_: the image has maps and takes part */
+  int32_t reserved0;
+} pm_fusion_image;
+
+typedef struct pm_fusion pm_fusion;  /* the fused points of one call */
+
+pm_status pm_fusion_options_init(pm_fusion_options* options);
+pm_status pm_fusion_run(pm_engine* engine, const pm_fusion_options* options, int32_t num_images,
+                        const pm_fusion_image* images, pm_fusion** out);
+/* Number of points and of visibility entries (sum over points). */
+pm_status pm_fusion_get_counts(const pm_fusion* fusion, int64_t* num_points, int64_t* num_visibility);
+/* The points in the reference's order: xyz (3 per point), normal (3, zero
+ * without normals), rgb (3), score (1); visibility as CSR: point p is seen
+ * by images vis_images[vis_offsets[p] .. vis_offsets[p+1]) in ascending
+ * order.  Any output may be NULL. */
+pm_status pm_fusion_get_points(const pm_fusion* fusion, float* xyz, float* normal, uint8_t* rgb, float* score,
+                               int64_t* vis_offsets, int32_t* vis_images);
+/* How the device got there: walks run (speculative and kept), rounds,
+ * walks rerun alone with large buffers, points, wall time of the call. */
+typedef struct pm_fusion_stats {
+  int64_t walks, kept, rounds, big_walks, points;
+  double total_ms;
+} pm_fusion_stats;
+pm_status pm_fusion_get_stats(const pm_fusion* fusion, pm_fusion_stats* out);
+void pm_fusion_destroy(pm_fusion* fusion);
 
 /* create + run + read back + destroy in one call. */
 pm_status pm_run_view(pm_engine* engine, const pm_options* options,

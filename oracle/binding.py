@@ -59,6 +59,14 @@ def load():
         lib.pmo_cross_filter.restype = C.c_int
         lib.pmo_cross_filter.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_cross_image), C.c_int,
                                          P(abi.pm_cross_view), C.c_int]
+        lib.pmo_fusion_run.restype = C.c_void_p
+        lib.pmo_fusion_run.argtypes = [P(abi.pm_fusion_options), C.c_int, P(abi.pm_fusion_image)]
+        lib.pmo_fusion_counts.restype = C.c_int
+        lib.pmo_fusion_counts.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64)]
+        lib.pmo_fusion_points.restype = C.c_int
+        lib.pmo_fusion_points.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        lib.pmo_fusion_free.restype = None
+        lib.pmo_fusion_free.argtypes = [C.c_void_p]
         lib.pmo_cross_dedup.restype = C.c_int
         lib.pmo_cross_dedup.argtypes = [P(abi.pm_cross_options), C.c_int, P(abi.pm_dedup_image), C.c_int,
                                         P(abi.pm_dedup_view), C.c_int]
@@ -261,6 +269,24 @@ def cross_filter(images, views, options=None, threads=0):
     if lib.pmo_cross_filter(C.byref(opt), len(images), inp.images, len(views), inp.views, int(threads)) != 0:
         raise RuntimeError("pmo_cross_filter failed")
     return inp.outputs
+
+
+def fusion(images, options=None):
+    """The transcription of StereoFusion's walk, same inputs and outputs as
+    sensemap_amd.fusion.fuse."""
+    from sensemap_amd.fusion import FusionInputs, StereoFusionOptions, unpack
+    lib = load()
+    opt = (options or StereoFusionOptions()).to_c()
+    inp = FusionInputs(images)
+    f = lib.pmo_fusion_run(C.byref(opt), len(images), inp.images)
+    if not f:
+        raise RuntimeError("pmo_fusion_run failed")
+    try:
+        n, nv = C.c_int64(), C.c_int64()
+        lib.pmo_fusion_counts(f, C.byref(n), C.byref(nv))
+        return unpack(n.value, nv.value, lambda *a: lib.pmo_fusion_points(f, *a))
+    finally:
+        lib.pmo_fusion_free(f)
 
 
 def cross_dedup(images, views, options=None, threads=0):

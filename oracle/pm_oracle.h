@@ -159,6 +159,14 @@ int pmo_cost_maps(const pm_options* opt, const pm_view_desc* view, const float* 
 /* RunFilter on view->init_depth into depth_out (W*H). */
 int pmo_run_filter(const pm_options* opt, const pm_view_desc* view, int dedup, float* depth_out, int num_threads);
 /* CrossFilterCuda::Run over the views (refined maps into views[k].refined). */
+/* StereoFusion's walk (pm_fusion_oracle.c), same options and images as the
+ * C-ABI's pm_fusion_run; NULL on failure. */
+struct pmo_fusion;
+struct pmo_fusion* pmo_fusion_run(const pm_fusion_options* options, int num_images, const pm_fusion_image* images);
+int pmo_fusion_counts(const struct pmo_fusion* f, int64_t* num_points, int64_t* num_vis);
+int pmo_fusion_points(const struct pmo_fusion* f, float* xyz, float* normal, uint8_t* rgb, float* score,
+                      int64_t* vis_offsets, int32_t* vis_images);
+void pmo_fusion_free(struct pmo_fusion* f);
 /* CrossFilterCuda::DedupRun and its getters over the views in order, same
  * arguments and outputs as the C-ABI's pm_cross_dedup. */
 int pmo_cross_dedup(const pm_cross_options* options, int num_images, const pm_dedup_image* images, int num_views,

@@ -198,6 +198,24 @@ class pm_dedup_view(C.Structure):
                 ("delt", C.POINTER(C.c_float))]
 
 
+class pm_fusion_options(C.Structure):
+    _fields_ = [("step_size", C.c_int32), ("min_num_pixels", C.c_int32), ("max_num_pixels", C.c_int32),
+                ("min_num_visible_images", C.c_int32), ("max_traversal_depth", C.c_int32), ("with_normal", C.c_int32),
+                ("max_reproj_error", C.c_double), ("max_depth_error", C.c_double), ("max_normal_error", C.c_double)]
+
+
+class pm_fusion_image(C.Structure):
+    _fields_ = [("cam", pm_camera), ("width", C.c_int32), ("height", C.c_int32), ("depth", C.POINTER(C.c_float)),
+                ("normal", C.POINTER(C.c_float)), ("rgb", C.POINTER(C.c_uint8)),
+                ("overlapping", C.POINTER(C.c_int32)), ("num_overlapping", C.c_int32), ("is_rig", C.c_int32),
+                ("used", C.c_int32), ("reserved0", C.c_int32)]
+
+
+class pm_fusion_stats(C.Structure):
+    _fields_ = [("walks", C.c_int64), ("kept", C.c_int64), ("rounds", C.c_int64), ("big_walks", C.c_int64),
+                ("points", C.c_int64), ("total_ms", C.c_double)]
+
+
 class pm_model(C.Structure):
     _fields_ = [("num_images", C.c_int32), ("R", C.POINTER(C.c_float)), ("T", C.POINTER(C.c_float)),
                 ("is_rig", C.POINTER(C.c_uint8)), ("num_points", C.c_int64), ("xyz", C.POINTER(C.c_float)),
@@ -271,6 +289,14 @@ _SIGNATURES = {
                                   C.c_int32, C.POINTER(pm_cross_view)]),
     "pm_cross_filter_problems": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32,
                                            C.POINTER(C.c_void_p), C.c_int32, C.POINTER(pm_cross_view)]),
+    "pm_fusion_options_init": (C.c_int, [C.POINTER(pm_fusion_options)]),
+    "pm_fusion_run": (C.c_int, [C.c_void_p, C.POINTER(pm_fusion_options), C.c_int32, C.POINTER(pm_fusion_image),
+                                C.POINTER(C.c_void_p)]),
+    "pm_fusion_get_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pm_fusion_get_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    "pm_fusion_destroy": (None, [C.c_void_p]),
+    "pm_fusion_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_fusion_stats)]),
     "pm_cross_dedup": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_dedup_image),
                                  C.c_int32, C.POINTER(pm_dedup_view)]),
     "pm_model_max_overlapping_images": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.c_int32, C.c_double,

@@ -29,7 +29,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", "-Wall", "-Wno-unused-function",
 ]
 
-HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip", "pm_server.hip", "pm_io.hip", "pm_cross.hip", "pm_model.hip"]
+HIP_SOURCES = ["pm_kernels.hip", "pm_engine.hip", "pm_server.hip", "pm_io.hip", "pm_cross.hip", "pm_model.hip",
+               "pm_fusion.hip"]
 # extra compile flags (tuning knobs such as -DPM_SWEEP_WAVES_PER_EU=6)
 EXTRA = os.environ.get("PM_EXTRA_HIPFLAGS", "").split()
 

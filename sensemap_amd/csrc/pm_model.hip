@@ -643,3 +643,4 @@ PM_API pm_status pm_model_depth_ranges(pm_engine* e, const pm_model* m, float* o
   PM_HIP(hipStreamSynchronize(st));
   return PM_OK;
 }
+

@@ -1,0 +1,94 @@
+"""Depth-map fusion on the device (pm_fusion_run) equals its transcription
+bit for bit -- positions, normals, colours, scores, visibility and the order
+of the points -- on the designed plane scenes, on an orbit of rendered views
+with noisy depths, and on a scene whose walks outgrow the per-walk buffers
+(rerun alone with buffers for max_num_pixels)."""
+import numpy as np
+import pytest
+
+from oracle import binding
+from sensemap_amd import abi, scene
+from sensemap_amd import patch_match as pmm
+from sensemap_amd.fusion import FusionImage, StereoFusionOptions, fuse
+from tests import test_oracle_fusion as designed
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    assert len(a["score"]) == len(b["score"]), (len(a["score"]), len(b["score"]))
+    for k in ("xyz", "normal", "score"):
+        x, y = a[k].view(np.uint32), b[k].view(np.uint32)
+        n = int(np.sum(x != y))
+        assert n == 0, (k, n)
+    for k in ("rgb", "vis_offsets", "vis_images"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def stats(eng, images, options):
+    import ctypes as C
+    from sensemap_amd.fusion import FusionInputs
+    inp = FusionInputs(images)
+    h = C.c_void_p()
+    abi.check(eng.lib.pm_fusion_run(eng.handle, C.byref(options.to_c()), len(images), inp.images, C.byref(h)))
+    s = abi.pm_fusion_stats()
+    abi.check(eng.lib.pm_fusion_get_stats(h, C.byref(s)))
+    eng.lib.pm_fusion_destroy(h)
+    return s
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_designed_scenes(case):
+    d = designed
+    O = StereoFusionOptions
+    cases = [
+        (d.scene(), O()),
+        (d.scene(), O(max_num_pixels=3, min_num_pixels=3)),
+        (d.scene(depth_scale=[1, 1, 1, 1.02, 1]), O(min_num_pixels=4)),
+        (d.scene(depth_scale=[1, 1, 1, 1.005, 1.008]), O()),
+        (d.scene(normal_tilt=[0, 0, 15.0, 0, 0]), O(min_num_pixels=4, with_normal=False)),
+        (d.scene(shifts=(0, 2, -2, 4, -3.6)), O(min_num_pixels=4, max_reproj_error=0.5)),
+        (d.scene(depth_scale=[1, 1.004, 1, 1, 1], rig=[1, 0, 0, 0, 0]), O()),
+    ]
+    ims, o = cases[case]
+    got, want = fuse(ims, o), binding.fusion(ims, o)
+    print("[fusion] designed case", case, "points", len(got["score"]), flush=True)
+    same(got, want)
+
+
+def orbit_images(seed=8, cams=12, noise=0.002, nbr=6):
+    o = scene.orbit_scene(seed=seed, num_cams=cams, min_px=120 * 90, max_px=200 * 150, grid=32, extent=20.0)
+    rng = np.random.default_rng(seed)
+    ims = []
+    n = len(o.views)
+    for k, v in enumerate(o.views):
+        d = (v.depth * (1 + noise * rng.standard_normal(v.depth.shape))).astype(np.float32)
+        ov = [(k + s) % n for s in (1, -1, 2, -2, 3, -3)][:nbr]
+        ims.append(FusionImage(v.camera.K, v.camera.R, v.camera.T, d, v.normal, v.image, ov))
+    return ims
+
+
+def test_orbit_equals_transcription():
+    eng = pmm.Engine(0)
+    ims = orbit_images()
+    for o in (StereoFusionOptions(), StereoFusionOptions(min_num_pixels=3, max_depth_error=0.02, step_size=2)):
+        got, want = fuse(ims, o, eng), binding.fusion(ims, o)
+        s = stats(eng, ims, o)
+        print("[fusion] orbit points", len(got["score"]), "walks", s.walks, "kept", s.kept, "rounds", s.rounds,
+              "alone", s.big_walks, "ms", round(s.total_ms, 2), flush=True)
+        assert len(got["score"]) > 100
+        same(got, want)
+
+
+def test_walks_that_outgrow_their_buffers():
+    # 36 cameras of one plane, 0.35 px apart: a walk accepts several pixels
+    # of every image (more than a walk record holds)
+    shifts = tuple(0.35 * k for k in range(36))
+    ims = designed.scene(shifts=shifts)
+    o = StereoFusionOptions(max_num_pixels=400)
+    eng = pmm.Engine(0)
+    got, want = fuse(ims, o, eng), binding.fusion(ims, o)
+    s = stats(eng, ims, o)
+    print("[fusion] dense walks: points", len(got["score"]), "alone", s.big_walks, "rounds", s.rounds, flush=True)
+    assert s.big_walks > 0
+    same(got, want)
