@@ -638,6 +638,50 @@ def run_filters_workload(args) -> dict:
             "cpu_ms": round(cpu_ms, 1), "cpu_cores": threads,
             "bitwise_equal": bool(np.array_equal(gc, want_c) and np.array_equal(gd.view(np.uint32), want_d.view(np.uint32))
                                   and np.array_equal(gn.view(np.uint32), want_n.view(np.uint32)))}
+    # deduplication after the cross filter (k_dedup): the 11 refined maps,
+    # every view a filtered reference, in view order
+    from sensemap_amd.cross_filter import dedup
+    dims = [(CrossImage(v.camera.K, v.camera.R, v.camera.T, outs[k], normals[k]), True) for k, v in enumerate(views)]
+    dedup(dims, cviews)
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        dd = dedup(dims, cviews)
+        times.append((time.perf_counter() - t) * 1000.0)
+    t = time.perf_counter()
+    dd_cpu = binding.cross_dedup(dims, cviews, threads=threads)
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    dedup_res = {"views": n, "call_ms": round(float(np.median(times)), 3), "cpu_ms": round(cpu_ms, 1),
+                 "cpu_cores": threads, "mpixel_per_s": round(n * px / 1e6 / (np.median(times) / 1000.0), 1),
+                 "kept": int(sum((v != 0).sum() for v in dd["visibility"])),
+                 "bitwise_equal": bool(all(np.array_equal(a, b) for a, b in zip(dd["visibility"], dd_cpu["visibility"]))
+                                       and all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                                               for key in ("dedup_depth", "delt")
+                                               for a, b in zip(dd[key], dd_cpu[key])))}
+
+    # depth-map fusion (pm_fusion_run) of the 11 refined maps with the
+    # photometric normals and the images' intensities; every view overlaps
+    # the other 10
+    from sensemap_amd.fusion import FusionImage, StereoFusionOptions, fuse
+    fims = [FusionImage(v.camera.K, v.camera.R, v.camera.T, outs[k], normals[k], v.image,
+                        [j for j in range(n) if j != k]) for k, v in enumerate(views)]
+    fo_ = StereoFusionOptions()
+    fuse(fims, fo_)
+    times = []
+    for _ in range(max(2, reps // 2)):
+        t = time.perf_counter()
+        fz = fuse(fims, fo_)
+        times.append((time.perf_counter() - t) * 1000.0)
+    t = time.perf_counter()
+    fz_cpu = binding.fusion(fims, fo_)
+    cpu_ms = (time.perf_counter() - t) * 1000.0
+    fusion_res = {"views": n, "points": int(len(fz["score"])), "call_ms": round(float(np.median(times)), 2),
+                  "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1,
+                  "mpixel_per_s": round(n * px / 1e6 / (np.median(times) / 1000.0), 2),
+                  "bitwise_equal": bool(len(fz["score"]) == len(fz_cpu["score"]) and all(
+                      np.array_equal(fz[k].view(np.uint32) if fz[k].dtype == np.float32 else fz[k],
+                                     fz_cpu[k].view(np.uint32) if fz_cpu[k].dtype == np.float32 else fz_cpu[k])
+                      for k in ("xyz", "normal", "rgb", "score", "vis_offsets", "vis_images")))}
     for e in photos:
         e.close()
     return {"metric": "filter and encoding throughput (Mpixel/s) after the path", "value": cross["mpixel_per_s"],
@@ -645,8 +689,10 @@ def run_filters_workload(args) -> dict:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "1920x1080 heightfield scene, 11 views: RunFilter of the reference (10 sources), "
                                    "cross-view filter of all 11 views (10 sources each), normal encoding, "
-                                   "curvature map + median filter + mask after the geometric pass"},
-            "run_filter": run_filter, "cross_filter": cross, "normal_encoding": enc, "post_steps": post}
+                                   "curvature map + median filter + mask after the geometric pass, deduplication "
+                                   "and depth-map fusion of the 11 refined maps"},
+            "run_filter": run_filter, "cross_filter": cross, "normal_encoding": enc, "post_steps": post,
+            "dedup": dedup_res, "fusion": fusion_res}
 
 
 def main() -> int:

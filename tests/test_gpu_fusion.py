@@ -81,10 +81,10 @@ def test_orbit_equals_transcription():
 
 
 def test_walks_that_outgrow_their_buffers():
-    # 36 cameras of one plane, 0.35 px apart: a walk accepts several pixels
-    # of every image (more than a walk record holds)
-    shifts = tuple(0.35 * k for k in range(36))
-    ims = designed.scene(shifts=shifts)
+    # 100 cameras of one plane, 0.35 px apart: a walk accepts one or more
+    # pixels of every image, more than a walk record holds
+    shifts = tuple(0.35 * k for k in range(100))
+    ims = designed.scene(shifts=shifts, width=96)
     o = StereoFusionOptions(max_num_pixels=400)
     eng = pmm.Engine(0)
     got, want = fuse(ims, o, eng), binding.fusion(ims, o)

@@ -33,22 +33,23 @@ def base_color(wc, r):
     return (wc * 7 + r * 13) % 200 + 20
 
 
-def scene(depth_scale=None, normal_tilt=None, color_off=None, shifts=SHIFTS, rig=None):
+def scene(depth_scale=None, normal_tilt=None, color_off=None, shifts=SHIFTS, rig=None, width=W):
     n = len(shifts)
     ims = []
-    rows, cols = np.mgrid[0:H, 0:W]
+    rows, cols = np.mgrid[0:H, 0:width]
+    cx = (width - 1) / 2.0
     for k, s in enumerate(shifts):
         wc = cols + s  # world column (image-0 units) seen at this pixel
         off = 0 if color_off is None else color_off[k]
         grey = np.clip(base_color(np.floor(wc).astype(int), rows) + off, 0, 255).astype(np.uint8)
-        d = np.full((H, W), Z * (1.0 if depth_scale is None else depth_scale[k]), np.float32)
-        nm = np.zeros((3, H, W), np.float32)
+        d = np.full((H, width), Z * (1.0 if depth_scale is None else depth_scale[k]), np.float32)
+        nm = np.zeros((3, H, width), np.float32)
         nm[2] = -1.0
         if normal_tilt is not None and normal_tilt[k]:
             a = math.radians(normal_tilt[k])
             nm[0], nm[2] = math.sin(a), -math.cos(a)
         T = np.array([-0.1 * s, 0.0, 0.0])
-        ims.append(FusionImage((F, CX, F, CY), np.eye(3), T, d, nm, grey, [j for j in range(n) if j != k],
+        ims.append(FusionImage((F, cx, F, CY), np.eye(3), T, d, nm, grey, [j for j in range(n) if j != k],
                                is_rig=bool(rig and rig[k])))
     return ims
 
