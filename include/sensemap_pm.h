@@ -607,6 +607,23 @@ pm_status pm_model_max_overlapping_images(pm_engine* engine, const pm_model* mod
  * out: num_images x 2 (depth_min, depth_max). */
 pm_status pm_model_depth_ranges(pm_engine* engine, const pm_model* model, float* out);
 
+/* pm_model_depth_ranges without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * All pointers are device memory: d_R num_images x 9 and d_T num_images x 3
+ * (world->camera, row-major), d_xyz num_points x 3, d_out num_images x 2
+ * (depth_min, depth_max), the same values as pm_model_depth_ranges.  The
+ * camera centres and the bounding box live in a fixed device array of the
+ * library (one per GPU, held until the process ends), so num_images is at
+ * most PM_DEPTH_RANGES_NOALLOC_MAX_IMAGES; more is PM_ERR_INVALID_ARG.  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (from
+ * several threads or engines) race.  The work runs on the engine's stream,
+ * which is synchronised before returning; no hipMalloc, hipFree or
+ * allocating library call is made. */
+#define PM_DEPTH_RANGES_NOALLOC_MAX_IMAGES 65535
+pm_status pm_model_depth_ranges_noalloc(pm_engine* engine, int32_t num_images, const float* d_R, const float* d_T,
+                                        int64_t num_points, const float* d_xyz, float* d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 
 PM_MAX_NUM_SRC = 32
 PM_POSE_STRIDE = 44
+PM_DEPTH_RANGES_NOALLOC_MAX_IMAGES = 65535
 
 PM_OK = 0
 (PM_ERR_INVALID_ARG, PM_ERR_OUT_OF_MEMORY, PM_ERR_DEVICE, PM_ERR_TOO_MANY_SOURCES, PM_ERR_NOT_READY,
@@ -297,6 +298,8 @@ _SIGNATURES = {
                                        C.c_void_p]),
     "pm_fusion_destroy": (None, [C.c_void_p]),
     "pm_fusion_get_stats": (C.c_int, [C.c_void_p, C.POINTER(pm_fusion_stats)]),
+    "pm_model_depth_ranges_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                                C.c_void_p, C.c_void_p]),
     "pm_cross_dedup": (C.c_int, [C.c_void_p, C.POINTER(pm_cross_options), C.c_int32, C.POINTER(pm_dedup_image),
                                  C.c_int32, C.POINTER(pm_dedup_view)]),
     "pm_model_max_overlapping_images": (C.c_int, [C.c_void_p, C.POINTER(pm_model), C.c_int32, C.c_double,

@@ -78,7 +78,7 @@ __host__ __device__ inline float dot3f(float a0, float a1, float a2, float b0, f
 __host__ __device__ inline double sq3(double x, double y, double z) { return (x * x + y * y) + z * z; }
 
 // ComputeProjectionCenter (mvs/utils.cc:28-33): C = -R^T T in FP32.
-void projection_center(const float* R, const float* T, float C[3]) {
+__host__ __device__ void projection_center(const float* R, const float* T, float C[3]) {
   for (int i = 0; i < 3; ++i) C[i] = -dot3f(R[i], R[3 + i], R[6 + i], T[0], T[1], T[2]);
 }
 
@@ -644,3 +644,83 @@ PM_API pm_status pm_model_depth_ranges(pm_engine* e, const pm_model* m, float* o
   return PM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// pm_model_depth_ranges_noalloc: the same ranges from device-resident inputs,
+// with the camera centres and the bounding box in the library's fixed device
+// array `scratch` instead of allocated buffers.  Layout of `scratch` (32-bit
+// words): box at [0, 6) (order-preserving min x, y, z, then max x, y, z),
+// camera centres at [kScratchCentres, kScratchCentres + 3 * num_images).
+
+namespace {
+
+constexpr int32_t kNoAllocMaxImages = PM_DEPTH_RANGES_NOALLOC_MAX_IMAGES;
+constexpr size_t kScratchCentres = 64;  // 256 bytes: the centres start on their own line
+constexpr size_t kScratchWords = kScratchCentres + 3 * (size_t)kNoAllocMaxImages;
+
+// Each GPU holds its own copy until the process ends.
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+// The per-device address of `scratch` (the host-side name is a stand-in and
+// must never reach a kernel), looked up once per device.
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[256] = {};
+
+pm_status scratch_address(int device, uint32_t** out) {
+  if (device < 0 || device >= 256) return fail(PM_ERR_INVALID_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PM_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = (uint32_t*)g_scratch_ptr[device];
+  return PM_OK;
+}
+
+// The box starts empty (a kernel, not a memset).
+__global__ void k_box_init(uint32_t* __restrict__ box) {
+  if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_centres(const float* __restrict__ R, const float* __restrict__ T, int32_t I,
+                                                    float* __restrict__ centres) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= I) return;
+  float c[3];
+  projection_center(R + 9 * (size_t)i, T + 3 * (size_t)i, c);
+  for (int k = 0; k < 3; ++k) centres[3 * (size_t)i + k] = c[k];
+}
+
+}  // namespace
+
+PM_API pm_status pm_model_depth_ranges_noalloc(pm_engine* e, int32_t num_images, const float* d_R, const float* d_T,
+                                               int64_t num_points, const float* d_xyz, float* d_out) {
+  if (!e || !d_R || !d_T || !d_out || num_points < 0 || (num_points > 0 && !d_xyz))
+    return fail(PM_ERR_INVALID_ARG, "invalid depth-range arguments");
+  if (num_images < 1 || num_images > kNoAllocMaxImages)
+    return fail(PM_ERR_INVALID_ARG, "num_images must be in [1, " + std::to_string(kNoAllocMaxImages) +
+                                        "] (the size of the fixed device array)");
+  PM_HIP(hipSetDevice(e->device));
+  uint32_t* words = nullptr;
+  if (pm_status s = scratch_address(e->device, &words)) return s;
+  hipStream_t st = e->stream;
+  uint32_t* box = words;
+  float* centres = (float*)(words + kScratchCentres);
+  hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, st, box);
+  PM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_centres, dim3((num_images + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d_R, d_T, num_images,
+                     centres);
+  PM_HIP(hipGetLastError());
+  const int64_t n = num_points + num_images;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 2048);
+  hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(kBlock), 0, st, d_xyz, num_points, (const float*)centres, num_images,
+                     box);
+  PM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_depth_ranges, dim3((num_images + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     (const uint32_t*)box, d_R, d_T, num_images, d_out);
+  PM_HIP(hipGetLastError());
+  PM_HIP(hipStreamSynchronize(st));
+  return PM_OK;
+}
