@@ -859,6 +859,39 @@ def main() -> int:
                 "mpixel_iter_per_s": round(full_mpix / full_s, 3), "views_per_s": round(n_views / full_s, 4),
                 "depth_equal_to_step": bool(np.array_equal(geo.GetDepthMap().view(np.uint32),
                                                            step_depth.view(np.uint32)))}
+    # the same pipeline with the 11 photometric passes advanced together in
+    # shared launches (pm_batch, hipGraph replay), the geometric pass fed from
+    # the batch's views on the device
+    full_batched = None
+    if not args.only_photometric:
+        items = [(photo_opts, pmm.Problem(k, [j for j in range(n_views) if j != k], images, sc.depth_min,
+                                          sc.depth_max)) for k in range(n_views)]
+        fb = pmm.PatchMatchBatch(items, engine=engine, graph=True)
+
+        def batched_pipeline():
+            fb.Run()
+            for i in range(1, n_views):
+                geo.SetSrcDepthFrom(i - 1, fb.views[i])
+            geo.InitFrom(fb.views[0])
+            geo.Run()
+
+        batched_pipeline()  # graph capture
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            batched_pipeline()
+            runs.append(time.perf_counter() - t)
+        fb_s = float(np.median(runs))
+        full_batched = {"workload": "the same 11 photometric passes in one batch (shared persistent launches, "
+                                    "hipGraph replay), then the reference's geometric pass",
+                        "seconds": round(fb_s, 3), "runs_s": [round(x, 3) for x in runs],
+                        "mpixel_iter_per_s": round(full_mpix / fb_s, 3), "views_per_s": round(n_views / fb_s, 4),
+                        "speedup_vs_one_view_per_launch": round(full_s / fb_s, 3),
+                        "depth_equal_to_step": bool(np.array_equal(geo.GetDepthMap().view(np.uint32),
+                                                                   step_depth.view(np.uint32)))}
+        fb.close()
+        for i, p in enumerate(src_photo):  # restore the geometric pass's inputs
+            geo.SetSrcDepthFrom(i, p)
     # the reference's flow for the same step: each pass's problem created from
     # host buffers (images; for the geometric pass the sources' depth maps and
     # the photometric maps as its initial state), run, maps read back
@@ -913,6 +946,7 @@ def main() -> int:
         "other_storage": other,
         "fast_reciprocal": fast,
         "full_pipeline": full,
+        "full_pipeline_batched": full_batched,
         "host_buffers": host,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),

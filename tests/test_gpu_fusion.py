@@ -81,11 +81,12 @@ def test_orbit_equals_transcription():
 
 
 def test_walks_that_outgrow_their_buffers():
-    # 100 cameras of one plane, 0.35 px apart: a walk accepts one or more
-    # pixels of every image, more than a walk record holds
-    shifts = tuple(0.35 * k for k in range(100))
-    ims = designed.scene(shifts=shifts, width=96)
-    o = StereoFusionOptions(max_num_pixels=400)
+    # 24 cameras of one plane, 0.35 px apart, and a 4-pixel reprojection
+    # bound: a walk accepts a disc of pixels in every image, more than a walk
+    # graph holds, so those walks run alone
+    shifts = tuple(0.35 * k for k in range(24))
+    ims = designed.scene(shifts=shifts, width=64)
+    o = StereoFusionOptions(max_reproj_error=4.0)
     eng = pmm.Engine(0)
     got, want = fuse(ims, o, eng), binding.fusion(ims, o)
     s = stats(eng, ims, o)
