@@ -558,6 +558,7 @@ def run_filters_workload(args) -> dict:
                   "kept": int((got > 0).sum()), "bitwise_equal": bool(np.array_equal(got.view(np.uint32),
                                                                                      want.view(np.uint32)))}
 
+    progress("cross-view filter")
     # cross-view filter (k_cross_filter): all 11 views against the other 10
     ims = [CrossImage(v.camera.K, v.camera.R, v.camera.T, depths[k], normals[k]) for k, v in enumerate(views)]
     cviews = [(k, [j for j in range(n) if j != k]) for k in range(n)]
@@ -587,6 +588,7 @@ def run_filters_workload(args) -> dict:
              "bitwise_equal": bool(all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
                                        for a, b in zip(outs, couts)))}
 
+    progress("normal encoding")
     # NormalMap encoding on the device (k_encode_normals + read back)
     ref_photo.GetNormalCodes()
     times = []
@@ -605,6 +607,7 @@ def run_filters_workload(args) -> dict:
     # k_median, k_mask), timed on the device (pm_run_stats.post_ms) and
     # checked against the transcription's chain applied to the same pass's
     # maps without those steps
+    progress("steps after the geometric pass")
     po = c2_options(pmm, True, abi.PM_RCP_EXACT)
     rng = np.random.default_rng(3)
     mask = (rng.random((H, W)) > 0.1).astype(np.uint8) * 255
@@ -638,6 +641,7 @@ def run_filters_workload(args) -> dict:
             "cpu_ms": round(cpu_ms, 1), "cpu_cores": threads,
             "bitwise_equal": bool(np.array_equal(gc, want_c) and np.array_equal(gd.view(np.uint32), want_d.view(np.uint32))
                                   and np.array_equal(gn.view(np.uint32), want_n.view(np.uint32)))}
+    progress("deduplication")
     # deduplication after the cross filter (k_dedup): the 11 refined maps,
     # every view a filtered reference, in view order
     from sensemap_amd.cross_filter import dedup
@@ -659,23 +663,32 @@ def run_filters_workload(args) -> dict:
                                                for key in ("dedup_depth", "delt")
                                                for a, b in zip(dd[key], dd_cpu[key])))}
 
+    progress("depth-map fusion")
     # depth-map fusion (pm_fusion_run) of the 11 refined maps with the
     # photometric normals and the images' intensities; every view overlaps
     # the other 10
-    from sensemap_amd.fusion import FusionImage, StereoFusionOptions, fuse
+    from sensemap_amd.fusion import FusionImage, StereoFusionOptions, fuse, fusion_stats
     fims = [FusionImage(v.camera.K, v.camera.R, v.camera.T, outs[k], normals[k], v.image,
                         [j for j in range(n) if j != k]) for k, v in enumerate(views)]
     fo_ = StereoFusionOptions()
+    t = time.perf_counter()
     fuse(fims, fo_)
+    progress(f"fusion first call {time.perf_counter() - t:.2f} s")
     times = []
     for _ in range(max(2, reps // 2)):
         t = time.perf_counter()
         fz = fuse(fims, fo_)
         times.append((time.perf_counter() - t) * 1000.0)
+        progress(f"fusion call {times[-1]:.1f} ms")
     t = time.perf_counter()
     fz_cpu = binding.fusion(fims, fo_)
     cpu_ms = (time.perf_counter() - t) * 1000.0
+    progress(f"fusion transcription {cpu_ms:.1f} ms")
+    fst = fusion_stats(fims, fo_)
     fusion_res = {"views": n, "points": int(len(fz["score"])), "call_ms": round(float(np.median(times)), 2),
+                  "walks": int(fst.walks), "walks_with_pixels": int(fst.kept), "dataflow_launches": int(fst.rounds),
+                  "walks_run_alone": int(fst.big_walks), "longest_chains": int(fst.chain),
+                  "dataflow_ms": round(float(fst.flow_ms), 1),
                   "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1,
                   "mpixel_per_s": round(n * px / 1e6 / (np.median(times) / 1000.0), 2),
                   "bitwise_equal": bool(len(fz["score"]) == len(fz_cpu["score"]) and all(

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 5
+#define PM_ABI_VERSION 6
 #define PM_MAX_NUM_SRC 32      /* the reference caps at MAX_NUM_SRC_IMAGE = 8 */
 #define PM_MAX_WINDOW_RADIUS 32
 #define PM_POSE_STRIDE 44      /* floats of packed pose per source view */
@@ -540,11 +540,16 @@ pm_status pm_fusion_get_counts(const pm_fusion* fusion, int64_t* num_points, int
  * order.  Any output may be NULL. */
 pm_status pm_fusion_get_points(const pm_fusion* fusion, float* xyz, float* normal, uint8_t* rgb, float* score,
                                int64_t* vis_offsets, int32_t* vis_images);
-/* How the device got there: walks run (speculative and kept), rounds,
- * walks rerun alone with large buffers, points, wall time of the call. */
+/* How the device got there: walks run, walks that accepted pixels,
+ * dataflow launches, walks run alone (their graphs outgrew the device's
+ * per-walk buffers), points, wall time of the call; the longest chains of
+ * walks that waited one for another, summed over the windows, and the time
+ * the dataflow launches took. */
 typedef struct pm_fusion_stats {
   int64_t walks, kept, rounds, big_walks, points;
   double total_ms;
+  int64_t chain;
+  double flow_ms;
 } pm_fusion_stats;
 pm_status pm_fusion_get_stats(const pm_fusion* fusion, pm_fusion_stats* out);
 void pm_fusion_destroy(pm_fusion* fusion);

@@ -214,7 +214,7 @@ class pm_fusion_image(C.Structure):
 
 class pm_fusion_stats(C.Structure):
     _fields_ = [("walks", C.c_int64), ("kept", C.c_int64), ("rounds", C.c_int64), ("big_walks", C.c_int64),
-                ("points", C.c_int64), ("total_ms", C.c_double)]
+                ("points", C.c_int64), ("total_ms", C.c_double), ("chain", C.c_int64), ("flow_ms", C.c_double)]
 
 
 class pm_model(C.Structure):

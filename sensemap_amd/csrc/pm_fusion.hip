@@ -5,34 +5,40 @@
 //
 // The walks are sequential by definition: a pixel belongs to the first walk
 // that accepts it, and each seed's walk sees the pixels earlier walks took.
-// The device runs them speculatively instead, a window of seeds at a time,
-// one thread per walk, each walk against the masks committed so far (plus
-// its own accepted pixels), and then keeps exactly the walks whose result
-// the sequential order would give:
+// The device keeps that order exactly while it runs walks side by side:
 //
-//   * A walk's result depends on the masks only through the pixels it would
-//     accept: a popped pixel that is masked and one that fails the
-//     consistency tests both end in `continue` with no other effect.
-//   * So a walk is exact when no earlier walk of the round accepted any of
-//     the pixels it accepted (one atomicMin per accepted pixel finds the
-//     earliest walk that accepted it).
-//   * A walk that is not exact reruns in the next round, when every earlier
-//     walk has committed.  Its rerun accepts a subset of what it accepted
-//     speculatively (the masks only grew, and the accepted set is the
-//     closure of the seed through the unmasked consistent pixels), so a later
-//     walk that is exact now stays exact -- unless the speculative walk was
-//     cut by max_traversal_depth or max_num_pixels (then a rerun may go
-//     elsewhere): a round keeps no walk after such a cut walk that is not
-//     exact.
-//   * The first walk of a round is always exact, so every round commits at
-//     least one walk; a walk that outgrew its record buffers reruns alone
-//     with buffers sized for max_num_pixels.
+//   * Graphs.  Whether a pixel passes a walk's tests depends only on the
+//     seed (its point and normal) and on the pixel, and the pixels a passing
+//     pixel pushes depend only on the pixel.  k_graph records, one wave per
+//     seed of a window, every pixel the walk could reach against the masks
+//     of the window's start (breadth first; a masked pixel stays masked),
+//     whether it passes, its values and its pushes in order.  Replaying the
+//     depth-first walk over the graph against later masks gives the same
+//     pixels, order and cuts as walking the images again.
+//   * Dependencies.  A walk's result depends on the masks only at the
+//     passing pixels of its graph, and a walk sets masks only at pixels it
+//     accepts, which pass in its own graph.  So a walk may run once, for
+//     each passing pixel of its graph, the last earlier walk whose graph
+//     passes that pixel has finished (that walk waited in turn for the ones
+//     before it).  The walks whose graphs pass each pixel are listed by a
+//     count, a prefix sum and a scatter.
+//   * Dataflow.  k_flow hands the walks out in seed order through an atomic
+//     ticket, so every walk a wave waits for is already held by a running
+//     wave, and the waits end.  A wave replays its walk against the masks it
+//     sees before it waits; after the wait, if none of the pixels it
+//     accepted has been taken meanwhile, that replay is the walk's result
+//     (masks only grow, and a popped pixel that is masked ends in `continue`
+//     as one that fails its tests does), else it replays again.  It then
+//     sets its pixels' mask bits and publishes its completion with release /
+//     acquire at agent scope, since the waves run on every XCD.  Each wait
+//     is bounded in time; one that runs out fails the call.
+//   * A graph that outgrows the wave's LDS marks its walk to run alone
+//     (k_walk, pixel by pixel, with buffers for max_num_pixels) between two
+//     dataflow launches.
 //
-// Per walk the kernel keeps the accepted pixels' values in its record, in
-// visiting order, and computes the point there (medians by rank, the mean
-// gradient in visiting order, the score).  Committing sets the masks and
-// copies the point into its seed's slot; the host collects the slots of a
-// window in seed order.
+// The points (medians by rank, the mean gradient in visiting order, the
+// score) are formed afterwards, one wave per walk that accepted pixels, and
+// gathered in seed order on the device.
 #include "pm_host.h"
 
 #include <algorithm>
@@ -48,19 +54,20 @@ using namespace pmh;
 
 namespace {
 
-constexpr int kBlock = 128;
-constexpr int kWindow = 65536;   // walks in flight
-constexpr int kAccCap = 128;     // accepted pixels per walk record
-constexpr int kStackCap = 2048;  // walk stack entries
-constexpr int kVals = 11;        // x y z nx ny nz r g b grad err
-
-enum : int32_t { kOverflow = 1, kCut = 2 };
+constexpr int kVals = 11;                         // x y z nx ny nz r g b grad err
+constexpr int kNodeCap = 256;                     // passing pixels per stored walk graph
+constexpr int kChildCap = 2048;                   // pushes per stored walk graph
+constexpr int kProvCap = 512;                     // pixels (passing or not) met building a graph
+constexpr int kProvChild = 4096;                  // push slots met building a graph
+constexpr int kHash = 1024;                       // open-addressing slots (gid -> pixel)
+constexpr int64_t kMaxWindow = int64_t(1) << 23;  // seeds per window
+constexpr int kFlowWavesPerCU = 8;
+constexpr double kWaitSeconds = 4.0;  // one wait's bound
 
 struct FusImgDev {
   const float* depth;
   const float* normal;  // null without normals
   const uint8_t* rgb;
-  uint8_t* mask;
   const int32_t* ov;
   int64_t base;  // first global pixel id of the image
   int32_t W, H, nov, used;
@@ -76,42 +83,44 @@ struct FusParams {
   double max_depth_error;
 };
 
+// An accepted pixel of a walk run alone: its image and values.
 struct AccEntry {
   int32_t img, pad;
   float v[kVals];
 };
 
 struct WalkRec {
-  int32_t flags, n_acc, has_point, nvis;
-  float xyz[3], nrm[3], rgb[3], score;
+  int32_t n_acc, pad;
 };
 
 struct StackEntry {
   int32_t img, row, col, depth;
 };
 
+// A fused point as formed on the device.
+struct PointRec {
+  float xyz[3], nrm[3], rgb[3], score;
+  int32_t nvis, pad;
+};
+
+// The mask: one bit per pixel of every image (global pixel ids).
+__device__ inline bool masked(const uint32_t* __restrict__ mask, int64_t gid) {
+  return (mask[gid >> 5] >> (gid & 31)) & 1u;
+}
+
 __device__ inline float row4(const float* m, float x, float y, float z) { return ((m[0] * x + m[1] * y) + m[2] * z) + m[3]; }
 
-// Fuse (fusion.cc:1794-2182) from one seed, against the committed masks:
-// the accepted pixels (ids in gids, values in acc) in visiting order.  The
-// point is formed later, for the walks that are kept.
-__global__ void __launch_bounds__(kBlock) k_walk(const FusImgDev* __restrict__ imgs,
-                                                 const uint8_t* __restrict__ fused, FusParams prm,
-                                                 const int32_t* __restrict__ seeds, int32_t n, WalkRec* __restrict__ recs,
-                                                 int64_t* __restrict__ gids, AccEntry* __restrict__ acc,
-                                                 StackEntry* __restrict__ stacks) {
-  const int w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= n) return;
-  int64_t* G = gids + (size_t)w * prm.acc_cap;
-  AccEntry* A = acc + (size_t)w * prm.acc_cap;
-  StackEntry* S = stacks + (size_t)w * prm.stack_cap;
-  WalkRec rec;
-  rec.flags = 0;
-  rec.has_point = 0;
-  rec.nvis = 0;
-  const int s = seeds[w];
-  const int seed_row = 3 + (s / prm.seed_cols) * prm.step;
-  const int seed_col = 3 + (s % prm.seed_cols) * prm.step;
+// Fuse (fusion.cc:1794-2182) from one seed, against the committed masks,
+// pixel by pixel: the accepted pixels (ids in G, values in A) in visiting
+// order.  Run for one walk at a time, with buffers for max_num_pixels
+// accepted pixels and for every push they can make.
+__global__ void __launch_bounds__(64) k_walk(const FusImgDev* __restrict__ imgs, const uint32_t* __restrict__ mask,
+                                             const uint8_t* __restrict__ fused, FusParams prm, int32_t seed,
+                                             WalkRec* __restrict__ rec_out, int64_t* __restrict__ G,
+                                             AccEntry* __restrict__ A, StackEntry* __restrict__ S) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int seed_row = 3 + (seed / prm.seed_cols) * prm.step;
+  const int seed_col = 3 + (seed % prm.seed_cols) * prm.step;
   int sp = 0, na = 0;
   S[sp++] = {prm.cur, seed_row, seed_col, 0};
   float ref_pt[3] = {0, 0, 0}, ref_n[3] = {0, 0, 0};
@@ -120,8 +129,8 @@ __global__ void __launch_bounds__(kBlock) k_walk(const FusImgDev* __restrict__ i
     const FusImgDev& I = imgs[e.img];
     const int W = I.W, H = I.H;
     const int64_t p = (int64_t)e.row * W + e.col;
-    if (I.mask[p]) continue;
     const int64_t gid = I.base + p;
+    if (masked(mask, gid)) continue;
     bool mine = false;  // taken earlier by this walk
     for (int j = 0; j < na && !mine; ++j) mine = G[j] == gid;
     if (mine) continue;
@@ -157,10 +166,6 @@ __global__ void __launch_bounds__(kBlock) k_walk(const FusImgDev* __restrict__ i
       const float gx = (float)(l - r) / 255.0f, gy = (float)(t - b) / 255.0f;
       grad = sqrtf(gx * gx + gy * gy);
     }
-    if (na == prm.acc_cap) {
-      rec.flags |= kOverflow;
-      break;
-    }
     G[na] = gid;
     AccEntry& a = A[na++];
     a.img = e.img;
@@ -183,17 +188,12 @@ __global__ void __launch_bounds__(kBlock) k_walk(const FusImgDev* __restrict__ i
       ref_n[1] = nrm[1];
       ref_n[2] = nrm[2];
     }
-    if (na >= prm.max_num_pixels) {
-      rec.flags |= kCut;
-      break;
-    }
+    if (na >= prm.max_num_pixels) break;
     const int nd = e.depth + 1;
-    if (nd >= prm.max_traversal) {
-      rec.flags |= kCut;
-      continue;
-    }
-    bool full = false;
-    for (int k = 0; k < I.nov && !full; ++k) {
+    if (nd >= prm.max_traversal) continue;
+    // at most one push per (accepted pixel, overlapping image) plus the
+    // seed: the stack (stack_cap entries) never fills
+    for (int k = 0; k < I.nov && sp < prm.stack_cap; ++k) {
       const int j = I.ov[k];
       if (j < 0 || j >= prm.n_images) continue;
       const FusImgDev& J = imgs[j];
@@ -205,57 +205,21 @@ __global__ void __launch_bounds__(kBlock) k_walk(const FusImgDev* __restrict__ i
       if (!(fabsf(fc) < 1073741824.0f) || !(fabsf(fr) < 1073741824.0f)) continue;
       const int nc = (int)fc, nr = (int)fr;
       if (nc < 3 || nr < 3 || nc >= J.W - 3 || nr >= J.H - 3) continue;
-      if (sp == prm.stack_cap) {
-        full = true;
-        break;
-      }
       S[sp++] = {j, nr, nc, nd};
     }
-    if (full) {
-      rec.flags |= kOverflow;
-      break;
-    }
   }
-  rec.n_acc = na;
-  recs[w] = rec;
+  rec_out->n_acc = na;
 }
 
-// ---------------------------------------------------------------------------
-// The walk as a graph.  Whether a pixel passes the walk's tests depends only
-// on the seed (its point and normal) and on the pixel, and the pixels a
-// passing pixel pushes depend only on the pixel; the masks decide which of
-// them the depth-first walk takes.  k_graph therefore records, once per
-// window and one wave per seed, every pixel the walk could reach (breadth
-// first, with the masks of the window's start: masked pixels stay masked),
-// whether it passes, its values and its pushes in order.  Each round,
-// k_replay replays the depth-first walk over that graph in LDS against the
-// masks of the round -- the same pixels, order and cuts as walking the
-// images again, without touching them.  A graph that does not fit is marked
-// and its walk runs alone through k_walk.
-
-constexpr int kNodeCap = 256;    // pixels per walk graph
-constexpr int kChildCap = 2048;  // pushes per walk graph
-constexpr int kHash = 512;       // open-addressing slots (gid -> node)
-
-struct Graph {  // structure of arrays over the window's seeds
-  int32_t* n;        // nodes; -1 when the graph did not fit
-  int32_t* nc;       // child slots used
-  int64_t* gid;      // [seed][kNodeCap]
-  int32_t* img;      // [seed][kNodeCap]
-  uint8_t* pass;     // [seed][kNodeCap]
-  int16_t* cbase;    // [seed][kNodeCap]: first child slot, -1 when not expanded
-  uint8_t* ccnt;     // [seed][kNodeCap]: child slots (the overlapping list's length)
-  int16_t* child;    // [seed][kChildCap]: node index or -1 (no push)
-  float* val;        // [seed][kNodeCap][kVals]
-};
-
 // the tests and values of one pixel against the seed's point and normal
-// (fusion.cc:1884-2040); returns whether it passes
-__device__ bool test_pixel(const FusImgDev& I, int row, int col, bool seed, const float ref_pt[3], const float ref_n[3],
-                           const FusParams& prm, float* v) {
+// (fusion.cc:1884-2040); returns whether it passes.  Without check_mask the
+// mask is not read (the values of a pixel already accepted).
+__device__ bool test_pixel(const FusImgDev& I, const uint32_t* __restrict__ mask, int row, int col, bool seed,
+                           const float ref_pt[3], const float ref_n[3], const FusParams& prm, float* v,
+                           bool check_mask = true) {
   const int W = I.W, H = I.H;
   const int64_t p = (int64_t)row * W + col;
-  if (I.mask[p]) return false;
+  if (check_mask && masked(mask, I.base + p)) return false;
   const float depth = I.depth[p];
   if (depth <= 0.0f) return false;
   float depth_error = 0.0f;
@@ -301,84 +265,92 @@ __device__ bool test_pixel(const FusImgDev& I, int row, int col, bool seed, cons
   return true;
 }
 
+// The window's walk graphs, packed one after another.  A stored graph holds
+// only the pixels that pass the walk's tests (node 0 is the seed), and per
+// pixel only its pushes of passing pixels, in push order: a pushed pixel
+// that fails ends in `continue` when popped and changes nothing.
+struct Pool {
+  int32_t* n;          // [seed] nodes; 0: the seed fails; -1: outgrew LDS; -2: no room in the pool
+  int32_t* nc;         // [seed] pushes
+  int32_t* node_off;   // [seed] first node
+  int32_t* child_off;  // [seed] first push
+  int64_t* gid;        // [node]
+  int32_t* img;        // [node]
+  int16_t* cbase;      // [node] first push within the graph's; -1 when not expanded
+  int16_t* ccnt;       // [node] pushes
+  int16_t* child;      // [push] node index
+  unsigned long long* used;  // [2] nodes and pushes handed out
+  int32_t* no_room;          // first seed of the window that found no room
+  int64_t node_cap, child_cap;
+};
+
+__device__ inline int hash_slot(int64_t gid) { return (int)((uint64_t)gid * 0x9E3779B97F4A7C15ull >> 54); }
+
 // One wave (one block of 64) per seed of the window.
-__global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs, const uint8_t* __restrict__ fused,
-                                              FusParams prm, int32_t window_first, int32_t count, Graph g) {
+__global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs, const uint32_t* __restrict__ mask,
+                                              const uint8_t* __restrict__ fused, FusParams prm, int32_t window_first,
+                                              int32_t count, Pool g) {
   __shared__ int64_t hkey[kHash];
   __shared__ int16_t hval[kHash];
-  __shared__ int32_t nimg[kNodeCap];
-  __shared__ int32_t nrc[kNodeCap];  // row << 16 | col
-  __shared__ uint8_t npass[kNodeCap], ndep[kNodeCap];
-  __shared__ int16_t cbase[kNodeCap];
-  __shared__ uint8_t ccnt[kNodeCap];
-  __shared__ float nxyz[kNodeCap][3];
-  __shared__ int16_t child[kChildCap];
-  __shared__ int16_t front[kNodeCap];
+  __shared__ uint16_t nimg[kProvCap];
+  __shared__ int32_t nrc[kProvCap];  // row << 16 | col
+  __shared__ uint8_t npass[kProvCap];
+  __shared__ int16_t cbase[kProvCap], ccnt[kProvCap];
+  __shared__ float nxyz[kProvCap][3];
+  __shared__ int16_t child[kProvChild];
+  __shared__ int16_t front[kProvCap];  // the frontier; after the walk, each pixel's node (-1: fails)
+  __shared__ int16_t vfirst[kProvCap];  // first stored push of each pixel
+  __shared__ long long offs[2];
   const int ws = blockIdx.x, lane = threadIdx.x;
   if (ws >= count) return;
   const int s = window_first + ws;
   const int seed_row = 3 + (s / prm.seed_cols) * prm.step;
   const int seed_col = 3 + (s % prm.seed_cols) * prm.step;
-  const size_t nb = (size_t)ws * kNodeCap;
-  for (int i = lane; i < kHash; i += 64) hkey[i] = -1;
-  __syncthreads();
-  // the seed (node 0)
+  // the seed (pixel 0); a seed that fails has an empty walk and no graph
   float ref_pt[3], ref_n[3];
   {
     const FusImgDev& I = imgs[prm.cur];
     float v[kVals];
-    const bool ok = test_pixel(I, seed_row, seed_col, true, nullptr, nullptr, prm, v);
-    const int64_t gid = I.base + (int64_t)seed_row * I.W + seed_col;
-    if (lane == 0) {
-      nimg[0] = prm.cur;
-      nrc[0] = (seed_row << 16) | seed_col;
-      npass[0] = ok ? 1 : 0;
-      ndep[0] = 0;
-      const int h = (int)((uint64_t)gid * 0x9E3779B97F4A7C15ull >> 55);
-      hkey[h] = gid;
-      hval[h] = 0;
-      if (ok)
-        for (int k = 0; k < kVals; ++k) g.val[nb * kVals + k] = v[k];
+    if (!test_pixel(I, mask, seed_row, seed_col, true, nullptr, nullptr, prm, v)) {
+      if (lane == 0) g.n[ws] = 0;
+      return;
     }
     for (int k = 0; k < 3; ++k) {
       ref_pt[k] = v[k];
       ref_n[k] = v[3 + k];
-      if (lane == 0 && ok) nxyz[0][k] = v[k];
     }
-    if (!ok) {
-      if (lane == 0) {
-        g.n[ws] = 1;
-        g.nc[ws] = 0;
-        g.gid[nb] = gid;
-        g.img[nb] = prm.cur;
-        g.pass[nb] = 0;
-        g.cbase[nb] = -1;
-        g.ccnt[nb] = 0;
-      }
-      return;
+    for (int i = lane; i < kHash; i += 64) hkey[i] = -1;
+    __syncthreads();
+    if (lane == 0) {
+      const int64_t gid = I.base + (int64_t)seed_row * I.W + seed_col;
+      nimg[0] = (uint16_t)prm.cur;
+      nrc[0] = (seed_row << 16) | seed_col;
+      npass[0] = 1;
+      const int h = hash_slot(gid);
+      hkey[h] = gid;
+      hval[h] = 0;
+      for (int k = 0; k < 3; ++k) nxyz[0][k] = v[k];
+      cbase[0] = -1;
+      ccnt[0] = 0;
+      front[0] = 0;
     }
   }
-  int nn = 1, nch = 0, nf = 0, level = 0;
+  int nn = 1, nch = 0, nf = 1 < prm.max_traversal ? 1 : 0, level = 0;
   bool overflow = false;
-  if (1 < prm.max_traversal) {
-    if (lane == 0) front[0] = 0;
-    nf = 1;
-  }
-  if (lane == 0) cbase[0] = -1, ccnt[0] = 0;
   __syncthreads();
   while (nf > 0 && !overflow) {
     const int level_start = nn;
     for (int fi = 0; fi < nf && !overflow; ++fi) {
-      const int f = front[fi];
-      const FusImgDev& F = imgs[nimg[f]];
-      const int nov = F.nov;
-      if (nch + nov > kChildCap) {
+      const int f = __builtin_amdgcn_readfirstlane(front[fi]);
+      const FusImgDev& F = imgs[__builtin_amdgcn_readfirstlane(nimg[f])];
+      const int nov = __builtin_amdgcn_readfirstlane(F.nov);
+      if (nch + nov > kProvChild) {
         overflow = true;
         break;
       }
       if (lane == 0) {
         cbase[f] = (int16_t)nch;
-        ccnt[f] = (uint8_t)nov;
+        ccnt[f] = (int16_t)nov;
       }
       const float x = nxyz[f][0], y = nxyz[f][1], z = nxyz[f][2];
       for (int k0 = 0; k0 < nov && !overflow; k0 += 64) {
@@ -404,23 +376,24 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
         // insertion in push order, one candidate at a time (uniform)
         const int kn = min(64, nov - k0);
         for (int l = 0; l < kn; ++l) {
-          const int j = __shfl(cj, l), r = __shfl(crow, l), c = __shfl(ccol, l);
+          const int j = __builtin_amdgcn_readfirstlane(__shfl(cj, l));
+          const int r = __builtin_amdgcn_readfirstlane(__shfl(crow, l));
+          const int c = __builtin_amdgcn_readfirstlane(__shfl(ccol, l));
           int idx = -1;
           if (j >= 0) {
             const int64_t gid = imgs[j].base + (int64_t)r * imgs[j].W + c;
-            int h = (int)((uint64_t)gid * 0x9E3779B97F4A7C15ull >> 55);
+            int h = hash_slot(gid);
             while (hkey[h] != -1 && hkey[h] != gid) h = (h + 1) & (kHash - 1);
             if (hkey[h] == gid) {
               idx = hval[h];
-            } else if (nn < kNodeCap) {
+            } else if (nn < kProvCap) {
               idx = nn++;
               __syncthreads();
               if (lane == 0) {
                 hkey[h] = gid;
                 hval[h] = (int16_t)idx;
-                nimg[idx] = j;
+                nimg[idx] = (uint16_t)j;
                 nrc[idx] = (r << 16) | c;
-                ndep[idx] = (uint8_t)(level + 1);
               }
               __syncthreads();
             } else {
@@ -436,7 +409,7 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
     }
     if (overflow) break;
     // test the new pixels (in parallel); the passing ones below the depth
-    // cap form the next frontier, in node order
+    // cap form the next frontier, in pixel order
     int nnext = 0;
     for (int b = level_start; b < nn; b += 64) {
       const int i = b + lane;
@@ -444,14 +417,10 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
       if (i < nn) {
         float v[kVals];
         const int rc = nrc[i];
-        ok = test_pixel(imgs[nimg[i]], rc >> 16, rc & 0xffff, false, ref_pt, ref_n, prm, v);
+        ok = test_pixel(imgs[nimg[i]], mask, rc >> 16, rc & 0xffff, false, ref_pt, ref_n, prm, v);
         npass[i] = ok ? 1 : 0;
-        if (ok) {
-          for (int k = 0; k < kVals; ++k) g.val[(nb + i) * kVals + k] = v[k];
-          nxyz[i][0] = v[0];
-          nxyz[i][1] = v[1];
-          nxyz[i][2] = v[2];
-        }
+        if (ok)
+          for (int k = 0; k < 3; ++k) nxyz[i][k] = v[k];
         cbase[i] = -1;
         ccnt[i] = 0;
       }
@@ -465,172 +434,445 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
     level++;
     __syncthreads();
   }
+  // the stored graph: the passing pixels in order, and their pushes of
+  // passing pixels
+  int np = 0, nch2 = 0;
+  if (!overflow) {
+    for (int b = 0; b < nn; b += 64) {
+      const int i = b + lane;
+      const bool p = i < nn && npass[i];
+      const uint64_t m = __ballot(p);
+      if (i < nn) front[i] = p ? (int16_t)(np + __popcll(m & ((1ull << lane) - 1))) : (int16_t)-1;
+      np += __popcll(m);
+    }
+    __syncthreads();
+    for (int b = 0; b < nn; b += 64) {
+      const int i = b + lane;
+      int x = 0;
+      if (i < nn && cbase[i] >= 0)
+        for (int k = 0; k < ccnt[i]; ++k) {
+          const int ch = child[cbase[i] + k];
+          x += ch >= 0 && npass[ch];
+        }
+      int v = x;  // inclusive prefix over the wave
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+      }
+      if (i < nn) vfirst[i] = (int16_t)(nch2 + v - x);
+      nch2 += __shfl(v, 63);
+    }
+    overflow = np > kNodeCap || nch2 > kChildCap;
+  }
   if (overflow) {
     if (lane == 0) g.n[ws] = -1;
     return;
   }
-  for (int i = lane; i < nn; i += 64) {
-    const int rc = nrc[i];
-    const FusImgDev& I = imgs[nimg[i]];
-    g.gid[nb + i] = I.base + (int64_t)(rc >> 16) * I.W + (rc & 0xffff);
-    g.img[nb + i] = nimg[i];
-    g.pass[nb + i] = npass[i];
-    g.cbase[nb + i] = cbase[i];
-    g.ccnt[nb + i] = ccnt[i];
-  }
-  for (int c = lane; c < nch; c += 64) g.child[(size_t)ws * kChildCap + c] = child[c];
   if (lane == 0) {
-    g.n[ws] = nn;
-    g.nc[ws] = nch;
+    const unsigned long long no = atomicAdd(g.used, (unsigned long long)np);
+    const unsigned long long co = atomicAdd(g.used + 1, (unsigned long long)nch2);
+    if (no + np > (unsigned long long)g.node_cap || co + nch2 > (unsigned long long)g.child_cap) {
+      offs[0] = -1;
+      atomicMin(g.no_room, ws);
+      g.n[ws] = -2;
+    } else {
+      offs[0] = (long long)no;
+      offs[1] = (long long)co;
+    }
+  }
+  __syncthreads();
+  if (offs[0] < 0) return;
+  const int64_t no = offs[0], co = offs[1];
+  for (int i = lane; i < nn; i += 64) {
+    const int j = front[i];
+    if (j < 0) continue;
+    const int rc = nrc[i], im = nimg[i];
+    const FusImgDev& I = imgs[im];
+    g.gid[no + j] = I.base + (int64_t)(rc >> 16) * I.W + (rc & 0xffff);
+    g.img[no + j] = im;
+    int w = 0;
+    if (cbase[i] >= 0)
+      for (int k = 0; k < ccnt[i]; ++k) {
+        const int ch = child[cbase[i] + k];
+        if (ch >= 0 && npass[ch]) g.child[co + vfirst[i] + w++] = front[ch];
+      }
+    g.cbase[no + j] = cbase[i] >= 0 ? vfirst[i] : (int16_t)-1;
+    g.ccnt[no + j] = (int16_t)w;
+  }
+  if (lane == 0) {
+    g.n[ws] = np;
+    g.nc[ws] = nch2;
+    g.node_off[ws] = (int32_t)no;
+    g.child_off[ws] = (int32_t)co;
   }
 }
 
-// The depth-first walk of Fuse over a recorded graph against the masks of
-// the round, then the walk's claims; one wave per pending walk.  Writes the
-// accepted pixels in visiting order, as k_walk does.
-__global__ void __launch_bounds__(64) k_replay(const FusImgDev* __restrict__ imgs, FusParams prm, Graph g,
-                                               const int32_t* __restrict__ seeds, int32_t window_first, int32_t n,
-                                               WalkRec* __restrict__ recs, int64_t* __restrict__ gids,
-                                               AccEntry* __restrict__ acc, int32_t* __restrict__ first) {
+// The walks whose graphs pass each pixel (hold it as a node): counted, then
+// (after an exclusive prefix sum turns the counts into starts) scattered
+// into per-pixel lists.  One wave per seed.
+__global__ void __launch_bounds__(256) k_count(Pool g, int32_t count, int32_t* __restrict__ cnt) {
+  const int ws = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (ws >= count) return;
+  const int n = g.n[ws];
+  if (n <= 0) return;
+  const int64_t no = g.node_off[ws];
+  for (int i = lane; i < n; i += 64) atomicAdd(cnt + g.gid[no + i], 1);
+}
+
+__global__ void __launch_bounds__(256) k_scatter(Pool g, int32_t count, int32_t* __restrict__ end,
+                                                 int32_t* __restrict__ lists) {
+  const int ws = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (ws >= count) return;
+  const int n = g.n[ws];
+  if (n <= 0) return;
+  const int64_t no = g.node_off[ws];
+  for (int i = lane; i < n; i += 64) lists[atomicAdd(end + g.gid[no + i], 1)] = ws;
+}
+
+// Exclusive prefix sums of int32 counts (totals stay below 2^31): per-tile
+// sums, a scan of the tile sums in one block, then each tile's scan.
+constexpr int kScanThreads = 256, kScanItems = 8, kScanTile = kScanThreads * kScanItems;
+
+__device__ int32_t block_exclusive(int32_t x, int32_t* sh, int32_t* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t v = x;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  if (lane == 63) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+      const int32_t t = sh[w];
+      sh[w] = s;
+      s += t;
+    }
+    sh[kScanThreads / 64] = s;
+  }
+  __syncthreads();
+  const int32_t r = sh[wid] + v - x;
+  *total = sh[kScanThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int32_t* __restrict__ in, int64_t n,
+                                                             int32_t* __restrict__ sums) {
+  __shared__ int32_t sh[kScanThreads / 64 + 1];
+  const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k)
+    if (b + k < n) s += in[b + k];
+  int32_t tot;
+  (void)block_exclusive(s, sh, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int32_t* __restrict__ sums, int64_t ntiles,
+                                                            int32_t* __restrict__ total) {
+  __shared__ int32_t sh[kScanThreads / 64 + 1];
+  int32_t carry = 0;
+  for (int64_t b0 = 0; b0 < ntiles; b0 += kScanTile) {
+    const int64_t b = b0 + (int64_t)threadIdx.x * kScanItems;
+    int32_t x[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+      x[k] = b + k < ntiles ? sums[b + k] : 0;
+      s += x[k];
+    }
+    int32_t tot;
+    int32_t run = carry + block_exclusive(s, sh, &tot);
+    for (int k = 0; k < kScanItems; ++k)
+      if (b + k < ntiles) {
+        sums[b + k] = run;
+        run += x[k];
+      }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                             int64_t n, const int32_t* __restrict__ sums) {
+  __shared__ int32_t sh[kScanThreads / 64 + 1];
+  const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t x[kScanItems], s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    x[k] = b + k < n ? in[b + k] : 0;
+    s += x[k];
+  }
+  int32_t tot;
+  int32_t run = sums[blockIdx.x] + block_exclusive(s, sh, &tot);
+  for (int k = 0; k < kScanItems; ++k)
+    if (b + k < n) {
+      out[b + k] = run;
+      run += x[k];
+    }
+}
+
+// The dataflow's state, for one segment of a window's walks (the window's
+// walks between two that run alone).
+struct Flow {
+  const int32_t* tickets;  // the segment's seeds with a graph, in seed order
+  int32_t n_tickets;
+  int32_t seg_lo;          // the segment's first seed: earlier walks have finished
+  int32_t* queue;          // [n_tickets] ready walks (seeds), -1 until written
+  int32_t* counters;       // [0] slots claimed [1] slots written [2] walks finished [3] stop [4] longest chain
+  int32_t* pending;        // [seed] earlier walks it depends on that have not finished
+  int32_t* npred;          // [seed] earlier walks it depends on
+  int32_t* preds;          // [node] those walks, at the graph's node offset
+  int32_t* nsucc;          // [seed] later walks that depend on it
+  int32_t* succ_end;       // [seed] end of its successor list (the prefix sum, advanced by the scatter)
+  int32_t* succ;           // successor lists
+  int32_t* depth;          // [seed] 1 + the longest chain of walks it depends on
+  int32_t* res_cnt;        // [seed] pixels accepted
+  int16_t* res_nodes;      // [node] accepted nodes in visiting order, at the graph's node offset
+  const int32_t* cnt;      // [pixel] walks whose graphs hold the pixel
+  const int32_t* end;      // [pixel] end of its list
+  const int32_t* lists;
+  long long wait_ticks;    // how long no walk may finish before the waves give up
+};
+
+// A set of up to kNodeCap = 256 nodes in four 64-bit words (named, not an
+// array: a dynamically indexed array would live in scratch memory).
+struct NodeSet {
+  uint64_t w0, w1, w2, w3;
+  __device__ bool has(int i) const {  // any lane's i (masks, not selects: no lookup table)
+    const uint32_t sh = (uint32_t)i & 63u;
+    const uint64_t q = (uint64_t)((uint32_t)i >> 6);
+    const uint64_t b = ((w0 >> sh) & (uint64_t)(q == 0)) | ((w1 >> sh) & (uint64_t)(q == 1)) |
+                       ((w2 >> sh) & (uint64_t)(q == 2)) | ((w3 >> sh) & (uint64_t)(q == 3));
+    return (b & 1ull) != 0;
+  }
+  __device__ void add(int i) {
+    const uint64_t b = 1ull << (i & 63);
+    const int q = i >> 6;
+    w0 |= q == 0 ? b : 0ull;
+    w1 |= q == 1 ? b : 0ull;
+    w2 |= q == 2 ? b : 0ull;
+    w3 |= q == 3 ? b : 0ull;
+  }
+};
+
+// The depth-first walk of Fuse over a graph in LDS, run by the whole wave:
+// the accepted nodes in visiting order (accl), their count returned.
+// `ok` holds the nodes not masked; the taken set lives in scalar registers.
+// A push of a node that is masked or already taken would end in `continue`
+// when popped (a node is never untaken), so it is not stacked; the others
+// are stacked in push order, one lane per push.  Every node is taken at
+// most once, so at most (pushes + 1) entries are stacked.
+__device__ __attribute__((always_inline)) inline int replay(const FusParams& prm, const NodeSet ok,
+                                                             const int16_t* nbase, const int16_t* ncnt,
+                                                             const int16_t* child, int32_t* stack, int16_t* accl) {
+  const int lane = threadIdx.x % 64;
+  NodeSet taken{0ull, 0ull, 0ull, 0ull};
+  if (lane == 0) stack[0] = 0;  // node 0 at depth 0
+  __builtin_amdgcn_wave_barrier();
+  int sp = 1, na = 0;
+  while (sp > 0) {
+    --sp;
+    const int e = __builtin_amdgcn_readfirstlane(stack[sp]);
+    const int node = e & 0xffff, d = e >> 16;
+    if (!ok.has(node) || taken.has(node)) continue;
+    taken.add(node);
+    if (lane == 0) accl[na] = (int16_t)node;
+    ++na;
+    if (na >= prm.max_num_pixels) break;
+    if (d + 1 >= prm.max_traversal) continue;
+    const int b = __builtin_amdgcn_readfirstlane(nbase[node]);
+    const int cnt = __builtin_amdgcn_readfirstlane(ncnt[node]);
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int k = k0 + lane;
+      int c = 0;
+      bool push = false;
+      if (k < cnt) {
+        c = child[b + k];
+        push = ok.has(c) && !taken.has(c);
+      }
+      const uint64_t m = __ballot(push);
+      if (push) stack[sp + __popcll(m & ((1ull << lane) - 1))] = ((d + 1) << 16) | c;
+      sp += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();  // the wave's LDS writes precede its next pop
+  }
+  return na;
+}
+
+__device__ inline bool masked_now(const uint32_t* mask, int64_t gid) {
+  return (__hip_atomic_load(mask + (gid >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> (gid & 31)) & 1u;
+}
+
+// The walks a walk depends on: for each pixel of its graph, the last earlier
+// walk of the segment whose graph holds it (that one depended in turn on the
+// ones before it).  Stored once each; each is counted as having one more
+// successor.  One wave per walk of the segment.
+constexpr int kPredWaves = 4;
+__global__ void __launch_bounds__(64 * kPredWaves) k_preds(Pool g, Flow f) {
+  __shared__ int32_t pl[kPredWaves][kNodeCap];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int t = blockIdx.x * kPredWaves + wave;
+  if (t >= f.n_tickets) return;
+  const int ws = f.tickets[t];
+  const int nn = g.n[ws];
+  const int64_t no = g.node_off[ws];
+  int32_t* P = pl[wave];
+  for (int i = lane; i < nn; i += 64) {
+    const int64_t gid = g.gid[no + i];
+    const int e = f.end[gid], c = f.cnt[gid];
+    int pred = -1;
+    for (int j = e - c; j < e; ++j) {
+      const int v = f.lists[j];
+      if (v < ws && v > pred) pred = v;
+    }
+    P[i] = pred >= f.seg_lo ? pred : -1;
+  }
+  __builtin_amdgcn_wave_barrier();
+  int np = 0;
+  for (int b = 0; b < nn; b += 64) {
+    const int i = b + lane;
+    const int p = i < nn ? P[i] : -1;
+    bool first = p >= 0;
+    for (int j = 0; j < i && first; ++j) first = P[j] != p;
+    const uint64_t m = __ballot(first);
+    if (first) {
+      f.preds[no + np + __popcll(m & ((1ull << lane) - 1))] = p;
+      atomicAdd(f.nsucc + p, 1);
+    }
+    np += __popcll(m);
+  }
+  if (lane == 0) {
+    f.npred[ws] = np;
+    f.pending[ws] = np;
+  }
+}
+
+// Successor lists (after a prefix sum of nsucc into succ_end), and the walks
+// ready at the start in the queue.
+__global__ void __launch_bounds__(256) k_succ(Pool g, Flow f) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= f.n_tickets) return;
+  const int ws = f.tickets[t];
+  const int np = f.npred[ws];
+  const int64_t no = g.node_off[ws];
+  for (int k = 0; k < np; ++k) {
+    const int p = f.preds[no + k];
+    f.succ[atomicAdd(f.succ_end + p, 1)] = ws;
+  }
+  if (np == 0) f.queue[atomicAdd(f.counters + 1, 1)] = ws;
+}
+
+// The dataflow: persistent waves take queue slots in order; a slot's walk is
+// written once all the walks it depends on have finished.  A wave replays
+// its walk over the graph against the masks, sets the accepted pixels' mask
+// bits, publishes them (release fence), and then counts itself off at each
+// successor, queueing those it was the last one for.  All reads of what
+// other waves wrote (queue slots, mask words, depths) are agent-scope atomic
+// loads.  The slots are written as the walks become ready, and every walk
+// becomes ready (a walk depends only on earlier walks), so every wave
+// claiming a slot below n_tickets gets its walk; a wave gives up only when no
+// walk anywhere finished for wait_ticks.
+__global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* mask, Flow f) {
   __shared__ int64_t ngid[kNodeCap];
-  __shared__ uint8_t nok[kNodeCap], taken[kNodeCap], ncnt[kNodeCap];
-  __shared__ int16_t nbase[kNodeCap];
+  __shared__ int16_t nbase[kNodeCap], ncnt[kNodeCap];
   __shared__ int16_t child[kChildCap];
   __shared__ int32_t stack[kChildCap + 1];
-  __shared__ int16_t accl[kAccCap];
-  __shared__ int32_t sh[2];
-  const int w = blockIdx.x, lane = threadIdx.x;
-  if (w >= n) return;
-  const int ws = seeds[w] - window_first;
-  const int nn = g.n[ws];
-  if (nn < 0) {
+  __shared__ int16_t accl[kNodeCap];
+  const int lane = threadIdx.x;
+  for (;;) {
+    // the slot: every lane takes part in the atomic (only lane 0 adds), the
+    // result is wave-uniform.  A lane-0-only branch here lets the compiler
+    // run lane 0 and the other lanes round different loops.
+    const int h = __builtin_amdgcn_readfirstlane(atomicAdd(f.counters, lane == 0 ? 1 : 0));
+    if (h >= f.n_tickets) break;
+    int ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ws < 0) {
+      long long t0 = wall_clock64();
+      int seen = __hip_atomic_load(f.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (ws < 0) {
+        __builtin_amdgcn_s_sleep(2);
+        ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ws >= 0) break;
+        if (__hip_atomic_load(f.counters + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const int fin = __hip_atomic_load(f.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (fin != seen) {
+          seen = fin;
+          t0 = wall_clock64();
+        } else if (wall_clock64() - t0 > f.wait_ticks) {
+          __hip_atomic_store(f.counters + 3, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    ws = __builtin_amdgcn_readfirstlane(ws);
+    if (ws < 0) break;  // stopped
+    const int nn = g.n[ws], nch = g.nc[ws];
+    const int64_t no = g.node_off[ws], co = g.child_off[ws];
+    // the graph; the masks, final for its pixels (every earlier walk that
+    // could take one has finished)
+    for (int i = lane; i < nn; i += 64) {
+      ngid[i] = g.gid[no + i];
+      nbase[i] = g.cbase[no + i];
+      ncnt[i] = g.ccnt[no + i];
+    }
+    for (int c = lane; c < nch; c += 64) child[c] = g.child[co + c];
+    __builtin_amdgcn_wave_barrier();
+    NodeSet ok;
+    ok.w0 = __ballot(lane < nn && !masked_now(mask, ngid[lane]));
+    ok.w1 = __ballot(64 + lane < nn && !masked_now(mask, ngid[64 + lane]));
+    ok.w2 = __ballot(128 + lane < nn && !masked_now(mask, ngid[128 + lane]));
+    ok.w3 = __ballot(192 + lane < nn && !masked_now(mask, ngid[192 + lane]));
+    const int na = replay(prm, ok, nbase, ncnt, child, stack, accl);
+    __builtin_amdgcn_wave_barrier();
+    for (int j = lane; j < na; j += 64) {
+      const int node = accl[j];
+      const int64_t gid = ngid[node];
+      __hip_atomic_fetch_or(mask + (gid >> 5), 1u << (gid & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      f.res_nodes[no + j] = (int16_t)node;
+    }
+    // the longest chain of walks behind this one
+    const int np = f.npred[ws];
+    int dep = 0;
+    for (int k = lane; k < np; k += 64)
+      dep = max(dep, __hip_atomic_load(f.depth + f.preds[no + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int d = 32; d > 0; d >>= 1) dep = max(dep, __shfl_xor(dep, d));
     if (lane == 0) {
-      WalkRec r{};
-      r.flags = kOverflow;
-      recs[w] = r;
+      f.res_cnt[ws] = na;
+      __hip_atomic_store(f.depth + ws, dep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicMax(f.counters + 4, dep + 1);
     }
-    return;
-  }
-  const size_t nb = (size_t)ws * kNodeCap;
-  const int nch = g.nc[ws];
-  for (int i = lane; i < nn; i += 64) {
-    const int64_t gid = g.gid[nb + i];
-    const FusImgDev& I = imgs[g.img[nb + i]];
-    ngid[i] = gid;
-    nok[i] = g.pass[nb + i] && !I.mask[gid - I.base];
-    taken[i] = 0;
-    nbase[i] = g.cbase[nb + i];
-    ncnt[i] = g.ccnt[nb + i];
-  }
-  for (int c = lane; c < nch; c += 64) child[c] = g.child[(size_t)ws * kChildCap + c];
-  __syncthreads();
-  if (lane == 0) {
-    int sp = 0, na = 0, flags = 0;
-    stack[sp++] = 0;  // node 0 at depth 0
-    while (sp > 0) {
-      const int e = stack[--sp];
-      const int node = e & 0xffff, d = e >> 16;
-      if (!nok[node] || taken[node]) continue;
-      if (na == kAccCap) {
-        flags |= kOverflow;
-        break;
-      }
-      taken[node] = 1;
-      accl[na++] = (int16_t)node;
-      if (na >= prm.max_num_pixels) {
-        flags |= kCut;
-        break;
-      }
-      if (d + 1 >= prm.max_traversal) {
-        flags |= kCut;
-        continue;
-      }
-      const int b = nbase[node], cnt = ncnt[node];
-      for (int k = 0; k < cnt; ++k) {
-        const int c = child[b + k];
-        if (c >= 0) stack[sp++] = ((d + 1) << 16) | c;
-      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // the successors this walk was the last one for
+    const int ns = f.nsucc[ws], se = f.succ_end[ws];
+    for (int k = lane; k < ns; k += 64) {
+      const int sw = f.succ[se - ns + k];
+      if (atomicSub(f.pending + sw, 1) == 1)
+        __hip_atomic_store(f.queue + atomicAdd(f.counters + 1, 1), sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    sh[0] = na;
-    sh[1] = flags;
-  }
-  __syncthreads();
-  const int na = sh[0];
-  for (int j = lane; j < na; j += 64) {
-    const int node = accl[j];
-    const int64_t gid = ngid[node];
-    gids[(size_t)w * kAccCap + j] = gid;
-    AccEntry& a = acc[(size_t)w * kAccCap + j];
-    a.img = g.img[nb + node];
-    for (int k = 0; k < kVals; ++k) a.v[k] = g.val[(nb + node) * kVals + k];
-    atomicMin(first + gid, w);
-  }
-  if (lane == 0) {
-    WalkRec r{};
-    r.flags = sh[1];
-    r.n_acc = na;
-    recs[w] = r;
+    if (lane == 0) atomicAdd(f.counters + 2, 1);
   }
 }
 
-// The earliest walk of the round that accepted each pixel.
-__global__ void __launch_bounds__(kBlock) k_claim(const WalkRec* __restrict__ recs, const int64_t* __restrict__ gids,
-                                                  int32_t n, int32_t acc_cap, int32_t* __restrict__ first) {
-  const int w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= n) return;
-  const int na = min(recs[w].n_acc, acc_cap);
-  for (int j = 0; j < na; ++j) atomicMin(first + gids[(size_t)w * acc_cap + j], w);
-}
-
-// Exact walks; the first inexact walk that was cut or overflowed bounds the
-// round (stop).
-__global__ void __launch_bounds__(kBlock) k_validate(const WalkRec* __restrict__ recs, const int64_t* __restrict__ gids,
-                                                     int32_t n, int32_t acc_cap, const int32_t* __restrict__ first,
-                                                     uint8_t* __restrict__ exact, int32_t* __restrict__ stop) {
-  const int w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= n) return;
-  const WalkRec r = recs[w];
-  bool ok = !(r.flags & kOverflow);
-  for (int j = 0; j < r.n_acc && ok; ++j) ok = first[gids[(size_t)w * acc_cap + j]] == w;
-  exact[w] = ok ? 1 : 0;
-  if (!ok && (r.flags & (kOverflow | kCut))) atomicMin(stop, w);
-}
-
-// Commit the kept walks (masks; their seed slots start without a point) and
-// reset the claims.
-__global__ void __launch_bounds__(kBlock) k_commit(const FusImgDev* __restrict__ imgs, const WalkRec* __restrict__ recs,
-                                                   const int64_t* __restrict__ gids, const AccEntry* __restrict__ acc,
-                                                   const int32_t* __restrict__ seeds, int32_t n, int32_t acc_cap,
-                                                   int32_t window_first, const int32_t* __restrict__ stop,
-                                                   uint8_t* __restrict__ exact, int32_t* __restrict__ first,
-                                                   WalkRec* __restrict__ slots) {
-  const int w = blockIdx.x * kBlock + threadIdx.x;
-  if (w >= n) return;
-  const WalkRec r = recs[w];
-  const int na = min(r.n_acc, acc_cap);
-  const bool keep = exact[w] && w < *stop;
-  for (int j = 0; j < na; ++j) {
-    const int64_t g = gids[(size_t)w * acc_cap + j];
-    first[g] = INT_MAX;
-    if (keep) {
-      const FusImgDev& I = imgs[acc[(size_t)w * acc_cap + j].img];
-      I.mask[g - I.base] = 1;
-    }
+// Mask bits of the pixels a walk run alone accepted.
+__global__ void __launch_bounds__(256) k_commit_alone(const WalkRec* __restrict__ rec, const int64_t* __restrict__ gids,
+                                                      uint32_t* __restrict__ mask) {
+  const int na = rec->n_acc;
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < na; j += gridDim.x * 256) {
+    const int64_t g = gids[j];
+    atomicOr(mask + (g >> 5), 1u << (g & 31));
   }
-  exact[w] = keep ? 1 : 0;
-  if (!keep) return;
-  WalkRec out = r;
-  out.has_point = 0;
-  slots[seeds[w] - window_first] = out;
 }
 
-// The point of a kept walk (fusion.cc:2098-2180), one wave per walk.  The
-// medians (internal::Median, fusion.cc:47-60: the element of rank n/2 and,
-// for an even count, the mean with the element of rank n/2 - 1) come from
-// each entry's rank, counted in LDS; the mean gradient is summed in
-// visiting order.  Walks with more entries than LDS holds (run alone) are
-// read from the record directly.
-constexpr int kPtWaves = 4;
+// The point of a walk (fusion.cc:2098-2180), one wave per walk.  The medians
+// (internal::Median, fusion.cc:47-60: the element of rank n/2 and, for an
+// even count, the mean with the element of rank n/2 - 1) come from each
+// entry's rank, counted in LDS; the mean gradient is summed in visiting
+// order.  A graph walk's values are recomputed from its pixels and its seed
+// (the same function, so the same bits as while building the graph); a walk
+// run alone has them in its record, and one with more entries than LDS
+// holds is read from there directly.
+constexpr int kPtWaves = 2;
+constexpr int kLdsAcc = kNodeCap;           // accepted pixels staged in LDS per wave
 constexpr int kMaxImages = 65536;           // fusion takes at most this many images
 constexpr int kImgWords = kMaxImages / 32;  // visibility bitmap per wave
 
@@ -640,51 +882,75 @@ __device__ inline float pick(bool c0, float x0, bool c1, float x1) {
   return __shfl(x1, __ffsll((long long)m1) - 1);
 }
 
-__global__ void __launch_bounds__(64 * kPtWaves) k_points(const WalkRec* __restrict__ recs,
-                                                          const AccEntry* __restrict__ acc,
-                                                          const int32_t* __restrict__ seeds, int32_t n, FusParams prm,
-                                                          int32_t window_first, const uint8_t* __restrict__ kept,
-                                                          WalkRec* __restrict__ slots, int32_t* __restrict__ vis_out,
-                                                          int32_t vis_stride) {
-  __shared__ float lv[kPtWaves][kVals][kAccCap];
-  __shared__ uint32_t bits[kPtWaves][kImgWords];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int w = blockIdx.x * kPtWaves + wave;
-  WalkRec r{};
-  bool active = w < n && kept[w];
-  if (active) {
-    r = recs[w];
-    active = !(r.flags & kOverflow) && r.n_acc >= prm.min_num_pixels;
+struct GraphSrc {  // accepted pixels of a graph walk, values recomputed
+  const FusImgDev* imgs;
+  const int64_t* gid;
+  const int32_t* img;
+  const int16_t* nodes;
+  int64_t no;
+  const FusParams* prm;
+  float ref_pt[3], ref_n[3];
+  __device__ void fill(int i, float* v, int32_t* im) const {
+    const int node = nodes[no + i];
+    const int m = img[no + node];
+    const FusImgDev& I = imgs[m];
+    const int64_t p = gid[no + node] - I.base;
+    test_pixel(I, nullptr, (int)(p / I.W), (int)(p % I.W), node == 0, ref_pt, ref_n, *prm, v, false);
+    *im = m;
   }
-  const int na = active ? r.n_acc : 0;
-  const AccEntry* A = acc + (size_t)(active ? w : 0) * prm.acc_cap;
-  const bool in_lds = na <= kAccCap;
+  static constexpr bool kLdsOnly = true;  // at most kNodeCap = kLdsAcc pixels
+};
+
+struct AccSrc {  // accepted pixels of a walk run alone
+  const AccEntry* a;
+  __device__ void fill(int i, float* v, int32_t* im) const {
+    for (int k = 0; k < kVals; ++k) v[k] = a[i].v[k];
+    *im = a[i].img;
+  }
+  __device__ float v(int i, int k) const { return a[i].v[k]; }
+  static constexpr bool kLdsOnly = false;
+};
+
+// Called by every thread of the block; the `active` waves form their walk's
+// point.  Returns whether a point was formed (uniform over the wave).
+template <class Src>
+__device__ bool form_point(bool active, int na, const Src& src, const FusParams& prm, float (*lv)[kLdsAcc],
+                           uint32_t* bits, PointRec* out, int32_t* vis_out) {
+  const int lane = threadIdx.x % 64;
+  const bool in_lds = na <= kLdsAcc;
   const int words = (prm.n_images + 31) / 32;
-  for (int k = lane; k < words; k += 64) bits[wave][k] = 0u;
+  for (int k = lane; k < words; k += 64) bits[k] = 0u;
   __syncthreads();
   if (active) {
     for (int i = lane; i < na; i += 64) {
+      float x[kVals];
+      int32_t m;
+      src.fill(i, x, &m);
       if (in_lds)
-        for (int v = 0; v < kVals; ++v) lv[wave][v][i] = A[i].v[v];
-      const int32_t m = A[i].img;
-      atomicOr(&bits[wave][m >> 5], 1u << (m & 31));
+        for (int k = 0; k < kVals; ++k) lv[k][i] = x[k];
+      atomicOr(&bits[m >> 5], 1u << (m & 31));
     }
   }
   __syncthreads();
-  if (!active) return;
-  auto val = [&](int i, int v) -> float { return in_lds ? lv[wave][v][i] : A[i].v[v]; };
-  // rank-based median of value v over entries [b, na)
-  auto median = [&](int v, int b) -> float {
+  if (!active) return false;
+  auto val = [&](int i, int k) -> float {
+    if constexpr (Src::kLdsOnly)
+      return lv[k][i];
+    else
+      return in_lds ? lv[k][i] : src.v(i, k);
+  };
+  // rank-based median of value k over entries [b, na)
+  auto median = [&](int k, int b) -> float {
     const int cnt = na - b, mid = cnt / 2;
     float hi = 0.0f, lo = 0.0f;
     for (int base = b; base < na; base += 128) {
       const int i0 = base + lane, i1 = base + 64 + lane;
       float x0 = 0.0f, x1 = 0.0f;
       int l0 = 0, e0 = 0, l1 = 0, e1 = 0;
-      if (i0 < na) x0 = val(i0, v);
-      if (i1 < na) x1 = val(i1, v);
+      if (i0 < na) x0 = val(i0, k);
+      if (i1 < na) x1 = val(i1, k);
       for (int j = b; j < na; ++j) {
-        const float y = val(j, v);
+        const float y = val(j, k);
         l0 += y < x0;
         e0 += y <= x0;
         l1 += y < x1;
@@ -702,43 +968,36 @@ __global__ void __launch_bounds__(64 * kPtWaves) k_points(const WalkRec* __restr
   const int per = (words + 63) / 64;
   const int k0 = min(lane * per, words), k1 = min(k0 + per, words);
   int mine = 0;
-  for (int k = k0; k < k1; ++k) mine += __popc(bits[wave][k]);
+  for (int k = k0; k < k1; ++k) mine += __popc(bits[k]);
   int before = 0, nvis = 0;
   for (int l = 0; l < 64; ++l) {
     const int c = __shfl(mine, l);
     before += l < lane ? c : 0;
     nvis += c;
   }
-  if (nvis < prm.min_num_visible) return;
-  if (vis_out) {
-    int32_t* V = vis_out + (size_t)(seeds[w] - window_first) * vis_stride;
-    int pos = before;
-    for (int k = k0; k < k1; ++k)
-      for (uint32_t m = bits[wave][k]; m; m &= m - 1) V[pos++] = 32 * k + __ffs(m) - 1;
-  }
-  WalkRec out = r;
-  out.nvis = nvis;
+  if (nvis < prm.min_num_visible) return false;
+  PointRec r{};
+  r.nvis = nvis;
   bool ok = true;
   for (int c = 0; c < 3; ++c) {
-    out.xyz[c] = median(c, 0);
-    if (isnan(out.xyz[c]) || isinf(out.xyz[c])) ok = false;
+    r.xyz[c] = median(c, 0);
+    if (isnan(r.xyz[c]) || isinf(r.xyz[c])) ok = false;
   }
-  out.nrm[0] = out.nrm[1] = out.nrm[2] = 0.0f;
   if (ok && prm.with_normal) {
     const float nx = median(3, 0), ny = median(4, 0), nz = median(5, 0);
     const float norm = sqrtf(nx * nx + (ny * ny + nz * nz));
     if (norm < FLT_EPSILON) {
       ok = false;
     } else {
-      out.nrm[0] = nx / norm;
-      out.nrm[1] = ny / norm;
-      out.nrm[2] = nz / norm;
+      r.nrm[0] = nx / norm;
+      r.nrm[1] = ny / norm;
+      r.nrm[2] = nz / norm;
     }
   }
   if (ok) {
     for (int c = 0; c < 3; ++c) {
       const float m = roundf(median(6 + c, 0));
-      out.rgb[c] = m < 0.0f ? 0.0f : (m > 255.0f ? 255.0f : m);
+      r.rgb[c] = m < 0.0f ? 0.0f : (m > 255.0f ? 255.0f : m);
     }
     float m_conf = 0.0f;
     for (int j = 0; j < na; ++j) m_conf += val(j, 9);
@@ -753,19 +1012,106 @@ __global__ void __launch_bounds__(64 * kPtWaves) k_points(const WalkRec* __restr
       float score = (float)((double)med / prm.max_depth_error);
       score = fminf(0.6f * score + 0.4f * m_conf, 1.0f);
       if (isnan(score) || isinf(score)) ok = false;
-      out.score = score;
+      r.score = score;
     }
   }
-  out.has_point = ok ? 1 : 0;
-  if (lane == 0) slots[seeds[w] - window_first] = out;
+  if (!ok) return false;
+  int pos = before;
+  for (int k = k0; k < k1; ++k)
+    for (uint32_t m = bits[k]; m; m &= m - 1) vis_out[pos++] = 32 * k + __ffs(m) - 1;
+  if (lane == 0) *out = r;
+  return true;
+}
+
+// The points of the window's graph walks: a PointRec and a flag per seed,
+// the visibility at the graph's node offset (a point sees at most as many
+// images as its walk accepted pixels).
+__global__ void __launch_bounds__(64 * kPtWaves) k_points(const FusImgDev* __restrict__ imgs, FusParams prm,
+                                                          int32_t window_first, Pool g, Flow f,
+                                                          PointRec* __restrict__ pts, int32_t* __restrict__ has,
+                                                          int32_t* __restrict__ vis_pool,
+                                                          unsigned long long* __restrict__ kept) {
+  __shared__ float lv[kPtWaves][kVals][kLdsAcc];
+  __shared__ uint32_t bits[kPtWaves][kImgWords];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int t = blockIdx.x * kPtWaves + wave;
+  const int ws = __builtin_amdgcn_readfirstlane(t < f.n_tickets ? f.tickets[t] : 0);
+  const int na = __builtin_amdgcn_readfirstlane(t < f.n_tickets ? f.res_cnt[ws] : 0);
+  if (lane == 0 && na > 0) atomicAdd(kept, 1ull);
+  const bool active = na >= prm.min_num_pixels;
+  const int64_t no = active ? g.node_off[ws] : 0;
+  GraphSrc src{imgs, g.gid, g.img, f.res_nodes, no, &prm, {0, 0, 0}, {0, 0, 0}};
+  if (active) {  // the seed's point and normal
+    const int s = window_first + ws;
+    float v[kVals];
+    test_pixel(imgs[prm.cur], nullptr, 3 + (s / prm.seed_cols) * prm.step, 3 + (s % prm.seed_cols) * prm.step, true,
+               nullptr, nullptr, prm, v, false);
+    for (int k = 0; k < 3; ++k) {
+      src.ref_pt[k] = v[k];
+      src.ref_n[k] = v[3 + k];
+    }
+  }
+  if (form_point(active, na, src, prm, lv[wave], bits[wave], pts + ws, vis_pool + no) && lane == 0) has[ws] = 1;
+}
+
+// The point of a walk run alone (the block's first wave).
+__global__ void __launch_bounds__(64 * kPtWaves) k_points_alone(FusParams prm, const WalkRec* __restrict__ rec,
+                                                                const AccEntry* __restrict__ acc,
+                                                                PointRec* __restrict__ out, int32_t* __restrict__ has,
+                                                                int32_t* __restrict__ vis_out) {
+  __shared__ float lv[kPtWaves][kVals][kLdsAcc];
+  __shared__ uint32_t bits[kPtWaves][kImgWords];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int na = __builtin_amdgcn_readfirstlane(wave == 0 ? rec->n_acc : 0);
+  const bool active = wave == 0 && na >= prm.min_num_pixels;
+  const AccSrc src{acc};
+  const bool made = form_point(active, na, src, prm, lv[wave], bits[wave], out, vis_out);
+  if (wave == 0 && lane == 0) *has = made ? 1 : 0;
+}
+
+// The window's points in seed order: PointRec, seed and visibility count at
+// the point's rank among the seeds with a point, then the visibility lists.
+__global__ void __launch_bounds__(256) k_gather_points(const PointRec* __restrict__ pts, const int32_t* __restrict__ has,
+                                                       const int32_t* __restrict__ rank, int32_t count,
+                                                       PointRec* __restrict__ out, int32_t* __restrict__ out_seed,
+                                                       int32_t* __restrict__ out_nvis) {
+  const int ws = blockIdx.x * 256 + threadIdx.x;
+  if (ws >= count || !has[ws]) return;
+  const int k = rank[ws];
+  out[k] = pts[ws];
+  out_seed[k] = ws;
+  out_nvis[k] = pts[ws].nvis;
+}
+
+__global__ void __launch_bounds__(256) k_gather_vis(const int32_t* __restrict__ out_seed,
+                                                    const int32_t* __restrict__ out_nvis,
+                                                    const int32_t* __restrict__ vis_at, int32_t n_points, Pool g,
+                                                    const int32_t* __restrict__ vis_pool,
+                                                    int32_t* __restrict__ out_vis) {
+  const int k = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (k >= n_points) return;
+  const int64_t no = g.node_off[out_seed[k]];
+  const int nv = out_nvis[k], at = vis_at[k];
+  for (int i = lane; i < nv; i += 64) out_vis[at + i] = vis_pool[no + i];
 }
 
 __global__ void __launch_bounds__(256) k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
-__global__ void __launch_bounds__(256) k_fill_u8(uint8_t* __restrict__ p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0;
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0u;
 }
+__global__ void k_window_reset(unsigned long long* used, int32_t* no_room, int32_t count, int32_t* status) {
+  used[0] = used[1] = 0ull;
+  *no_room = count;
+  status[0] = status[1] = status[2] = 0;
+}
+// a segment's stop flag and longest chain into the window's status
+__global__ void k_segment_end(const int32_t* counters, int32_t* status) {
+  status[0] |= counters[3];
+  status[2] += counters[4];
+}
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 unsigned fill_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
@@ -785,7 +1131,19 @@ struct Buf {
     bytes = std::max<size_t>(b, 1);
     return PM_OK;
   }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
 };
+
+// Exclusive prefix sum of n int32 values into out; the total lands in *total.
+void scan(const int32_t* in, int32_t* out, int64_t n, int32_t* sums, int32_t* total, hipStream_t st) {
+  const int64_t tiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, sums, tiles, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, out, n, (const int32_t*)sums);
+}
 
 // FindNextImage (fusion.cc:697-727).
 int find_next(const pm_fusion_image* im, int n, const std::vector<uint8_t>& fused, int prev) {
@@ -809,6 +1167,16 @@ struct pm_fusion {
   std::vector<int64_t> vis_offsets{0};
   std::vector<int32_t> vis_images;
   pm_fusion_stats stats{};
+  void add(const PointRec& r, const int32_t* vis) {
+    for (int c = 0; c < 3; ++c) {
+      xyz.push_back(r.xyz[c]);
+      normal.push_back(r.nrm[c]);
+      rgb.push_back((uint8_t)r.rgb[c]);
+    }
+    score.push_back(r.score);
+    vis_images.insert(vis_images.end(), vis, vis + r.nvis);
+    vis_offsets.push_back((int64_t)vis_images.size());
+  }
 };
 
 PM_API pm_status pm_fusion_options_init(pm_fusion_options* o) {
@@ -832,7 +1200,9 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   if (o->step_size < 1 || o->min_num_pixels < 2 || o->max_num_pixels < 1 || o->max_traversal_depth < 1 ||
       !(o->max_reproj_error >= 0.0) || !(o->max_depth_error > 0.0) || !(o->max_normal_error >= 0.0))
     return fail(PM_ERR_INVALID_ARG, "invalid fusion options (min_num_pixels >= 2, step_size >= 1, ...)");
+  if (n > kMaxImages) return fail(PM_ERR_INVALID_ARG, "fusion takes at most 65536 images");
   int max_nov = 0;
+  int64_t max_seeds = 1;
   for (int32_t i = 0; i < n; ++i) {
     const pm_fusion_image& x = im[i];
     if (!x.used) continue;
@@ -841,40 +1211,43 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
         !finite_camera(x.cam))
       return fail(PM_ERR_INVALID_ARG, "invalid fusion image " + std::to_string(i));
     max_nov = std::max(max_nov, x.num_overlapping);
+    const int64_t sc = (x.width - 6 + o->step_size - 1) / o->step_size;
+    const int64_t sr = (x.height - 6 + o->step_size - 1) / o->step_size;
+    max_seeds = std::max(max_seeds, sc * sr);
   }
-  if (n > kMaxImages) return fail(PM_ERR_INVALID_ARG, "fusion takes at most 65536 images");
   PM_HIP(hipSetDevice(e->device));
   hipStream_t st = e->stream;
+  int cus = 0, clock_khz = 0;
+  PM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+  PM_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, e->device));
   struct Events {  // destroyed on every return
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
     ~Events() {
-      if (a) (void)hipEventDestroy(a);
-      if (b) (void)hipEventDestroy(b);
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
     }
   } ev;
-  PM_HIP(hipEventCreate(&ev.a));
-  PM_HIP(hipEventCreate(&ev.b));
-  hipEvent_t t0 = ev.a, t1 = ev.b;
+  for (hipEvent_t& x : ev.e) PM_HIP(hipEventCreate(&x));
+  hipEvent_t t0 = ev.e[0], t1 = ev.e[1], f0 = ev.e[2], f1 = ev.e[3];
   PM_HIP(hipEventRecord(t0, st));
 
-  // every used image's maps on the device, masks zero, claims INT_MAX
+  // every used image's maps on the device, mask bits zero
   std::vector<FusImgDev> tab(n);
   std::vector<int64_t> base(n, 0);
-  int64_t total = 0, rgb_total = 0, ov_total = 0;
+  int64_t total = 0, ov_total = 0;
   for (int32_t i = 0; i < n; ++i) {
     if (!im[i].used) continue;
     base[i] = total;
     total += (int64_t)im[i].width * im[i].height;
     ov_total += im[i].num_overlapping;
   }
-  rgb_total = 3 * total;
-  Buf d_depth, d_normal, d_rgb, d_mask, d_first, d_ov, d_tab, d_fused;
+  const int64_t mask_words = (total + 31) / 32;
+  Buf d_depth, d_normal, d_rgb, d_mask, d_ov, d_tab, d_fused;
   if (pm_status s = d_depth.get((size_t)total * 4)) return s;
   if (o->with_normal)
     if (pm_status s = d_normal.get((size_t)total * 12)) return s;
-  if (pm_status s = d_rgb.get((size_t)rgb_total)) return s;
-  if (pm_status s = d_mask.get((size_t)total)) return s;
-  if (pm_status s = d_first.get((size_t)total * 4)) return s;
+  if (pm_status s = d_rgb.get((size_t)total * 3)) return s;
+  if (pm_status s = d_mask.get((size_t)mask_words * 4)) return s;
   if (pm_status s = d_ov.get((size_t)std::max<int64_t>(ov_total, 1) * 4)) return s;
   if (pm_status s = d_fused.get((size_t)n)) return s;
   int64_t ov_at = 0;
@@ -887,20 +1260,20 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
     if (!t.used) continue;
     const int64_t px = (int64_t)t.W * t.H;
     t.base = base[i];
-    t.depth = (const float*)d_depth.p + base[i];
-    t.rgb = (const uint8_t*)d_rgb.p + 3 * base[i];
-    t.mask = (uint8_t*)d_mask.p + base[i];
-    t.ov = (const int32_t*)d_ov.p + ov_at;
+    t.depth = d_depth.as<const float>() + base[i];
+    t.rgb = d_rgb.as<const uint8_t>() + 3 * base[i];
+    t.ov = d_ov.as<const int32_t>() + ov_at;
     t.nov = im[i].num_overlapping;
-    PM_HIP(hipMemcpyAsync((float*)d_depth.p + base[i], im[i].depth, (size_t)px * 4, hipMemcpyHostToDevice, st));
-    PM_HIP(hipMemcpyAsync((uint8_t*)d_rgb.p + 3 * base[i], im[i].rgb, (size_t)px * 3, hipMemcpyHostToDevice, st));
+    PM_HIP(hipMemcpyAsync(d_depth.as<float>() + base[i], im[i].depth, (size_t)px * 4, hipMemcpyHostToDevice, st));
+    PM_HIP(hipMemcpyAsync(d_rgb.as<uint8_t>() + 3 * base[i], im[i].rgb, (size_t)px * 3, hipMemcpyHostToDevice, st));
     if (o->with_normal) {
-      t.normal = (const float*)d_normal.p + 3 * base[i];
-      PM_HIP(hipMemcpyAsync((float*)d_normal.p + 3 * base[i], im[i].normal, (size_t)px * 12, hipMemcpyHostToDevice,
+      t.normal = d_normal.as<const float>() + 3 * base[i];
+      PM_HIP(hipMemcpyAsync(d_normal.as<float>() + 3 * base[i], im[i].normal, (size_t)px * 12, hipMemcpyHostToDevice,
                             st));
     }
     if (t.nov > 0)
-      PM_HIP(hipMemcpyAsync((int32_t*)d_ov.p + ov_at, im[i].overlapping, (size_t)t.nov * 4, hipMemcpyHostToDevice, st));
+      PM_HIP(hipMemcpyAsync(d_ov.as<int32_t>() + ov_at, im[i].overlapping, (size_t)t.nov * 4, hipMemcpyHostToDevice,
+                            st));
     ov_at += t.nov;
     // P and its inverse: the pose packing of a source seen from an identity reference
     pm_image si{};
@@ -921,9 +1294,10 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   }
   if (pm_status s = d_tab.get(tab.size() * sizeof(FusImgDev))) return s;
   PM_HIP(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(FusImgDev), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(total)), dim3(256), 0, st, (uint8_t*)d_mask.p, total);
-  hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(total)), dim3(256), 0, st, (int32_t*)d_first.p, total, INT_MAX);
-  hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(256), 0, st, (uint8_t*)d_fused.p, (int64_t)n);
+  uint32_t* mask = d_mask.as<uint32_t>();
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(mask_words)), dim3(256), 0, st, mask, mask_words);
+  std::vector<uint8_t> fused(n, 0);
+  PM_HIP(hipMemcpyAsync(d_fused.p, fused.data(), (size_t)n, hipMemcpyHostToDevice, st));
   PM_HIP(hipGetLastError());
 
   FusParams prm{};
@@ -940,45 +1314,112 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
       (float)std::cos(o->max_normal_error * 0.0174532925199432954743716805978692718781530857086181640625);
   prm.max_depth_error = o->max_depth_error;
 
-  // walk records: a window of kWindow walks, and one large record for a walk
-  // that outgrew its buffers
+  // window buffers: per seed, the graph pool, per pixel the walk lists.  A
+  // window is as many seeds as the pool holds graphs for (at most one
+  // image).  child_cap = 8 * node_cap holds node_cap / kNodeCap graphs of
+  // any size.
+  const int64_t wcap = std::min(max_seeds, kMaxWindow);
+  const int64_t node_cap = std::min<int64_t>(std::max<int64_t>(wcap * 48, int64_t(1) << 16), int64_t(1) << 27);
+  const int64_t child_cap = node_cap * (kChildCap / kNodeCap);
+  Buf p_n, p_nc, p_noff, p_coff, p_gid, p_img, p_cbase, p_ccnt, p_child, p_used, p_room;
+  Buf w_cnt, w_end, w_lists, w_sums, w_total, w_tickets, w_queue, w_counters, w_pending, w_npred, w_preds, w_nsucc,
+      w_succend, w_succ, w_depth, w_rescnt, w_resnodes, w_status, w_pts, w_has, w_rank, w_visp, w_out, w_oseed, w_onvis,
+      w_visat, w_ovis, w_kept;
+  const size_t ws_b = (size_t)wcap;
+  const size_t nd_b = (size_t)node_cap;
+  if (pm_status s = p_n.get(ws_b * 4)) return s;
+  if (pm_status s = p_nc.get(ws_b * 4)) return s;
+  if (pm_status s = p_noff.get(ws_b * 4)) return s;
+  if (pm_status s = p_coff.get(ws_b * 4)) return s;
+  if (pm_status s = p_gid.get(nd_b * 8)) return s;
+  if (pm_status s = p_img.get(nd_b * 4)) return s;
+  if (pm_status s = p_cbase.get(nd_b * 2)) return s;
+  if (pm_status s = p_ccnt.get(nd_b * 2)) return s;
+  if (pm_status s = p_child.get((size_t)child_cap * 2)) return s;
+  if (pm_status s = p_used.get(16)) return s;
+  if (pm_status s = p_room.get(4)) return s;
+  if (pm_status s = w_cnt.get((size_t)total * 4)) return s;
+  if (pm_status s = w_end.get((size_t)total * 4)) return s;
+  if (pm_status s = w_lists.get(nd_b * 4)) return s;
+  const int64_t scan_n = std::max(total, wcap);
+  if (pm_status s = w_sums.get((size_t)((scan_n + kScanTile - 1) / kScanTile + 1) * 4)) return s;
+  if (pm_status s = w_total.get(8)) return s;
+  if (pm_status s = w_tickets.get(ws_b * 4)) return s;
+  if (pm_status s = w_queue.get(ws_b * 4)) return s;
+  if (pm_status s = w_counters.get(32)) return s;
+  if (pm_status s = w_pending.get(ws_b * 4)) return s;
+  if (pm_status s = w_npred.get(ws_b * 4)) return s;
+  if (pm_status s = w_preds.get(nd_b * 4)) return s;
+  if (pm_status s = w_nsucc.get(ws_b * 4)) return s;
+  if (pm_status s = w_succend.get(ws_b * 4)) return s;
+  if (pm_status s = w_succ.get(nd_b * 4)) return s;
+  if (pm_status s = w_rescnt.get(ws_b * 4)) return s;
+  if (pm_status s = w_resnodes.get(nd_b * 2)) return s;
+  if (pm_status s = w_status.get(16)) return s;
+  if (pm_status s = w_depth.get(ws_b * 4)) return s;
+  if (pm_status s = w_pts.get(ws_b * sizeof(PointRec))) return s;
+  if (pm_status s = w_has.get(ws_b * 4)) return s;
+  if (pm_status s = w_rank.get(ws_b * 4)) return s;
+  if (pm_status s = w_visp.get(nd_b * 4)) return s;
+  if (pm_status s = w_out.get(ws_b * sizeof(PointRec))) return s;
+  if (pm_status s = w_oseed.get(ws_b * 4)) return s;
+  if (pm_status s = w_onvis.get(ws_b * 4)) return s;
+  if (pm_status s = w_visat.get(ws_b * 4)) return s;
+  if (pm_status s = w_ovis.get(nd_b * 4)) return s;
+  if (pm_status s = w_kept.get(8)) return s;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, w_kept.as<unsigned long long>());
+  Pool g{};
+  g.n = p_n.as<int32_t>();
+  g.nc = p_nc.as<int32_t>();
+  g.node_off = p_noff.as<int32_t>();
+  g.child_off = p_coff.as<int32_t>();
+  g.gid = p_gid.as<int64_t>();
+  g.img = p_img.as<int32_t>();
+  g.cbase = p_cbase.as<int16_t>();
+  g.ccnt = p_ccnt.as<int16_t>();
+  g.child = p_child.as<int16_t>();
+  g.used = p_used.as<unsigned long long>();
+  g.no_room = p_room.as<int32_t>();
+  g.node_cap = node_cap;
+  g.child_cap = child_cap;
+  Flow f{};
+  f.tickets = w_tickets.as<const int32_t>();
+  f.queue = w_queue.as<int32_t>();
+  f.counters = w_counters.as<int32_t>();
+  f.pending = w_pending.as<int32_t>();
+  f.npred = w_npred.as<int32_t>();
+  f.preds = w_preds.as<int32_t>();
+  f.nsucc = w_nsucc.as<int32_t>();
+  f.succ_end = w_succend.as<int32_t>();
+  f.succ = w_succ.as<int32_t>();
+  f.depth = w_depth.as<int32_t>();
+  f.res_cnt = w_rescnt.as<int32_t>();
+  f.res_nodes = w_resnodes.as<int16_t>();
+  f.cnt = w_cnt.as<const int32_t>();
+  f.end = w_end.as<const int32_t>();
+  f.lists = w_lists.as<const int32_t>();
+  f.wait_ticks = (long long)(kWaitSeconds * 1000.0 * std::max(clock_khz, 1000));
+  const unsigned flow_grid = (unsigned)std::max(1, cus * kFlowWavesPerCU);
+
+  // a walk run alone: buffers for max_num_pixels
   const int big_acc = o->max_num_pixels;
   const int64_t big_stack = (int64_t)o->max_num_pixels * std::max(max_nov, 1) + 1;
   if (big_stack > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "fusion: max_num_pixels x overlapping images too large");
-  Buf d_recs, d_gids, d_acc, d_stack, d_seeds, d_exact, d_stop, d_slots, d_slot_vis, b_gids, b_acc, b_stack, b_vis;
-  if (pm_status s = d_recs.get((size_t)kWindow * sizeof(WalkRec))) return s;
-  if (pm_status s = d_gids.get((size_t)kWindow * kAccCap * sizeof(int64_t))) return s;
-  if (pm_status s = d_acc.get((size_t)kWindow * kAccCap * sizeof(AccEntry))) return s;
-  if (pm_status s = d_stack.get((size_t)kWindow * kStackCap * sizeof(StackEntry))) return s;
-  if (pm_status s = d_seeds.get((size_t)kWindow * 4)) return s;
-  if (pm_status s = d_exact.get((size_t)kWindow)) return s;
-  if (pm_status s = d_stop.get(4)) return s;
-  if (pm_status s = d_slots.get((size_t)kWindow * sizeof(WalkRec))) return s;
-  if (pm_status s = d_slot_vis.get((size_t)kWindow * kAccCap * 4)) return s;
-  Buf g_n, g_nc, g_gid, g_img, g_pass, g_cbase, g_ccnt, g_child, g_val;
-  const size_t nodes = (size_t)kWindow * kNodeCap;
-  if (pm_status s = g_n.get((size_t)kWindow * 4)) return s;
-  if (pm_status s = g_nc.get((size_t)kWindow * 4)) return s;
-  if (pm_status s = g_gid.get(nodes * 8)) return s;
-  if (pm_status s = g_img.get(nodes * 4)) return s;
-  if (pm_status s = g_pass.get(nodes)) return s;
-  if (pm_status s = g_cbase.get(nodes * 2)) return s;
-  if (pm_status s = g_ccnt.get(nodes)) return s;
-  if (pm_status s = g_child.get((size_t)kWindow * kChildCap * 2)) return s;
-  if (pm_status s = g_val.get(nodes * kVals * 4)) return s;
-  Graph graph{(int32_t*)g_n.p,   (int32_t*)g_nc.p,    (int64_t*)g_gid.p,  (int32_t*)g_img.p, (uint8_t*)g_pass.p,
-              (int16_t*)g_cbase.p, (uint8_t*)g_ccnt.p, (int16_t*)g_child.p, (float*)g_val.p};
+  Buf b_rec, b_gids, b_acc, b_stack, b_pt, b_has, b_vis;
 
   std::unique_ptr<pm_fusion> res(new (std::nothrow) pm_fusion());
   if (!res) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
   pm_fusion_stats& stt = res->stats;
-  std::vector<uint8_t> fused(n, 0);
-  std::vector<int32_t> pending, next;
-  std::vector<uint8_t> exact(kWindow);
-  std::vector<WalkRec> slots(kWindow);
-  std::vector<int32_t> slot_vis((size_t)kWindow * kAccCap);
-  std::vector<std::vector<int32_t>> big_vis(kWindow);  // visibility of walks run alone (may exceed kAccCap)
-  std::vector<uint8_t> big_slot(kWindow);
+  std::vector<int32_t> nodes_of, tickets;
+  std::vector<int64_t> barriers;
+  std::vector<PointRec> pts_h;
+  std::vector<int32_t> seed_h, vis_h;
+  struct Alone {
+    int32_t ws;
+    PointRec r;
+    std::vector<int32_t> vis;
+  };
+  std::vector<Alone> alone;
 
   int start = 0;
   for (int32_t i = 0; i < n; ++i)
@@ -991,131 +1432,180 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
     const int W = im[cur].width, H = im[cur].height, step = o->step_size;
     const int seed_cols = (W - 6 + step - 1) / step, seed_rows = (H - 6 + step - 1) / step;
     const int64_t seeds = (int64_t)seed_cols * seed_rows;
-    if (seeds > INT32_MAX) return fail(PM_ERR_INVALID_ARG, "fusion: too many seeds in one image");
     prm.cur = cur;
     prm.seed_cols = seed_cols;
-    for (int64_t w0 = 0; w0 < seeds; w0 += kWindow) {
-      const int nwin = (int)std::min<int64_t>(kWindow, seeds - w0);
-      pending.resize(nwin);
-      for (int k = 0; k < nwin; ++k) pending[k] = (int32_t)(w0 + k);
-      std::fill(big_slot.begin(), big_slot.begin() + nwin, 0);
-      bool big_next = false;
+    int64_t w0 = 0;
+    int64_t limit = wcap;  // seeds of the next window: about as many as the pool held last
+    while (w0 < seeds) {
+      const int nwin = (int)std::min<int64_t>(limit, seeds - w0);
       // every seed's walk graph, against the masks of the window's start
-      hipLaunchKernelGGL(k_graph, dim3(nwin), dim3(64), 0, st, (const FusImgDev*)d_tab.p, (const uint8_t*)d_fused.p,
-                         prm, (int32_t)w0, nwin, graph);
+      hipLaunchKernelGGL(k_window_reset, dim3(1), dim3(1), 0, st, g.used, g.no_room, nwin, w_status.as<int32_t>());
+      hipLaunchKernelGGL(k_graph, dim3(nwin), dim3(64), 0, st, (const FusImgDev*)d_tab.p, (const uint32_t*)mask,
+                         (const uint8_t*)d_fused.p, prm, (int32_t)w0, nwin, g);
       PM_HIP(hipGetLastError());
-      while (!pending.empty()) {
-        const int m = big_next ? 1 : (int)pending.size();
-        int64_t* gid = (int64_t*)d_gids.p;
-        AccEntry* acc = (AccEntry*)d_acc.p;
-        StackEntry* stk = (StackEntry*)d_stack.p;
-        int32_t* vis_out = (int32_t*)d_slot_vis.p;
-        int32_t vis_stride = kAccCap;
-        prm.acc_cap = kAccCap;
-        prm.stack_cap = kStackCap;
-        if (big_next) {
-          if (pm_status s = b_gids.get((size_t)big_acc * sizeof(int64_t))) return s;
-          if (pm_status s = b_acc.get((size_t)big_acc * sizeof(AccEntry))) return s;
-          if (pm_status s = b_stack.get((size_t)big_stack * sizeof(StackEntry))) return s;
-          if (pm_status s = b_vis.get((size_t)std::min(big_acc, n) * 4)) return s;
-          gid = (int64_t*)b_gids.p;
-          acc = (AccEntry*)b_acc.p;
-          stk = (StackEntry*)b_stack.p;
-          vis_out = (int32_t*)b_vis.p;
-          vis_stride = 0;
-          prm.acc_cap = big_acc;
-          prm.stack_cap = (int32_t)big_stack;
-        }
-        PM_HIP(hipMemcpyAsync(d_seeds.p, pending.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
-        const int32_t no_stop = INT_MAX;
-        PM_HIP(hipMemcpyAsync(d_stop.p, &no_stop, 4, hipMemcpyHostToDevice, st));
-        const dim3 grid((m + kBlock - 1) / kBlock);
-        if (big_next) {
-          // the walk itself, pixel by pixel, with buffers for max_num_pixels
-          hipLaunchKernelGGL(k_walk, grid, dim3(kBlock), 0, st, (const FusImgDev*)d_tab.p,
-                             (const uint8_t*)d_fused.p, prm, (const int32_t*)d_seeds.p, m, (WalkRec*)d_recs.p, gid,
-                             acc, stk);
-          hipLaunchKernelGGL(k_claim, grid, dim3(kBlock), 0, st, (const WalkRec*)d_recs.p, (const int64_t*)gid, m,
-                             prm.acc_cap, (int32_t*)d_first.p);
-        } else {
-          hipLaunchKernelGGL(k_replay, dim3(m), dim3(64), 0, st, (const FusImgDev*)d_tab.p, prm, graph,
-                             (const int32_t*)d_seeds.p, (int32_t)w0, m, (WalkRec*)d_recs.p, gid, acc,
-                             (int32_t*)d_first.p);
-        }
-        hipLaunchKernelGGL(k_validate, grid, dim3(kBlock), 0, st, (const WalkRec*)d_recs.p, (const int64_t*)gid, m,
-                           prm.acc_cap, (const int32_t*)d_first.p, (uint8_t*)d_exact.p, (int32_t*)d_stop.p);
-        hipLaunchKernelGGL(k_commit, grid, dim3(kBlock), 0, st, (const FusImgDev*)d_tab.p, (const WalkRec*)d_recs.p,
-                           (const int64_t*)gid, (const AccEntry*)acc, (const int32_t*)d_seeds.p, m, prm.acc_cap,
-                           (int32_t)w0, (const int32_t*)d_stop.p, (uint8_t*)d_exact.p, (int32_t*)d_first.p,
-                           (WalkRec*)d_slots.p);
-        hipLaunchKernelGGL(k_points, dim3((m + kPtWaves - 1) / kPtWaves), dim3(64 * kPtWaves), 0, st,
-                           (const WalkRec*)d_recs.p, (const AccEntry*)acc, (const int32_t*)d_seeds.p, m, prm,
-                           (int32_t)w0, (const uint8_t*)d_exact.p, (WalkRec*)d_slots.p, vis_out, vis_stride);
-        PM_HIP(hipGetLastError());
-        PM_HIP(hipMemcpyAsync(exact.data(), d_exact.p, (size_t)m, hipMemcpyDeviceToHost, st));
-        WalkRec first_rec{};
-        PM_HIP(hipMemcpyAsync(&first_rec, d_recs.p, sizeof(WalkRec), hipMemcpyDeviceToHost, st));
-        WalkRec first_slot{};
-        if (big_next)
-          PM_HIP(hipMemcpyAsync(&first_slot, (const WalkRec*)d_slots.p + (pending[0] - (int32_t)w0), sizeof(WalkRec),
-                                hipMemcpyDeviceToHost, st));
-        PM_HIP(hipStreamSynchronize(st));
-        stt.rounds++;
-        stt.walks += m;
-        if (big_next) {
-          // run alone with buffers for max_num_pixels: always kept
-          if (!exact[0]) return fail(PM_ERR_INTERNAL, "fusion: a walk run alone was not kept");
-          stt.big_walks++;
-          const int slot = pending[0] - (int32_t)w0;
-          big_slot[slot] = 1;
-          big_vis[slot].assign(first_slot.has_point ? first_slot.nvis : 0, 0);
-          if (!big_vis[slot].empty())
-            PM_HIP(hipMemcpy(big_vis[slot].data(), b_vis.p, big_vis[slot].size() * 4, hipMemcpyDeviceToHost));
-          pending.erase(pending.begin());
-          stt.kept++;
-          big_next = false;
-          continue;
-        }
-        next.clear();
-        for (int k = 0; k < m; ++k)
-          if (!exact[k]) next.push_back(pending[k]);
-        stt.kept += m - (int64_t)next.size();
-        // the first walk of a round is kept unless it outgrew its buffers
-        big_next = !exact[0] && (first_rec.flags & kOverflow);
-        if (!exact[0] && !big_next) return fail(PM_ERR_INTERNAL, "fusion: the first walk of a round was not kept");
-        pending.swap(next);
-      }
-      // the window's points, in seed order
-      PM_HIP(hipMemcpyAsync(slots.data(), d_slots.p, (size_t)nwin * sizeof(WalkRec), hipMemcpyDeviceToHost, st));
-      PM_HIP(hipMemcpyAsync(slot_vis.data(), d_slot_vis.p, (size_t)nwin * kAccCap * 4, hipMemcpyDeviceToHost, st));
+      int32_t m = 0;
+      PM_HIP(hipMemcpyAsync(&m, g.no_room, 4, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
-      for (int k = 0; k < nwin; ++k) {
-        const WalkRec& r = slots[k];
-        if (!r.has_point) continue;
-        for (int c = 0; c < 3; ++c) {
-          res->xyz.push_back(r.xyz[c]);
-          res->normal.push_back(r.nrm[c]);
-          res->rgb.push_back((uint8_t)r.rgb[c]);
-        }
-        res->score.push_back(r.score);
-        if (big_slot[k]) {
-          res->vis_images.insert(res->vis_images.end(), big_vis[k].begin(), big_vis[k].end());
-          big_vis[k].clear();
-        } else {
-          res->vis_images.insert(res->vis_images.end(), slot_vis.begin() + (size_t)k * kAccCap,
-                                 slot_vis.begin() + (size_t)k * kAccCap + r.nvis);
-        }
-        res->vis_offsets.push_back((int64_t)res->vis_images.size());
+      if (m == 0) {
+        // the pool went to later seeds first: a window whose graphs all fit
+        limit = std::max<int64_t>(1, node_cap / kNodeCap);
+        continue;
       }
+      const bool full = m < nwin;
+      nodes_of.resize(m);
+      PM_HIP(hipMemcpy(nodes_of.data(), g.n, (size_t)m * 4, hipMemcpyDeviceToHost));
+      tickets.clear();
+      barriers.clear();
+      for (int k = 0; k < m; ++k) {
+        if (nodes_of[k] > 0) tickets.push_back(k);
+        if (nodes_of[k] == -1) barriers.push_back(k);
+      }
+      const int nt = (int)tickets.size();
+      // the walks passing each pixel
+      if (nt > 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(total)), dim3(256), 0, st, w_cnt.as<int32_t>(), total, 0);
+        hipLaunchKernelGGL(k_count, dim3((m + 3) / 4), dim3(256), 0, st, g, m, w_cnt.as<int32_t>());
+        scan(w_cnt.as<const int32_t>(), w_end.as<int32_t>(), total, w_sums.as<int32_t>(), w_total.as<int32_t>(), st);
+        hipLaunchKernelGGL(k_scatter, dim3((m + 3) / 4), dim3(256), 0, st, g, m, w_end.as<int32_t>(),
+                           w_lists.as<int32_t>());
+        PM_HIP(hipMemcpyAsync(w_tickets.p, tickets.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(m)), dim3(256), 0, st, f.res_cnt, (int64_t)m, 0);
+      }
+      hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(m)), dim3(256), 0, st, w_has.as<int32_t>(), (int64_t)m, 0);
+      PM_HIP(hipGetLastError());
+      // the dataflow, split at the walks that run alone
+      PM_HIP(hipEventRecord(f0, st));
+      size_t tk = 0;
+      for (size_t bi = 0; bi <= barriers.size(); ++bi) {
+        const int64_t stop = bi < barriers.size() ? barriers[bi] : m;
+        const size_t t_begin = tk;
+        while (tk < tickets.size() && tickets[tk] < stop) ++tk;
+        if (tk > t_begin) {
+          // the segment's dependencies: predecessors, successor lists, the
+          // walks ready at the start; then the dataflow
+          Flow fs = f;
+          fs.tickets = f.tickets + t_begin;
+          fs.n_tickets = (int32_t)(tk - t_begin);
+          fs.seg_lo = tickets[t_begin];
+          const int nts = fs.n_tickets;
+          const int64_t span = stop - fs.seg_lo;  // seeds [seg_lo, stop) hold every predecessor and successor
+          hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(span)), dim3(256), 0, st, f.nsucc + fs.seg_lo, span, 0);
+          hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(nts)), dim3(256), 0, st, f.queue, (int64_t)nts, -1);
+          hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, f.counters, (int64_t)8, 0);
+          hipLaunchKernelGGL(k_preds, dim3((nts + kPredWaves - 1) / kPredWaves), dim3(64 * kPredWaves), 0, st, g, fs);
+          scan(f.nsucc + fs.seg_lo, f.succ_end + fs.seg_lo, span, w_sums.as<int32_t>(), w_total.as<int32_t>(), st);
+          hipLaunchKernelGGL(k_succ, dim3((nts + 255) / 256), dim3(256), 0, st, g, fs);
+          hipLaunchKernelGGL(k_flow, dim3(std::min<unsigned>(flow_grid, (unsigned)nts)), dim3(64), 0, st, prm, g,
+                             mask, fs);
+          hipLaunchKernelGGL(k_segment_end, dim3(1), dim3(1), 0, st, (const int32_t*)f.counters,
+                             w_status.as<int32_t>());
+          PM_HIP(hipGetLastError());
+          stt.rounds++;
+          stt.walks += nts;
+        }
+        if (bi == barriers.size()) break;
+        // the walk itself, pixel by pixel, with buffers for max_num_pixels
+        const int32_t bws = (int32_t)barriers[bi];
+        if (pm_status s = b_rec.get(sizeof(WalkRec))) return s;
+        if (pm_status s = b_gids.get((size_t)big_acc * sizeof(int64_t))) return s;
+        if (pm_status s = b_acc.get((size_t)big_acc * sizeof(AccEntry))) return s;
+        if (pm_status s = b_stack.get((size_t)big_stack * sizeof(StackEntry))) return s;
+        if (pm_status s = b_pt.get(sizeof(PointRec))) return s;
+        if (pm_status s = b_has.get(4)) return s;
+        if (pm_status s = b_vis.get((size_t)std::min(big_acc, n) * 4)) return s;
+        FusParams pb = prm;
+        pb.acc_cap = big_acc;
+        pb.stack_cap = (int32_t)big_stack;
+        hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, st, (const FusImgDev*)d_tab.p, (const uint32_t*)mask,
+                           (const uint8_t*)d_fused.p, pb, (int32_t)(w0 + bws), b_rec.as<WalkRec>(),
+                           b_gids.as<int64_t>(), b_acc.as<AccEntry>(), b_stack.as<StackEntry>());
+        hipLaunchKernelGGL(k_commit_alone, dim3(fill_grid(big_acc)), dim3(256), 0, st, (const WalkRec*)b_rec.p,
+                           (const int64_t*)b_gids.p, mask);
+        hipLaunchKernelGGL(k_points_alone, dim3(1), dim3(64 * kPtWaves), 0, st, pb, (const WalkRec*)b_rec.p,
+                           (const AccEntry*)b_acc.p, b_pt.as<PointRec>(), b_has.as<int32_t>(), b_vis.as<int32_t>());
+        PM_HIP(hipGetLastError());
+        Alone a;
+        a.ws = bws;
+        int32_t made = 0;
+        WalkRec wr{};
+        PM_HIP(hipMemcpyAsync(&made, b_has.p, 4, hipMemcpyDeviceToHost, st));
+        PM_HIP(hipMemcpyAsync(&a.r, b_pt.p, sizeof(PointRec), hipMemcpyDeviceToHost, st));
+        PM_HIP(hipMemcpyAsync(&wr, b_rec.p, sizeof(WalkRec), hipMemcpyDeviceToHost, st));
+        PM_HIP(hipStreamSynchronize(st));
+        stt.big_walks++;
+        stt.walks++;
+        if (wr.n_acc > 0) stt.kept++;
+        if (made) {
+          a.vis.resize(a.r.nvis);
+          if (a.r.nvis > 0) PM_HIP(hipMemcpy(a.vis.data(), b_vis.p, (size_t)a.r.nvis * 4, hipMemcpyDeviceToHost));
+          alone.push_back(std::move(a));
+        }
+      }
+      PM_HIP(hipEventRecord(f1, st));
+      // the graph walks' points, gathered in seed order
+      int32_t np = 0, nv = 0;
+      if (nt > 0) {
+        Flow fa = f;
+        fa.n_tickets = nt;
+        hipLaunchKernelGGL(k_points, dim3((nt + kPtWaves - 1) / kPtWaves), dim3(64 * kPtWaves), 0, st,
+                           (const FusImgDev*)d_tab.p, prm, (int32_t)w0, g, fa, w_pts.as<PointRec>(),
+                           w_has.as<int32_t>(), w_visp.as<int32_t>(), w_kept.as<unsigned long long>());
+        scan(w_has.as<const int32_t>(), w_rank.as<int32_t>(), m, w_sums.as<int32_t>(), w_total.as<int32_t>(), st);
+        hipLaunchKernelGGL(k_gather_points, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                           (const PointRec*)w_pts.p, (const int32_t*)w_has.p, (const int32_t*)w_rank.p, m,
+                           w_out.as<PointRec>(), w_oseed.as<int32_t>(), w_onvis.as<int32_t>());
+        PM_HIP(hipGetLastError());
+        PM_HIP(hipMemcpyAsync(&np, w_total.p, 4, hipMemcpyDeviceToHost, st));
+        PM_HIP(hipStreamSynchronize(st));
+        if (np > 0) {
+          scan(w_onvis.as<const int32_t>(), w_visat.as<int32_t>(), np, w_sums.as<int32_t>(), w_total.as<int32_t>(),
+               st);
+          hipLaunchKernelGGL(k_gather_vis, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st,
+                             (const int32_t*)w_oseed.p, (const int32_t*)w_onvis.p, (const int32_t*)w_visat.p, np, g,
+                             (const int32_t*)w_visp.p, w_ovis.as<int32_t>());
+          PM_HIP(hipGetLastError());
+          PM_HIP(hipMemcpyAsync(&nv, w_total.p, 4, hipMemcpyDeviceToHost, st));
+          PM_HIP(hipStreamSynchronize(st));
+          pts_h.resize(np);
+          seed_h.resize(np);
+          vis_h.resize(std::max(nv, 1));
+          PM_HIP(hipMemcpyAsync(pts_h.data(), w_out.p, (size_t)np * sizeof(PointRec), hipMemcpyDeviceToHost, st));
+          PM_HIP(hipMemcpyAsync(seed_h.data(), w_oseed.p, (size_t)np * 4, hipMemcpyDeviceToHost, st));
+          if (nv > 0) PM_HIP(hipMemcpyAsync(vis_h.data(), w_ovis.p, (size_t)nv * 4, hipMemcpyDeviceToHost, st));
+        }
+      }
+      int32_t status[3] = {0, 0, 0};
+      PM_HIP(hipMemcpyAsync(status, w_status.p, 12, hipMemcpyDeviceToHost, st));
+      PM_HIP(hipStreamSynchronize(st));
+      if (status[0]) return fail(PM_ERR_INTERNAL, "fusion: no walk finished for 4 s; the dataflow gave up");
+      float flow_ms = 0.0f;
+      PM_HIP(hipEventElapsedTime(&flow_ms, f0, f1));
+      stt.flow_ms += flow_ms;
+      stt.chain += status[2];
+      // merged with the walks run alone, by seed
+      size_t ai = 0;
+      int64_t vat = 0;
+      for (int k = 0; k < np; ++k) {
+        for (; ai < alone.size() && alone[ai].ws < seed_h[k]; ++ai) res->add(alone[ai].r, alone[ai].vis.data());
+        res->add(pts_h[k], vis_h.data() + vat);
+        vat += pts_h[k].nvis;
+      }
+      for (; ai < alone.size(); ++ai) res->add(alone[ai].r, alone[ai].vis.data());
+      alone.clear();
+      w0 += m;
+      limit = full ? std::min<int64_t>(wcap, (int64_t)m + m / 4 + 1) : wcap;
     }
     fused[cur] = 255;
     const uint8_t one = 255;
     PM_HIP(hipMemcpyAsync((uint8_t*)d_fused.p + cur, &one, 1, hipMemcpyHostToDevice, st));
   }
+  unsigned long long kept = 0;
+  PM_HIP(hipMemcpyAsync(&kept, w_kept.p, 8, hipMemcpyDeviceToHost, st));
   PM_HIP(hipEventRecord(t1, st));
   PM_HIP(hipEventSynchronize(t1));
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, t0, t1);
+  stt.kept += (int64_t)kept;
   stt.total_ms = ms;
   stt.points = (int64_t)res->score.size();
   *out = res.release();

@@ -118,3 +118,20 @@ def fuse(images: List[FusionImage], options: Optional[StereoFusionOptions] = Non
         return unpack(n.value, nv.value, lambda *a: abi.check(eng.lib.pm_fusion_get_points(h, *a)))
     finally:
         eng.lib.pm_fusion_destroy(h)
+
+
+def fusion_stats(images: List[FusionImage], options: Optional[StereoFusionOptions] = None, engine=None):
+    """pm_fusion_stats of one fusion call (walks, walks that accepted pixels,
+    dataflow launches, walks run alone, points, wall time)."""
+    from .patch_match import default_engine
+    eng = engine or default_engine()
+    opt = (options or StereoFusionOptions()).to_c()
+    inp = FusionInputs(images)
+    h = C.c_void_p()
+    abi.check(eng.lib.pm_fusion_run(eng.handle, C.byref(opt), len(images), inp.images, C.byref(h)))
+    try:
+        s = abi.pm_fusion_stats()
+        abi.check(eng.lib.pm_fusion_get_stats(h, C.byref(s)))
+        return s
+    finally:
+        eng.lib.pm_fusion_destroy(h)

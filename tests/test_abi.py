@@ -98,4 +98,4 @@ def test_null_arguments_return_errors():
     assert lib.pm_problem_create(None, None, None, None) == 1
     assert lib.pm_model_max_overlapping_images(None, None, 8, 1.0, None, None, None, None) == 1
     assert lib.pm_model_depth_ranges(None, None, None) == 1
-    assert lib.pm_abi_version() == 5
+    assert lib.pm_abi_version() == 6
