@@ -303,6 +303,11 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
   __shared__ long long offs[2];
   const int ws = blockIdx.x, lane = threadIdx.x;
   if (ws >= count) return;
+  // past a seed that found the pool full: no room either (that seed ends the window)
+  if (ws > __hip_atomic_load(g.no_room, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    if (lane == 0) g.n[ws] = -2;
+    return;
+  }
   const int s = window_first + ws;
   const int seed_row = 3 + (s / prm.seed_cols) * prm.step;
   const int seed_col = 3 + (s % prm.seed_cols) * prm.step;
@@ -373,36 +378,45 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
             }
           }
         }
-        // insertion in push order, one candidate at a time (uniform)
-        const int kn = min(64, nov - k0);
-        for (int l = 0; l < kn; ++l) {
-          const int j = __builtin_amdgcn_readfirstlane(__shfl(cj, l));
-          const int r = __builtin_amdgcn_readfirstlane(__shfl(crow, l));
-          const int c = __builtin_amdgcn_readfirstlane(__shfl(ccol, l));
-          int idx = -1;
-          if (j >= 0) {
-            const int64_t gid = imgs[j].base + (int64_t)r * imgs[j].W + c;
-            int h = hash_slot(gid);
-            while (hkey[h] != -1 && hkey[h] != gid) h = (h + 1) & (kHash - 1);
-            if (hkey[h] == gid) {
-              idx = hval[h];
-            } else if (nn < kProvCap) {
-              idx = nn++;
-              __syncthreads();
-              if (lane == 0) {
-                hkey[h] = gid;
-                hval[h] = (int16_t)idx;
-                nimg[idx] = (uint16_t)j;
-                nrc[idx] = (r << 16) | c;
-              }
-              __syncthreads();
-            } else {
-              overflow = true;
+        // the candidates' pixels, all lanes at once: a compare-and-swap on
+        // the hash key claims a new pixel; the new ones get numbers in lane
+        // order (the numbering does not affect the walk, only the push order
+        // does, and each push keeps its slot)
+        int h = 0;
+        bool mine = false;
+        int64_t gid = -1;
+        if (cj >= 0) {
+          gid = imgs[cj].base + (int64_t)crow * imgs[cj].W + ccol;
+          h = hash_slot(gid);
+          for (;;) {
+            const unsigned long long old =
+                atomicCAS((unsigned long long*)&hkey[h], ~0ull, (unsigned long long)gid);
+            if (old == ~0ull) {
+              mine = true;
               break;
             }
+            if ((int64_t)old == gid) break;
+            h = (h + 1) & (kHash - 1);
           }
-          if (lane == 0) child[nch + k0 + l] = (int16_t)idx;
         }
+        const uint64_t mm = __ballot(mine);
+        const int fresh = __popcll(mm);
+        if (nn + fresh > kProvCap) {
+          overflow = true;
+          break;
+        }
+        int idx = -1;
+        if (mine) {
+          idx = nn + __popcll(mm & ((1ull << lane) - 1));
+          hval[h] = (int16_t)idx;
+          nimg[idx] = (uint16_t)cj;
+          nrc[idx] = (crow << 16) | ccol;
+        }
+        nn += fresh;
+        __builtin_amdgcn_wave_barrier();
+        if (cj >= 0 && !mine) idx = hval[h];
+        if (k < nov) child[nch + k] = (int16_t)idx;
+        __builtin_amdgcn_wave_barrier();
       }
       nch += nov;
       __syncthreads();
