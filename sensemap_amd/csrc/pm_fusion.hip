@@ -11,34 +11,36 @@
 //     seed (its point and normal) and on the pixel, and the pixels a passing
 //     pixel pushes depend only on the pixel.  k_graph records, one wave per
 //     seed of a window, every pixel the walk could reach against the masks
-//     of the window's start (breadth first; a masked pixel stays masked),
-//     whether it passes, its values and its pushes in order.  Replaying the
-//     depth-first walk over the graph against later masks gives the same
-//     pixels, order and cuts as walking the images again.
-//   * Dependencies.  A walk's result depends on the masks only at the
-//     passing pixels of its graph, and a walk sets masks only at pixels it
-//     accepts, which pass in its own graph.  So a walk may run once, for
-//     each passing pixel of its graph, the last earlier walk whose graph
-//     passes that pixel has finished (that walk waited in turn for the ones
-//     before it).  The walks whose graphs pass each pixel are listed by a
-//     count, a prefix sum and a scatter.
-//   * Dataflow.  k_flow hands the walks out in seed order through an atomic
-//     ticket, so every walk a wave waits for is already held by a running
-//     wave, and the waits end.  A wave replays its walk against the masks it
-//     sees before it waits; after the wait, if none of the pixels it
-//     accepted has been taken meanwhile, that replay is the walk's result
-//     (masks only grow, and a popped pixel that is masked ends in `continue`
-//     as one that fails its tests does), else it replays again.  It then
-//     sets its pixels' mask bits and publishes its completion with release /
-//     acquire at agent scope, since the waves run on every XCD.  Each wait
-//     is bounded in time; one that runs out fails the call.
+//     of the window's start (breadth first; a masked pixel stays masked) and
+//     keeps the ones that pass, each with its pushes of passing pixels in
+//     push order (a pushed pixel that fails ends in `continue` when popped).
+//     Replaying the depth-first walk over the graph against later masks
+//     gives the same pixels, order and cuts as walking the images again.
+//   * Dependencies.  A walk's result depends on the masks only at the pixels
+//     of its graph, and a walk sets masks only at pixels it accepts, which
+//     are in its own graph.  So walk w needs, for each pixel of its graph,
+//     the last earlier walk whose graph holds the pixel to have finished
+//     (that walk needed the ones before it in turn).  The walks holding each
+//     pixel are listed (count, prefix sum, scatter); k_preds turns the lists
+//     into each walk's distinct predecessors and k_succ into successor lists.
+//   * Dataflow.  k_flow's persistent waves take walks from a ready queue: a
+//     walk enters it when its last predecessor finishes, so no wave ever
+//     waits on a walk.  A wave replays its walk over the graph (the whole
+//     wave: node sets in scalar registers, one lane per push), sets the
+//     accepted pixels' mask bits, publishes them with a release fence and
+//     counts itself off at each successor.  Everything a wave reads that
+//     another wrote (queue slots, mask words) is an agent-scope atomic load,
+//     since the waves run on every XCD.  A wave waiting for a queue slot
+//     gives up only when no walk anywhere finished for 4 s, which fails the
+//     call.
 //   * A graph that outgrows the wave's LDS marks its walk to run alone
 //     (k_walk, pixel by pixel, with buffers for max_num_pixels) between two
-//     dataflow launches.
+//     dataflow segments.
 //
 // The points (medians by rank, the mean gradient in visiting order, the
-// score) are formed afterwards, one wave per walk that accepted pixels, and
-// gathered in seed order on the device.
+// score) are formed afterwards, one wave per walk that accepted pixels, from
+// values recomputed as k_graph computed them, and gathered in seed order on
+// the device.
 #include "pm_host.h"
 
 #include <algorithm>
