@@ -30,6 +30,8 @@ def load():
             build()
         lib = C.CDLL(LIB)
         P = C.POINTER
+        lib.pmo_set_cost_fp64.restype = None
+        lib.pmo_set_cost_fp64.argtypes = [C.c_int]
         lib.pmo_run_view.restype = C.c_int
         lib.pmo_run_view.argtypes = [P(abi.pm_options), P(abi.pm_view_desc), P(abi.pm_outputs), C.c_int, C.c_int,
                                      C.c_int, P(pmo_counters)]
@@ -177,6 +179,12 @@ def _outputs(view, options, curvature=True):
         out["curvature"] = np.zeros((2, H, W), np.uint16)
         o.curvature = out["curvature"].ctypes.data_as(C.POINTER(C.c_uint16))
     return out, o
+
+
+def set_cost_fp64(on: bool) -> None:
+    """Runs that follow compute every NCC cost with the FP64 restatement,
+    rounded to FP32 for the decisions (pmo_set_cost_fp64); process-wide."""
+    load().pmo_set_cost_fp64(1 if on else 0)
 
 
 def run_view(options, view, row_begin=0, row_end=0, threads=0, count=False):

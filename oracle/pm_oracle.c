@@ -318,12 +318,23 @@ static inline void neumaier_add(float* s, float* c, float x) {
   *s = t;
 }
 
+static double ncc_f64(const pm_options* opt, const pm_view_desc* view, const float* poses, const float k8[8],
+                      int row, int col, const double hyp[4], int src);
+static int g_cost_fp64 = 0;
+void pmo_set_cost_fp64(int on) { g_cost_fp64 = on != 0; }
+
 /* ComputeNCCCost<R,S> (kernel_functions.cu:1245-1340). */
 static float ncc_cost(const pm_options* opt, const pmo_view* v, int row, int col,
                       const hyp4* h, int src, pmo_counters* cnt) {
   const size_t p = (size_t)row * v->W + col;
   const float var_r = v->var_r[p];
   if (cnt) __atomic_fetch_add(&cnt->ncc_evaluations, 1, __ATOMIC_RELAXED);
+  if (g_cost_fp64) {
+    const double hd[4] = {h->n[0], h->n[1], h->n[2], h->d};
+    const float k8[8] = {v->refK[0],    v->refK[1],    v->refK[2],    v->refK[3],
+                         v->refInvK[0], v->refInvK[1], v->refInvK[2], v->refInvK[3]};
+    return (float)ncc_f64(opt, v->desc, &v->pose[0][0], k8, row, col, hd, src);
+  }
   float H[9];
   compose_homography(v->pose[src], v->refInvK, row, col, h, H);
   const float fcol = (float)col, frow = (float)row;
@@ -1187,6 +1198,7 @@ static int prepare_view(const pm_options* opt, const pm_view_desc* d, pmo_view* 
   v->H = d->ref.height;
   v->N = d->num_src;
   v->view_id = d->view_id;
+  v->desc = d;
   v->depth_min = d->depth_min;
   v->depth_max = d->depth_max;
   if (v->N < 1 || v->N > PM_MAX_NUM_SRC || v->W <= 0 || v->H <= 0) return -1;

@@ -93,6 +93,7 @@ typedef struct pmo_view {
   float* inv_sw;
   float* err_r;                /* sum w (r - mu) / sum w, compensated mode */
   uint8_t* grad_ok;            /* ComputeGradientMap >= 1.0 */
+  const pm_view_desc* desc;    /* the view as given (for the FP64 cost restatement) */
 } pmo_view;
 
 typedef struct pmo_state {
@@ -108,6 +109,13 @@ typedef struct pmo_counters {
   uint64_t ncc_samples;
   uint64_t geo_evaluations;
 } pmo_counters;
+
+/* Cost precision of the runs that follow (process-wide; for the precision
+ * study): 0, the default, is the FP32 transcription; 1 computes every NCC
+ * cost with the FP64 restatement (pmo_ncc_cost_f64) and rounds it to FP32
+ * for the decisions, everything else unchanged: the "FP64 run" the FP32
+ * runs' depth maps are compared with. */
+void pmo_set_cost_fp64(int on);
 
 /* Full Run() of one view (photometric or geometric per options).  Pixels in
  * rows [row_begin, row_end) are updated; the others keep their initial state

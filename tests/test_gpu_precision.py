@@ -18,6 +18,17 @@ Tolerances (stated here, checked below):
   plain FP32 cost vs FP64        p99 <= 2e-2, max <= 0.1
   compensated FP32 cost vs FP64  p99 <= 2e-3, median <= plain median / 10
   flip rate                      <= 1e-2 (plain), <= plain (compensated)
+
+Depth against the FP64 run: the transcription's whole run with every NCC
+cost computed by its FP64 restatement and rounded to FP32 for the decisions
+(geometry and random draws as in the FP32 run), on this scene and on the
+textured 640x480 three-plane scene (amplitude 40, no noise).  Reported: the
+share of pixels whose depth differs, the median relative depth difference,
+the share within 1%, and each run's depth error against the ground truth.
+Tolerances: on the textured scene >= 99% of the engine's depths (plain and
+compensated) within 1% of the FP64 run's and median relative difference
+<= 5e-4; on both scenes the FP64 run's median error against the ground
+truth within 10% of the engine's (precision is not what limits it).
 """
 import json
 import os
@@ -99,3 +110,42 @@ def test_precision_low_texture():
     assert c["cost_err_p99"] <= 2e-3
     assert c["cost_err_median"] <= p["cost_err_median"] / 10
     assert p["flip_rate"] <= 1e-2 and c["flip_rate"] <= p["flip_rate"]
+
+
+def test_depth_against_the_fp64_cost_run():
+    report = {}
+    cases = [("low_texture", dict(amp=2.0, noise=0.5)), ("textured", dict(amp=40.0, noise=0.0))]
+    for name, kw in cases:
+        sc = helpers.small_planes(seed=1, width=640, height=480, f=500.0, **kw)
+        prob = scene.to_problem(sc)
+        gt = sc.ref.depth
+        valid = gt > 0
+        opt = helpers.c1_options(init_samples=8, image_storage=abi.PM_STORAGE_U8, accum_mode=abi.PM_ACCUM_PLAIN)
+        print("[precision] FP64 run,", name, flush=True)
+        binding.set_cost_fp64(True)
+        try:
+            d64 = binding.run_view(opt, pmm.ViewDesc(prob, opt))["depth"]
+        finally:
+            binding.set_cost_fp64(False)
+        rep = {"fp64_run_depth_median_rel_err_gt": float(np.median(helpers.depth_errors(d64, gt)))}
+        for accum, aname in ((abi.PM_ACCUM_PLAIN, "plain"), (abi.PM_ACCUM_COMPENSATED, "compensated")):
+            d = run(prob, helpers.c1_options(init_samples=8, image_storage=abi.PM_STORAGE_U8, accum_mode=accum))["depth"]
+            rel = np.abs(d[valid] - d64[valid]) / np.maximum(np.abs(d64[valid]), 1e-12)
+            rep[aname] = {"pixels_differing": float(np.mean(d[valid] != d64[valid])),
+                          "median_rel_diff_vs_fp64_run": float(np.median(rel)),
+                          "within_1pct_of_fp64_run": float(np.mean(rel < 0.01)),
+                          "depth_median_rel_err_gt": float(np.median(helpers.depth_errors(d, gt)))}
+        report[name] = rep
+        print("[precision]", name, json.dumps(rep), flush=True)
+    path = os.environ.get("PM_PRECISION_REPORT")
+    if path:
+        with open(path.replace(".json", "_fp64_run.json"), "w") as f:
+            json.dump(report, f, indent=1)
+    for name in ("low_texture", "textured"):
+        rep = report[name]
+        for aname in ("plain", "compensated"):
+            e32, e64 = rep[aname]["depth_median_rel_err_gt"], rep["fp64_run_depth_median_rel_err_gt"]
+            assert 0.9 * e32 <= e64 <= 1.1 * e32, (name, aname, e32, e64)
+    for aname in ("plain", "compensated"):
+        t = report["textured"][aname]
+        assert t["within_1pct_of_fp64_run"] >= 0.99 and t["median_rel_diff_vs_fp64_run"] <= 5e-4, t

@@ -146,6 +146,36 @@ def test_fp32_matches_fp64_on_rendered_views():
     assert worst < 1e-3, worst
 
 
+def test_fp64_cost_switch_routes_every_cost_through_the_fp64_restatement():
+    # with pmo_set_cost_fp64(1), the transcription's NCC cost is the FP64
+    # restatement rounded to FP32 (what the precision study's "FP64 run"
+    # uses); switched off again, the FP32 cost comes back unchanged
+    sc = helpers.small_planes()
+    prob = helpers.scene.to_problem(sc)
+    opts = helpers.c1_options()
+    view = pmm.ViewDesc(prob, opts)
+    rng = np.random.default_rng(9)
+    cases = []
+    for _ in range(60):
+        r, c = int(rng.integers(8, 112)), int(rng.integers(8, 152))
+        n = rng.normal(size=3)
+        n[2] = -abs(n[2]) - 0.5
+        n /= np.linalg.norm(n)
+        hyp = np.float32([*n, rng.uniform(sc.depth_min, sc.depth_max)])  # the FP32 hypothesis both paths see
+        cases.append((r, c, tuple(float(x) for x in hyp), int(rng.integers(0, 4))))
+    before = [_cost(view, opts, r, c, h, s) for r, c, h, s in cases]
+    binding.set_cost_fp64(True)
+    try:
+        during = [_cost(view, opts, r, c, h, s) for r, c, h, s in cases]
+    finally:
+        binding.set_cost_fp64(False)
+    after = [_cost(view, opts, r, c, h, s) for r, c, h, s in cases]
+    assert all(np.float32(d32) == np.float32(d64) for d32, d64 in during)
+    assert [b[0] for b in before] == [a[0] for a in after]
+    # the rounded FP64 costs are not the FP32 ones (the switch is in effect)
+    assert sum(np.float32(b[0]) != np.float32(d[0]) for b, d in zip(before, during)) > len(cases) // 2
+
+
 def _pose_view():
     rng = np.random.default_rng(11)
     ref = _textured(30, 40)
