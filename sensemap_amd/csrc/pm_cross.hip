@@ -347,10 +347,9 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
     const size_t n = (size_t)images[i].width * images[i].height;
     imgs[i].depth = (const float*)d_depth.p + img_off[i];
     imgs[i].codes = (const uint32_t*)d_codes.p + img_off[i];
-    PM_HIP(hipMemcpyAsync((float*)d_depth.p + img_off[i], images[i].depth, n * sizeof(float), hipMemcpyHostToDevice,
-                          st));
-    PM_HIP(hipMemcpyAsync((uint32_t*)d_codes.p + img_off[i], images[i].normal_codes, n * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, st));
+    if (pm_status s = copy_h2d(e, (float*)d_depth.p + img_off[i], images[i].depth, n * sizeof(float), st)) return s;
+    if (pm_status s = copy_h2d(e, (uint32_t*)d_codes.p + img_off[i], images[i].normal_codes, n * sizeof(uint32_t), st))
+      return s;
   }
   int64_t out_total = 0;
   const std::vector<int64_t> off = out_offsets(imgs, num_views, views, &out_total);
@@ -358,8 +357,7 @@ PM_API pm_status pm_cross_filter(pm_engine* e, const pm_cross_options* o, int32_
   if (pm_status s = run_cross(e, o, imgs, num_views, views, (float*)d_out.p, off, d_poses, d_tab)) return s;
   for (int32_t k = 0; k < num_views; ++k) {
     const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
-    PM_HIP(hipMemcpyAsync(views[k].refined, (const float*)d_out.p + off[k], n * sizeof(float), hipMemcpyDeviceToHost,
-                          st));
+    if (pm_status s = copy_d2h(e, views[k].refined, (const float*)d_out.p + off[k], n * sizeof(float), st)) return s;
   }
   PM_HIP(hipStreamSynchronize(st));
   return PM_OK;
@@ -621,8 +619,9 @@ PM_API pm_status pm_cross_dedup(pm_engine* e, const pm_cross_options* o, int32_t
   float* depth = (float*)d_depth.p;
   uint32_t* vis = (uint32_t*)d_vis.p;
   for (int32_t i = 0; i < num_images; ++i)
-    PM_HIP(hipMemcpyAsync(depth + off[i], images[i].depth, (size_t)images[i].width * images[i].height * sizeof(float),
-                          hipMemcpyHostToDevice, st));
+    if (pm_status s = copy_h2d(e, depth + off[i], images[i].depth,
+                               (size_t)images[i].width * images[i].height * sizeof(float), st))
+      return s;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(total)), dim3(256), 0, st, vis, total, 0x80000000u);
   PM_HIP(hipGetLastError());
 
@@ -721,18 +720,18 @@ PM_API pm_status pm_cross_dedup(pm_engine* e, const pm_cross_options* o, int32_t
   for (int32_t i = 0; i < num_images; ++i) {
     const size_t n = (size_t)images[i].width * images[i].height;
     if (images[i].visibility)
-      PM_HIP(hipMemcpyAsync(images[i].visibility, vis + off[i], n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      if (pm_status s = copy_d2h(e, images[i].visibility, vis + off[i], n * sizeof(uint32_t), st)) return s;
     if (images[i].dedup_depth)
-      PM_HIP(hipMemcpyAsync(images[i].dedup_depth, (const float*)d_out.p + off[i], n * sizeof(float),
-                            hipMemcpyDeviceToHost, st));
+      if (pm_status s = copy_d2h(e, images[i].dedup_depth, (const float*)d_out.p + off[i], n * sizeof(float), st))
+        return s;
   }
   for (int32_t k = 0; k < num_views; ++k) {
     if (!views[k].delt) continue;
     const size_t n = (size_t)images[views[k].ref].width * images[views[k].ref].height;
-    if (delt_off[k] >= 0)
-      PM_HIP(hipMemcpyAsync(views[k].delt, (const float*)d_delt.p + delt_off[k], n * sizeof(float),
-                            hipMemcpyDeviceToHost, st));
-    else
+    if (delt_off[k] >= 0) {
+      if (pm_status s = copy_d2h(e, views[k].delt, (const float*)d_delt.p + delt_off[k], n * sizeof(float), st))
+        return s;
+    } else
       std::memset(views[k].delt, 0, n * sizeof(float));
   }
   PM_HIP(hipStreamSynchronize(st));

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <new>
 
 #include "pm_math.h"
@@ -244,8 +245,94 @@ void engine_retain(pm_engine* e) { e->refs.fetch_add(1, std::memory_order_relaxe
 void engine_release(pm_engine* e) {
   if (e->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
   (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (int b = 0; b < 2; ++b) {
+    if (e->stage_ev[b]) (void)hipEventDestroy(e->stage_ev[b]);
+    if (e->stage[b]) (void)hipHostFree(e->stage[b]);
+  }
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
+}
+
+namespace {
+
+constexpr size_t kStageBytes = size_t(16) << 20;  // per pinned buffer
+constexpr size_t kDirectBytes = size_t(1) << 20;  // below: straight through hipMemcpyAsync
+
+// memcpy split over up to four threads (the staging copies are CPU-bound)
+void par_memcpy(void* d, const void* s, size_t n) {
+  constexpr size_t kPart = size_t(2) << 20;
+  const size_t parts = std::min<size_t>(4, (n + kPart - 1) / kPart);
+  if (parts <= 1) {
+    std::memcpy(d, s, n);
+    return;
+  }
+  const size_t per = (n + parts - 1) / parts;
+  std::thread th[3];
+  for (size_t p = 1; p < parts; ++p) {
+    const size_t o = p * per;
+    th[p - 1] = std::thread([=] { std::memcpy((char*)d + o, (const char*)s + o, std::min(per, n - o)); });
+  }
+  std::memcpy(d, s, std::min(per, n));
+  for (size_t p = 1; p < parts; ++p) th[p - 1].join();
+}
+
+pm_status stage_init(pm_engine* e) {
+  for (int b = 0; b < 2; ++b) {
+    if (!e->stage[b]) PM_HIP(hipHostMalloc((void**)&e->stage[b], kStageBytes, hipHostMallocDefault));
+    if (!e->stage_ev[b]) PM_HIP(hipEventCreateWithFlags(&e->stage_ev[b], hipEventDisableTiming));
+  }
+  return PM_OK;
+}
+
+}  // namespace
+
+pm_status copy_h2d(pm_engine* e, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes < kDirectBytes) {
+    PM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    return PM_OK;
+  }
+  std::lock_guard<std::mutex> lk(e->stage_mu);
+  if (pm_status s = stage_init(e)) return s;
+  int b = 0;
+  for (size_t off = 0; off < bytes; off += kStageBytes, b ^= 1) {
+    const size_t n = std::min(kStageBytes, bytes - off);
+    PM_HIP(hipEventSynchronize(e->stage_ev[b]));  // its last DMA has read it
+    par_memcpy(e->stage[b], (const char*)src + off, n);
+    PM_HIP(hipMemcpyAsync((char*)dst + off, e->stage[b], n, hipMemcpyHostToDevice, st));
+    PM_HIP(hipEventRecord(e->stage_ev[b], st));
+  }
+  return PM_OK;
+}
+
+pm_status copy_d2h(pm_engine* e, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes < kDirectBytes) {
+    PM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    PM_HIP(hipStreamSynchronize(st));
+    return PM_OK;
+  }
+  std::lock_guard<std::mutex> lk(e->stage_mu);
+  if (pm_status s = stage_init(e)) return s;
+  const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t k) -> pm_status {
+    const int b = (int)(k & 1);
+    const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+    PM_HIP(hipStreamWaitEvent(st, e->stage_ev[b], 0));  // after its last use on any stream
+    PM_HIP(hipMemcpyAsync(e->stage[b], (const char*)src + off, n, hipMemcpyDeviceToHost, st));
+    PM_HIP(hipEventRecord(e->stage_ev[b], st));
+    return PM_OK;
+  };
+  for (size_t k = 0; k < std::min<size_t>(2, chunks); ++k)
+    if (pm_status s = issue(k)) return s;
+  for (size_t k = 0; k < chunks; ++k) {
+    const int b = (int)(k & 1);
+    const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+    PM_HIP(hipEventSynchronize(e->stage_ev[b]));
+    par_memcpy((char*)dst + off, e->stage[b], n);
+    if (k + 2 < chunks)
+      if (pm_status s = issue(k + 2)) return s;
+  }
+  return PM_OK;
 }
 
 }  // namespace pmh

@@ -1280,12 +1280,11 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
     t.rgb = d_rgb.as<const uint8_t>() + 3 * base[i];
     t.ov = d_ov.as<const int32_t>() + ov_at;
     t.nov = im[i].num_overlapping;
-    PM_HIP(hipMemcpyAsync(d_depth.as<float>() + base[i], im[i].depth, (size_t)px * 4, hipMemcpyHostToDevice, st));
-    PM_HIP(hipMemcpyAsync(d_rgb.as<uint8_t>() + 3 * base[i], im[i].rgb, (size_t)px * 3, hipMemcpyHostToDevice, st));
+    if (pm_status s = copy_h2d(e, d_depth.as<float>() + base[i], im[i].depth, (size_t)px * 4, st)) return s;
+    if (pm_status s = copy_h2d(e, d_rgb.as<uint8_t>() + 3 * base[i], im[i].rgb, (size_t)px * 3, st)) return s;
     if (o->with_normal) {
       t.normal = d_normal.as<const float>() + 3 * base[i];
-      PM_HIP(hipMemcpyAsync(d_normal.as<float>() + 3 * base[i], im[i].normal, (size_t)px * 12, hipMemcpyHostToDevice,
-                            st));
+      if (pm_status s = copy_h2d(e, d_normal.as<float>() + 3 * base[i], im[i].normal, (size_t)px * 12, st)) return s;
     }
     if (t.nov > 0)
       PM_HIP(hipMemcpyAsync(d_ov.as<int32_t>() + ov_at, im[i].overlapping, (size_t)t.nov * 4, hipMemcpyHostToDevice,

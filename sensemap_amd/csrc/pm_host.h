@@ -141,6 +141,11 @@ struct pm_engine {
   int counting = 0;
   std::mutex tune_mu;
   std::map<pmh::TuneKey, pm_tuning> tuning;  // absent = default shape
+  // two pinned host buffers for large copies from and to pageable caller
+  // memory (pmh::copy_h2d / copy_d2h), made on first use
+  std::mutex stage_mu;
+  uint8_t* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct pm_problem {
@@ -223,6 +228,14 @@ pm_tuning shape_of(pm_engine* e, int kind, const pm_options& o, int N);
 // Validation shared by every entry point that takes a view.
 pm_status check_view(const pm_options* o, const pm_view_desc* v);
 bool finite_camera(const pm_camera& c);
+
+// Copies between caller (pageable) host memory and the device through the
+// engine's two pinned staging buffers, in chunks and double buffered: the
+// CPU copy of one chunk (split over a few threads) overlaps the DMA of the
+// other.  The host buffer may be reused on return; copy_d2h also waits for
+// the data.  Small copies go straight through hipMemcpyAsync.
+pm_status copy_h2d(pm_engine* e, void* dst, const void* src, size_t bytes, hipStream_t st);
+pm_status copy_d2h(pm_engine* e, void* dst, const void* src, size_t bytes, hipStream_t st);
 // NormalMap codes of a planar normal map on the device (k_encode_normals),
 // enqueued on s.
 hipError_t encode_normals_async(const float* normal, int64_t plane, uint32_t* out, hipStream_t s);

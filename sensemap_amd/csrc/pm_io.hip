@@ -9,6 +9,7 @@
 //   unit normal.
 #include "pm_host.h"
 
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -131,22 +132,44 @@ PM_API pm_status pm_mat_read(const char* path, int32_t width, int32_t height, in
   return PM_OK;
 }
 
+namespace {
+// f(begin, end) over [0, n) in contiguous ranges on up to 16 threads (large n)
+template <class F>
+void parallel_ranges(int64_t n, F f) {
+  const int64_t min_per = int64_t(1) << 16;
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int64_t parts = std::max<int64_t>(1, std::min<int64_t>({16, hw, n / min_per}));
+  if (parts == 1) {
+    f(0, n);
+    return;
+  }
+  const int64_t per = (n + parts - 1) / parts;
+  std::vector<std::thread> th;
+  for (int64_t p = 1; p < parts; ++p) th.emplace_back(f, p * per, std::min(n, (p + 1) * per));
+  f(0, std::min(n, per));
+  for (std::thread& t : th) t.join();
+}
+}  // namespace
+
 PM_API pm_status pm_normal_encode(int64_t pixels, const float* normal, uint32_t* codes) {
   if (pixels <= 0 || !normal || !codes) return fail(PM_ERR_INVALID_ARG, "invalid arguments");
-  for (int64_t i = 0; i < pixels; ++i)
-    codes[i] = encode_normal(normal[i], normal[pixels + i], normal[2 * pixels + i]);
+  parallel_ranges(pixels, [=](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) codes[i] = encode_normal(normal[i], normal[pixels + i], normal[2 * pixels + i]);
+  });
   return PM_OK;
 }
 
 PM_API pm_status pm_normal_decode(int64_t pixels, const uint32_t* codes, float* normal) {
   if (pixels <= 0 || !normal || !codes) return fail(PM_ERR_INVALID_ARG, "invalid arguments");
-  for (int64_t i = 0; i < pixels; ++i) {
-    float n[3];
-    decode_normal(codes[i], n);
-    normal[i] = n[0];
-    normal[pixels + i] = n[1];
-    normal[2 * pixels + i] = n[2];
-  }
+  parallel_ranges(pixels, [=](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      float n[3];
+      decode_normal(codes[i], n);
+      normal[i] = n[0];
+      normal[pixels + i] = n[1];
+      normal[2 * pixels + i] = n[2];
+    }
+  });
   return PM_OK;
 }
 
