@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <new>
@@ -242,13 +243,44 @@ namespace pmh {
 
 void engine_retain(pm_engine* e) { e->refs.fetch_add(1, std::memory_order_relaxed); }
 
+namespace {
+
+// Engines holding pinned staging buffers.  At exit, before the HIP runtime's
+// own teardown (this handler is registered after the runtime started, and
+// exit handlers run last-in first-out), the buffers and their events of
+// engines never destroyed are freed.
+std::mutex g_staged_mu;
+std::vector<pm_engine*>* g_staged = nullptr;  // never freed: outlives every engine
+
+void free_staging(pm_engine* e) {
+  for (int b = 0; b < 2; ++b) {
+    if (e->stage_ev[b]) (void)hipEventDestroy(e->stage_ev[b]);
+    if (e->stage[b]) (void)hipHostFree(e->stage[b]);
+    e->stage_ev[b] = nullptr;
+    e->stage[b] = nullptr;
+  }
+}
+
+void free_staging_at_exit() {
+  std::lock_guard<std::mutex> lk(g_staged_mu);
+  for (pm_engine* e : *g_staged) {
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    free_staging(e);
+  }
+  g_staged->clear();
+}
+
+}  // namespace
+
 void engine_release(pm_engine* e) {
   if (e->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  for (int b = 0; b < 2; ++b) {
-    if (e->stage_ev[b]) (void)hipEventDestroy(e->stage_ev[b]);
-    if (e->stage[b]) (void)hipHostFree(e->stage[b]);
+  {
+    std::lock_guard<std::mutex> lk(g_staged_mu);
+    if (g_staged) g_staged->erase(std::remove(g_staged->begin(), g_staged->end(), e), g_staged->end());
+    free_staging(e);
   }
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -278,10 +310,17 @@ void par_memcpy(void* d, const void* s, size_t n) {
 }
 
 pm_status stage_init(pm_engine* e) {
+  if (e->stage[0] && e->stage[1] && e->stage_ev[0] && e->stage_ev[1]) return PM_OK;
+  std::lock_guard<std::mutex> lk(g_staged_mu);
   for (int b = 0; b < 2; ++b) {
     if (!e->stage[b]) PM_HIP(hipHostMalloc((void**)&e->stage[b], kStageBytes, hipHostMallocDefault));
     if (!e->stage_ev[b]) PM_HIP(hipEventCreateWithFlags(&e->stage_ev[b], hipEventDisableTiming));
   }
+  if (!g_staged) {
+    g_staged = new std::vector<pm_engine*>();
+    std::atexit(free_staging_at_exit);
+  }
+  if (std::find(g_staged->begin(), g_staged->end(), e) == g_staged->end()) g_staged->push_back(e);
   return PM_OK;
 }
 

@@ -121,3 +121,31 @@ def test_server_never_destroyed_is_stopped_at_exit():
     assert r.returncode == 0
     assert "ran" in r.stdout
     assert "terminate" not in r.stderr
+
+
+def test_engine_with_staging_in_module_globals_at_exit():
+    # an engine that copied host maps through its pinned staging buffers,
+    # held only in the script's globals and never destroyed: its buffers are
+    # freed before the HIP runtime's teardown
+    script = """
+        import numpy as np
+        from sensemap_amd import patch_match as pmm
+        from sensemap_amd.cross_filter import CrossImage, cross_filter
+        eng = pmm.Engine(0)
+        rng = np.random.default_rng(1)
+        H, W = 480, 640
+        K = (500.0, (W - 1) / 2, 500.0, (H - 1) / 2)
+        ims = []
+        for k in range(3):
+            d = (5.0 + 0.1 * rng.random((H, W))).astype(np.float32)
+            n = np.zeros((3, H, W), np.float32)
+            n[2] = -1.0
+            ims.append(CrossImage(K, np.eye(3), np.array([0.1 * k, 0.0, 0.0]), d, n))
+        out = cross_filter(ims, [(0, [1, 2]), (1, [0, 2])], engine=eng)
+        print("ran", out[0].shape, flush=True)
+    """
+    print("[lifetime] child process with a staging engine in its globals", flush=True)
+    r = _child(script)
+    assert r.returncode == 0
+    assert "ran" in r.stdout
+    assert "terminate" not in r.stderr
