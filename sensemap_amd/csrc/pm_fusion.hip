@@ -824,45 +824,60 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
     if (ws < 0) break;  // stopped
     const int nn = g.n[ws], nch = g.nc[ws];
     const int64_t no = g.node_off[ws], co = g.child_off[ws];
-    // the graph; the masks, final for its pixels (every earlier walk that
-    // could take one has finished)
-    for (int i = lane; i < nn; i += 64) {
-      ngid[i] = g.gid[no + i];
-      nbase[i] = g.cbase[no + i];
-      ncnt[i] = g.ccnt[no + i];
-    }
-    for (int c = lane; c < nch; c += 64) child[c] = g.child[co + c];
-    __builtin_amdgcn_wave_barrier();
-    NodeSet ok;
-    ok.w0 = __ballot(lane < nn && !masked_now(mask, ngid[lane]));
-    ok.w1 = __ballot(64 + lane < nn && !masked_now(mask, ngid[64 + lane]));
-    ok.w2 = __ballot(128 + lane < nn && !masked_now(mask, ngid[128 + lane]));
-    ok.w3 = __ballot(192 + lane < nn && !masked_now(mask, ngid[192 + lane]));
-    const int na = replay(prm, ok, nbase, ncnt, child, stack, accl);
-    __builtin_amdgcn_wave_barrier();
-    for (int j = lane; j < na; j += 64) {
-      const int node = accl[j];
-      const int64_t gid = ngid[node];
-      __hip_atomic_fetch_or(mask + (gid >> 5), 1u << (gid & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      f.res_nodes[no + j] = (int16_t)node;
-    }
-    // the longest chain of walks behind this one
-    const int np = f.npred[ws];
+    const int np = f.npred[ws], ns = f.nsucc[ws], se = f.succ_end[ws];
+    // what the end of the walk needs, loaded early: the longest chain behind
+    // it and its successors
     int dep = 0;
     for (int k = lane; k < np; k += 64)
       dep = max(dep, __hip_atomic_load(f.depth + f.preds[no + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int sw0 = lane < ns ? f.succ[se - ns + lane] : -1;
+    // the seed first (node 0): taken by an earlier walk, this walk accepts
+    // nothing, and its graph is not needed.  The masks are final for the
+    // walk's pixels: every earlier walk that could take one has finished.
+    int na = 0;
+    if (!masked_now(mask, g.gid[no])) {
+#pragma unroll
+      for (int q = 0; q < kNodeCap / 64; ++q) {
+        const int i = 64 * q + lane;
+        if (i < nn) {
+          ngid[i] = g.gid[no + i];
+          nbase[i] = g.cbase[no + i];
+          ncnt[i] = g.ccnt[no + i];
+        }
+      }
+#pragma unroll 8
+      for (int c = lane; c < nch; c += 64) child[c] = g.child[co + c];
+      __builtin_amdgcn_wave_barrier();
+      NodeSet ok;
+      const bool m0 = lane < nn && masked_now(mask, ngid[lane]);
+      const bool m1 = 64 + lane < nn && masked_now(mask, ngid[64 + lane]);
+      const bool m2 = 128 + lane < nn && masked_now(mask, ngid[128 + lane]);
+      const bool m3 = 192 + lane < nn && masked_now(mask, ngid[192 + lane]);
+      ok.w0 = __ballot(lane < nn && !m0);
+      ok.w1 = __ballot(64 + lane < nn && !m1);
+      ok.w2 = __ballot(128 + lane < nn && !m2);
+      ok.w3 = __ballot(192 + lane < nn && !m3);
+      na = replay(prm, ok, nbase, ncnt, child, stack, accl);
+      __builtin_amdgcn_wave_barrier();
+      for (int j = lane; j < na; j += 64) {
+        const int node = accl[j];
+        const int64_t gid = ngid[node];
+        __hip_atomic_fetch_or(mask + (gid >> 5), 1u << (gid & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        f.res_nodes[no + j] = (int16_t)node;
+      }
+    }
     for (int d = 32; d > 0; d >>= 1) dep = max(dep, __shfl_xor(dep, d));
     if (lane == 0) {
       f.res_cnt[ws] = na;
       __hip_atomic_store(f.depth + ws, dep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       atomicMax(f.counters + 4, dep + 1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // publish the mask bits (a walk that took no pixel wrote none)
+    if (na > 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     // the successors this walk was the last one for
-    const int ns = f.nsucc[ws], se = f.succ_end[ws];
-    for (int k = lane; k < ns; k += 64) {
-      const int sw = f.succ[se - ns + k];
-      if (atomicSub(f.pending + sw, 1) == 1)
+    for (int k0 = 0; k0 < ns; k0 += 64) {
+      const int sw = k0 == 0 ? sw0 : (k0 + lane < ns ? f.succ[se - ns + k0 + lane] : -1);
+      if (sw >= 0 && atomicSub(f.pending + sw, 1) == 1)
         __hip_atomic_store(f.queue + atomicAdd(f.counters + 1, 1), sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) atomicAdd(f.counters + 2, 1);
