@@ -20,6 +20,10 @@ roofline: the checkerboard sweep kernel, algorithmic FP32 FLOPs per launch
 launch duration measured with hipEvents on the engine's stream during the
 timed steps.  cpu_baseline: the C/OpenMP transcription on a band of rows of
 the same view, photometric pass.
+
+after_path: the stages that follow the path on the same 11 views (cross
+filter over the device-resident photometric problems, deduplication, depth
+map fusion), each timed and compared bit for bit with its transcription.
 """
 from __future__ import annotations
 
@@ -708,6 +712,80 @@ def run_filters_workload(args) -> dict:
             "dedup": dedup_res, "fusion": fusion_res}
 
 
+def run_after_path(views, problems, with_cpu: bool) -> dict:
+    """The stages after the path on the step's own views: the cross-view
+    filter over the 11 photometric problems whose maps are already on the
+    device (every view against the other 10), deduplication of the refined
+    maps and depth-map fusion of the refined maps with the photometric
+    normals and the images.  Each stage is timed as one call, host copies
+    included, and compared bit for bit with its CPU transcription on the
+    same inputs (skipped with with_cpu False)."""
+    from oracle import binding
+    from sensemap_amd.cross_filter import CrossImage, cross_filter_problems, dedup
+    from sensemap_amd.fusion import FusionImage, StereoFusionOptions, fuse
+
+    threads = cpu_threads()[0]
+    n = len(views)
+    cviews = [(k, [j for j in range(n) if j != k]) for k in range(n)]
+    normals = [p.GetNormalMap() for p in problems]
+    depths = [p.GetDepthMap() for p in problems]
+
+    def timed(fn, reps=3):
+        fn()
+        ts, out = [], None
+        for _ in range(reps):
+            t = time.perf_counter()
+            out = fn()
+            ts.append((time.perf_counter() - t) * 1000.0)
+        return out, round(float(np.median(ts)), 2)
+
+    def same_f32(a, b):
+        return bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+
+    progress("after the path: cross filter")
+    refined, cross_ms = timed(lambda: cross_filter_problems(problems, cviews))
+    progress("after the path: deduplication")
+    dims = [(CrossImage(v.camera.K, v.camera.R, v.camera.T, refined[k], normals[k]), True)
+            for k, v in enumerate(views)]
+    dd, dedup_ms = timed(lambda: dedup(dims, cviews))
+    progress("after the path: fusion")
+    fims = [FusionImage(v.camera.K, v.camera.R, v.camera.T, refined[k], normals[k], v.image,
+                        [j for j in range(n) if j != k]) for k, v in enumerate(views)]
+    fo = StereoFusionOptions()
+    fz, fusion_ms = timed(lambda: fuse(fims, fo), reps=2)
+    res = {"workload": "cross filter of the 11 photometric maps on the device (10 sources each), "
+                       "deduplication of the refined maps, depth-map fusion of the refined maps with the "
+                       "photometric normals and the images (default StereoFusionOptions)",
+           "cross_filter_ms": cross_ms, "dedup_ms": dedup_ms, "fusion_ms": fusion_ms,
+           "seconds": round((cross_ms + dedup_ms + fusion_ms) / 1000.0, 3),
+           "kept_after_filter": int(sum((r > 0).sum() for r in refined)), "points": int(len(fz["score"])),
+           "bitwise_equal": None}
+    if not with_cpu:
+        return res
+    progress("after the path: transcriptions")
+    hims = [CrossImage(v.camera.K, v.camera.R, v.camera.T, depths[k], normals[k]) for k, v in enumerate(views)]
+    t = time.perf_counter()
+    want_refined = binding.cross_filter(hims, cviews, threads=threads)
+    cross_cpu = (time.perf_counter() - t) * 1000.0
+    t = time.perf_counter()
+    want_dd = binding.cross_dedup(dims, cviews, threads=threads)
+    dedup_cpu = (time.perf_counter() - t) * 1000.0
+    t = time.perf_counter()
+    want_fz = binding.fusion(fims, fo)
+    fusion_cpu = (time.perf_counter() - t) * 1000.0
+    eq = {"cross_filter": all(same_f32(a, b) for a, b in zip(refined, want_refined)),
+          "dedup": all(np.array_equal(a, b) for a, b in zip(dd["visibility"], want_dd["visibility"]))
+          and all(same_f32(a, b) for key in ("dedup_depth", "delt") for a, b in zip(dd[key], want_dd[key])),
+          "fusion": len(fz["score"]) == len(want_fz["score"]) and all(
+              same_f32(fz[k], want_fz[k]) if fz[k].dtype == np.float32 else np.array_equal(fz[k], want_fz[k])
+              for k in ("xyz", "normal", "rgb", "score", "vis_offsets", "vis_images"))}
+    res["bitwise_equal"] = {k: bool(v) for k, v in eq.items()}
+    res["cpu_ms"] = {"cross_filter": round(cross_cpu, 1), "dedup": round(dedup_cpu, 1),
+                     "fusion": round(fusion_cpu, 1)}
+    res["cpu_cores"] = {"cross_filter": threads, "dedup": threads, "fusion": 1}
+    return res
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -935,6 +1013,9 @@ def main() -> int:
                 "ms_per_step": round(hms, 2), "value": round(mpix_iter / (hms / 1000.0), 3),
                 "maps_equal_to_device_flow": bool(np.array_equal(runs[-1][1].view(np.uint32),
                                                                  geo.GetDepthMap().view(np.uint32)))}
+    after = None
+    if not args.only_photometric:
+        after = run_after_path(views, [photo] + src_photo, not args.no_cpu_baseline)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -961,6 +1042,7 @@ def main() -> int:
         "full_pipeline": full,
         "full_pipeline_batched": full_batched,
         "host_buffers": host,
+        "after_path": after,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "roofline": {
