@@ -543,8 +543,9 @@ pm_status pm_fusion_get_points(const pm_fusion* fusion, float* xyz, float* norma
 /* How the device got there: walks run, walks that accepted pixels,
  * dataflow launches, walks run alone (their graphs outgrew the device's
  * per-walk buffers), points, wall time of the call; the longest chains of
- * walks that waited one for another, summed over the windows, and the time
- * the dataflow launches took. */
+ * walks that waited one for another, summed over the windows (as the device
+ * counted them while it ran: a lower bound, exact unless a depth was read
+ * before it landed), and the time the dataflow launches took. */
 typedef struct pm_fusion_stats {
   int64_t walks, kept, rounds, big_walks, points;
   double total_ms;

@@ -63,7 +63,15 @@ constexpr int kProvCap = 512;                     // pixels (passing or not) met
 constexpr int kProvChild = 4096;                  // push slots met building a graph
 constexpr int kHash = 1024;                       // open-addressing slots (gid -> pixel)
 constexpr int64_t kMaxWindow = int64_t(1) << 23;  // seeds per window
-constexpr int kFlowWavesPerCU = 8;
+#ifndef PM_FLOW_WAVES_PER_8CU
+#define PM_FLOW_WAVES_PER_8CU 8
+#endif
+constexpr int kFlowWavesPer8CU = PM_FLOW_WAVES_PER_8CU;  // dataflow waves per 8 CUs
+#ifndef PM_POLL_SLEEP
+#define PM_POLL_SLEEP 16
+#endif
+constexpr int kPollSleep = PM_POLL_SLEEP;  // s_sleep units (64 clocks) between polls of a queue slot
+constexpr int kPollCheck = 64;             // polls between looks at the shared counters
 constexpr double kWaitSeconds = 4.0;  // one wait's bound
 
 struct FusImgDev {
@@ -302,6 +310,7 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
   __shared__ int16_t child[kProvChild];
   __shared__ int16_t front[kProvCap];  // the frontier; after the walk, each pixel's node (-1: fails)
   __shared__ int16_t vfirst[kProvCap];  // first stored push of each pixel
+  __shared__ uint8_t sown[64], sk[64];  // a push slot's frontier pixel (in the batch) and candidate
   __shared__ long long offs[2];
   const int ws = blockIdx.x, lane = threadIdx.x;
   if (ws >= count) return;
@@ -345,82 +354,126 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
   int nn = 1, nch = 0, nf = 1 < prm.max_traversal ? 1 : 0, level = 0;
   bool overflow = false;
   __syncthreads();
+  // The pushes of frontier pixel f, slots [s, s + n) of the level's pushes:
+  // candidate k is f's projection into its k-th overlapping image.  The
+  // candidates' pixels are claimed all lanes at once: a compare-and-swap on
+  // the hash key claims a new pixel, and the new ones get numbers in lane
+  // order (the numbering does not affect the walk, only the push order does,
+  // and each push keeps its slot).  Returns the pushed pixel's number (-1:
+  // none), or sets overflow when the pixels outgrow the graph.
+  auto push_slots = [&](int f, int k, bool active) __attribute__((always_inline)) {
+    int cj = -1, crow = 0, ccol = 0;
+    if (active) {
+      const FusImgDev& F = imgs[nimg[f]];
+      const int j = F.ov[k];
+      if (j >= 0 && j < prm.n_images && imgs[j].used && !fused[j]) {
+        const FusImgDev& J = imgs[j];
+        const float x = nxyz[f][0], y = nxyz[f][1], z = nxyz[f][2];
+        const float q0 = row4(J.P, x, y, z), q1 = row4(J.P + 4, x, y, z), q2 = row4(J.P + 8, x, y, z);
+        const float fc = roundf(q0 / q2), fr = roundf(q1 / q2);
+        if (fabsf(fc) < 1073741824.0f && fabsf(fr) < 1073741824.0f) {
+          const int nc = (int)fc, nr = (int)fr;
+          if (!(nc < 3 || nr < 3 || nc >= J.W - 3 || nr >= J.H - 3)) {
+            cj = j;
+            crow = nr;
+            ccol = nc;
+          }
+        }
+      }
+    }
+    int h = 0;
+    bool mine = false;
+    if (cj >= 0) {
+      const int64_t gid = imgs[cj].base + (int64_t)crow * imgs[cj].W + ccol;
+      h = hash_slot(gid);
+      for (;;) {
+        const unsigned long long old = atomicCAS((unsigned long long*)&hkey[h], ~0ull, (unsigned long long)gid);
+        if (old == ~0ull) {
+          mine = true;
+          break;
+        }
+        if ((int64_t)old == gid) break;
+        h = (h + 1) & (kHash - 1);
+      }
+    }
+    const uint64_t mm = __ballot(mine);
+    const int fresh = __popcll(mm);
+    if (nn + fresh > kProvCap) {
+      overflow = true;
+      return -1;
+    }
+    int idx = -1;
+    if (mine) {
+      idx = nn + __popcll(mm & ((1ull << lane) - 1));
+      hval[h] = (int16_t)idx;
+      nimg[idx] = (uint16_t)cj;
+      nrc[idx] = (crow << 16) | ccol;
+    }
+    nn += fresh;
+    __builtin_amdgcn_wave_barrier();
+    if (cj >= 0 && !mine) idx = hval[h];
+    return idx;
+  };
   while (nf > 0 && !overflow) {
     const int level_start = nn;
-    for (int fi = 0; fi < nf && !overflow; ++fi) {
-      const int f = __builtin_amdgcn_readfirstlane(front[fi]);
-      const FusImgDev& F = imgs[__builtin_amdgcn_readfirstlane(nimg[f])];
-      const int nov = __builtin_amdgcn_readfirstlane(F.nov);
-      if (nch + nov > kProvChild) {
-        overflow = true;
-        break;
+    for (int fi = 0; fi < nf && !overflow;) {
+      // the next frontier pixels whose pushes fit the wave's 64 lanes
+      // together, in frontier order: their pushes are the same slots as
+      // one pixel after another
+      const int bf = fi + lane < nf ? front[fi + lane] : -1;
+      const int bnov = bf >= 0 ? imgs[nimg[bf]].nov : 0;
+      int cum = bnov;  // inclusive prefix over the lanes
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(cum, d);
+        if (lane >= d) cum += t;
       }
-      if (lane == 0) {
-        cbase[f] = (int16_t)nch;
-        ccnt[f] = (int16_t)nov;
-      }
-      const float x = nxyz[f][0], y = nxyz[f][1], z = nxyz[f][2];
-      for (int k0 = 0; k0 < nov && !overflow; k0 += 64) {
-        // candidate of push k (the projection into overlapping image k)
-        const int k = k0 + lane;
-        int cj = -1, crow = 0, ccol = 0;
-        if (k < nov) {
-          const int j = F.ov[k];
-          if (j >= 0 && j < prm.n_images && imgs[j].used && !fused[j]) {
-            const FusImgDev& J = imgs[j];
-            const float q0 = row4(J.P, x, y, z), q1 = row4(J.P + 4, x, y, z), q2 = row4(J.P + 8, x, y, z);
-            const float fc = roundf(q0 / q2), fr = roundf(q1 / q2);
-            if (fabsf(fc) < 1073741824.0f && fabsf(fr) < 1073741824.0f) {
-              const int nc = (int)fc, nr = (int)fr;
-              if (!(nc < 3 || nr < 3 || nc >= J.W - 3 || nr >= J.H - 3)) {
-                cj = j;
-                crow = nr;
-                ccol = nc;
-              }
-            }
-          }
-        }
-        // the candidates' pixels, all lanes at once: a compare-and-swap on
-        // the hash key claims a new pixel; the new ones get numbers in lane
-        // order (the numbering does not affect the walk, only the push order
-        // does, and each push keeps its slot)
-        int h = 0;
-        bool mine = false;
-        int64_t gid = -1;
-        if (cj >= 0) {
-          gid = imgs[cj].base + (int64_t)crow * imgs[cj].W + ccol;
-          h = hash_slot(gid);
-          for (;;) {
-            const unsigned long long old =
-                atomicCAS((unsigned long long*)&hkey[h], ~0ull, (unsigned long long)gid);
-            if (old == ~0ull) {
-              mine = true;
-              break;
-            }
-            if ((int64_t)old == gid) break;
-            h = (h + 1) & (kHash - 1);
-          }
-        }
-        const uint64_t mm = __ballot(mine);
-        const int fresh = __popcll(mm);
-        if (nn + fresh > kProvCap) {
+      const int nb = __popcll(__ballot(bf >= 0 && cum <= 64));  // a prefix of the lanes
+      if (nb == 0) {
+        // a pixel with more pushes than lanes: 64 at a time
+        const int f = __builtin_amdgcn_readfirstlane(bf);
+        const int nov = __builtin_amdgcn_readfirstlane(bnov);
+        if (nch + nov > kProvChild) {
           overflow = true;
           break;
         }
-        int idx = -1;
-        if (mine) {
-          idx = nn + __popcll(mm & ((1ull << lane) - 1));
-          hval[h] = (int16_t)idx;
-          nimg[idx] = (uint16_t)cj;
-          nrc[idx] = (crow << 16) | ccol;
+        if (lane == 0) {
+          cbase[f] = (int16_t)nch;
+          ccnt[f] = (int16_t)nov;
         }
-        nn += fresh;
+        for (int k0 = 0; k0 < nov && !overflow; k0 += 64) {
+          const int idx = push_slots(f, k0 + lane, k0 + lane < nov);
+          if (overflow) break;
+          if (k0 + lane < nov) child[nch + k0 + lane] = (int16_t)idx;
+          __builtin_amdgcn_wave_barrier();
+        }
+        nch += nov;
+        fi += 1;
+      } else {
+        const int total = __shfl(cum, nb - 1);
+        if (nch + total > kProvChild) {
+          overflow = true;
+          break;
+        }
+        const int start = cum - bnov;
+        if (lane < nb) {
+          cbase[bf] = (int16_t)(nch + start);
+          ccnt[bf] = (int16_t)bnov;
+          for (int j = 0; j < bnov; ++j) {
+            sown[start + j] = (uint8_t)lane;
+            sk[start + j] = (uint8_t)j;
+          }
+        }
         __builtin_amdgcn_wave_barrier();
-        if (cj >= 0 && !mine) idx = hval[h];
-        if (k < nov) child[nch + k] = (int16_t)idx;
+        const bool act = lane < total;
+        const int b = act ? sown[lane] : 0, k = act ? sk[lane] : 0;
+        const int f = act ? front[fi + b] : 0;
+        const int idx = push_slots(f, k, act);
+        if (overflow) break;
+        if (act) child[nch + lane] = (int16_t)idx;
         __builtin_amdgcn_wave_barrier();
+        nch += total;
+        fi += nb;
       }
-      nch += nov;
       __syncthreads();
     }
     if (overflow) break;
@@ -485,9 +538,12 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
     return;
   }
   if (lane == 0) {
-    const unsigned long long no = atomicAdd(g.used, (unsigned long long)np);
-    const unsigned long long co = atomicAdd(g.used + 1, (unsigned long long)nch2);
-    if (no + np > (unsigned long long)g.node_cap || co + nch2 > (unsigned long long)g.child_cap) {
+    // nodes and pushes start at multiples of 8 elements (k_flow loads them
+    // 16 bytes at a time)
+    const unsigned long long np8 = (unsigned long long)((np + 7) & ~7), nc8 = (unsigned long long)((nch2 + 7) & ~7);
+    const unsigned long long no = atomicAdd(g.used, np8);
+    const unsigned long long co = atomicAdd(g.used + 1, nc8);
+    if (no + np8 > (unsigned long long)g.node_cap || co + nc8 > (unsigned long long)g.child_cap) {
       offs[0] = -1;
       atomicMin(g.no_room, ws);
       g.n[ws] = -2;
@@ -648,6 +704,9 @@ struct Flow {
   const int32_t* end;      // [pixel] end of its list
   const int32_t* lists;
   long long wait_ticks;    // how long no walk may finish before the waves give up
+  int64_t seed_base;       // the image's first global pixel id
+  int32_t seed_w;          // its width
+  int32_t window_first;    // the window's first seed
 };
 
 // A set of up to kNodeCap = 256 nodes in four 64-bit words (named, not an
@@ -678,26 +737,56 @@ struct NodeSet {
 // when popped (a node is never untaken), so it is not stacked; the others
 // are stacked in push order, one lane per push.  Every node is taken at
 // most once, so at most (pushes + 1) entries are stacked.
+//   * The last entry a node pushes is the next one popped, and it is neither
+//     masked nor taken (nothing is taken between its push and its pop): it is
+//     taken straight from the pushing lane's registers.
+//   * Otherwise the stack's top 64 entries are read at once and the topmost
+//     one neither masked nor taken is popped; the ones above it would have
+//     been popped and skipped one by one.
+//   * A node's first push and push count come from the lanes' registers:
+//     info[q] of lane l holds node 64 q + l as (first << 16) | count.
 __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm, const NodeSet ok,
-                                                             const int16_t* nbase, const int16_t* ncnt,
-                                                             const int16_t* child, int32_t* stack, int16_t* accl) {
+                                                             const int32_t info[4], const int16_t* child,
+                                                             int32_t* stack, int16_t* accl) {
   const int lane = threadIdx.x % 64;
   NodeSet taken{0ull, 0ull, 0ull, 0ull};
-  if (lane == 0) stack[0] = 0;  // node 0 at depth 0
-  __builtin_amdgcn_wave_barrier();
-  int sp = 1, na = 0;
-  while (sp > 0) {
-    --sp;
-    const int e = __builtin_amdgcn_readfirstlane(stack[sp]);
+  int sp = 0, na = 0;
+  int e = 0;                 // the entry to visit next: node 0 at depth 0
+  bool have = ok.has(0);
+  for (;;) {
+    if (!have) {
+      while (sp > 0) {
+        const int k = sp - 1 - lane;
+        int x = 0;
+        bool v = false;
+        if (k >= 0) {
+          x = stack[k];
+          v = ok.has(x & 0xffff) && !taken.has(x & 0xffff);
+        }
+        const uint64_t m = __ballot(v);
+        if (m) {
+          const int l = __ffsll((long long)m) - 1;
+          e = __builtin_amdgcn_readlane(x, l);
+          sp -= l + 1;
+          have = true;
+          break;
+        }
+        sp = sp > 64 ? sp - 64 : 0;
+      }
+      if (!have) break;
+    }
+    have = false;
     const int node = e & 0xffff, d = e >> 16;
-    if (!ok.has(node) || taken.has(node)) continue;
     taken.add(node);
     if (lane == 0) accl[na] = (int16_t)node;
     ++na;
     if (na >= prm.max_num_pixels) break;
     if (d + 1 >= prm.max_traversal) continue;
-    const int b = __builtin_amdgcn_readfirstlane(nbase[node]);
-    const int cnt = __builtin_amdgcn_readfirstlane(ncnt[node]);
+    const int q = node >> 6;
+    const int iv = q == 0 ? info[0] : (q == 1 ? info[1] : (q == 2 ? info[2] : info[3]));
+    const int nfo = __builtin_amdgcn_readlane(iv, node & 63);
+    const int b = nfo >> 16, cnt = nfo & 0xffff;
+    const int val0 = (d + 1) << 16;
     for (int k0 = 0; k0 < cnt; k0 += 64) {
       const int k = k0 + lane;
       int c = 0;
@@ -707,10 +796,15 @@ __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm
         push = ok.has(c) && !taken.has(c);
       }
       const uint64_t m = __ballot(push);
-      if (push) stack[sp + __popcll(m & ((1ull << lane) - 1))] = ((d + 1) << 16) | c;
+      if (push) stack[sp + __popcll(m & ((1ull << lane) - 1))] = val0 | c;
       sp += __popcll(m);
+      if (m) {
+        e = __builtin_amdgcn_readlane(val0 | c, 63 - __clzll((long long)m));
+        have = true;
+      }
     }
-    __builtin_amdgcn_wave_barrier();  // the wave's LDS writes precede its next pop
+    if (have) --sp;  // the last push, popped from the registers
+    __builtin_amdgcn_wave_barrier();  // the wave's LDS writes precede its next reads
   }
   return na;
 }
@@ -789,9 +883,10 @@ __global__ void __launch_bounds__(256) k_succ(Pool g, Flow f) {
 // claiming a slot below n_tickets gets its walk; a wave gives up only when no
 // walk anywhere finished for wait_ticks.
 __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* mask, Flow f) {
-  __shared__ int64_t ngid[kNodeCap];
-  __shared__ int16_t nbase[kNodeCap], ncnt[kNodeCap];
-  __shared__ int16_t child[kChildCap];
+  __shared__ __attribute__((aligned(16))) int64_t ngid[kNodeCap];
+  __shared__ __attribute__((aligned(16))) int16_t nbase[kNodeCap];
+  __shared__ __attribute__((aligned(16))) int16_t ncnt[kNodeCap];
+  __shared__ __attribute__((aligned(16))) int16_t child[kChildCap];
   __shared__ int32_t stack[kChildCap + 1];
   __shared__ int16_t accl[kNodeCap];
   const int lane = threadIdx.x;
@@ -803,12 +898,16 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
     if (h >= f.n_tickets) break;
     int ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (ws < 0) {
+      // Most waves wait here at any time: each polls only its own slot, and
+      // looks at the shared counters (stop flag, walks finished) once every
+      // kPollCheck polls, so that the waiting adds little to the memory
+      // traffic the running walks wait on.
       long long t0 = wall_clock64();
       int seen = __hip_atomic_load(f.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      while (ws < 0) {
-        __builtin_amdgcn_s_sleep(2);
+      for (int it = 1; ws < 0; ++it) {
+        __builtin_amdgcn_s_sleep(kPollSleep);
         ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (ws >= 0) break;
+        if (ws >= 0 || it % kPollCheck != 0) continue;
         if (__hip_atomic_load(f.counters + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         const int fin = __hip_atomic_load(f.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (fin != seen) {
@@ -826,28 +925,51 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
     const int64_t no = g.node_off[ws], co = g.child_off[ws];
     const int np = f.npred[ws], ns = f.nsucc[ws], se = f.succ_end[ws];
     // what the end of the walk needs, loaded early: the longest chain behind
-    // it and its successors
+    // it (a statistic: a predecessor's depth is stored without ordering
+    // against its countdown and may still read as the segment's 0, so the
+    // chain counted is a lower bound) and its successors
     int dep = 0;
     for (int k = lane; k < np; k += 64)
       dep = max(dep, __hip_atomic_load(f.depth + f.preds[no + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const int sw0 = lane < ns ? f.succ[se - ns + lane] : -1;
-    // the seed first (node 0): taken by an earlier walk, this walk accepts
-    // nothing, and its graph is not needed.  The masks are final for the
-    // walk's pixels: every earlier walk that could take one has finished.
+    // the seed first (node 0, its pixel from the seed's number): taken by an
+    // earlier walk, this walk accepts nothing, and its graph is not needed.
+    // The masks are final for the walk's pixels: every earlier walk that
+    // could take one has finished.
+    const int sd = f.window_first + ws;
+    const int64_t seed_gid = f.seed_base + (int64_t)(3 + (sd / prm.seed_cols) * prm.step) * f.seed_w +
+                             (3 + (sd % prm.seed_cols) * prm.step);
     int na = 0;
-    if (!masked_now(mask, g.gid[no])) {
-#pragma unroll
-      for (int q = 0; q < kNodeCap / 64; ++q) {
-        const int i = 64 * q + lane;
-        if (i < nn) {
-          ngid[i] = g.gid[no + i];
-          nbase[i] = g.cbase[no + i];
-          ncnt[i] = g.ccnt[no + i];
-        }
-      }
-#pragma unroll 8
-      for (int c = lane; c < nch; c += 64) child[c] = g.child[co + c];
+    if (!masked_now(mask, seed_gid)) {
+      // the graph into LDS, every load in flight at once: a graph's nodes
+      // and pushes start at multiples of 8 elements, so 16-byte loads cover
+      // them (the tails past nn and nch are never read)
+      const int ol = lane & 31;
+      const bool lg0 = 2 * lane < nn, lg1 = 2 * (64 + lane) < nn, lc = 8 * ol < nn;
+      const bool lh0 = 8 * lane < nch, lh1 = 8 * (64 + lane) < nch, lh2 = 8 * (128 + lane) < nch,
+                 lh3 = 8 * (192 + lane) < nch;
+      const uint4 z4 = {0u, 0u, 0u, 0u};
+      const uint4 g0 = lg0 ? *(const uint4*)(g.gid + no + 2 * lane) : z4;
+      const uint4 g1 = lg1 ? *(const uint4*)(g.gid + no + 2 * (64 + lane)) : z4;
+      const uint4 c0 = lc ? *(const uint4*)((lane < 32 ? g.cbase : g.ccnt) + no + 8 * ol) : z4;
+      const uint4 h0 = lh0 ? *(const uint4*)(g.child + co + 8 * lane) : z4;
+      const uint4 h1 = lh1 ? *(const uint4*)(g.child + co + 8 * (64 + lane)) : z4;
+      const uint4 h2 = lh2 ? *(const uint4*)(g.child + co + 8 * (128 + lane)) : z4;
+      const uint4 h3 = lh3 ? *(const uint4*)(g.child + co + 8 * (192 + lane)) : z4;
+      if (lg0) *(uint4*)(ngid + 2 * lane) = g0;
+      if (lg1) *(uint4*)(ngid + 2 * (64 + lane)) = g1;
+      if (lc) *(uint4*)((lane < 32 ? nbase : ncnt) + 8 * ol) = c0;
+      if (lh0) *(uint4*)(child + 8 * lane) = h0;
+      if (lh1) *(uint4*)(child + 8 * (64 + lane)) = h1;
+      if (lh2) *(uint4*)(child + 8 * (128 + lane)) = h2;
+      if (lh3) *(uint4*)(child + 8 * (192 + lane)) = h3;
       __builtin_amdgcn_wave_barrier();
+      int32_t info[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 64 * r + lane;
+        info[r] = i < nn ? (int32_t)(((uint32_t)(uint16_t)nbase[i] << 16) | (uint16_t)ncnt[i]) : 0;
+      }
       NodeSet ok;
       const bool m0 = lane < nn && masked_now(mask, ngid[lane]);
       const bool m1 = 64 + lane < nn && masked_now(mask, ngid[64 + lane]);
@@ -857,7 +979,7 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
       ok.w1 = __ballot(64 + lane < nn && !m1);
       ok.w2 = __ballot(128 + lane < nn && !m2);
       ok.w3 = __ballot(192 + lane < nn && !m3);
-      na = replay(prm, ok, nbase, ncnt, child, stack, accl);
+      na = replay(prm, ok, info, child, stack, accl);
       __builtin_amdgcn_wave_barrier();
       for (int j = lane; j < na; j += 64) {
         const int node = accl[j];
@@ -1429,7 +1551,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   f.end = w_end.as<const int32_t>();
   f.lists = w_lists.as<const int32_t>();
   f.wait_ticks = (long long)(kWaitSeconds * 1000.0 * std::max(clock_khz, 1000));
-  const unsigned flow_grid = (unsigned)std::max(1, cus * kFlowWavesPerCU);
+  const unsigned flow_grid = (unsigned)std::max(1, cus * kFlowWavesPer8CU / 8);
 
   // a walk run alone: buffers for max_num_pixels
   const int big_acc = o->max_num_pixels;
@@ -1514,12 +1636,16 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
           // the segment's dependencies: predecessors, successor lists, the
           // walks ready at the start; then the dataflow
           Flow fs = f;
+          fs.seed_base = tab[cur].base;
+          fs.seed_w = W;
+          fs.window_first = (int32_t)w0;
           fs.tickets = f.tickets + t_begin;
           fs.n_tickets = (int32_t)(tk - t_begin);
           fs.seg_lo = tickets[t_begin];
           const int nts = fs.n_tickets;
           const int64_t span = stop - fs.seg_lo;  // seeds [seg_lo, stop) hold every predecessor and successor
           hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(span)), dim3(256), 0, st, f.nsucc + fs.seg_lo, span, 0);
+          hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(span)), dim3(256), 0, st, f.depth + fs.seg_lo, span, 0);
           hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(nts)), dim3(256), 0, st, f.queue, (int64_t)nts, -1);
           hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, f.counters, (int64_t)8, 0);
           hipLaunchKernelGGL(k_preds, dim3((nts + kPredWaves - 1) / kPredWaves), dim3(64 * kPredWaves), 0, st, g, fs);
