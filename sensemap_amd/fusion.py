@@ -71,8 +71,8 @@ class FusionInputs:
             d = np.ascontiguousarray(im.depth, np.float32)
             H, W = d.shape
             col = np.asarray(im.color, np.uint8)
-            if col.ndim == 2:
-                col = np.repeat(col[:, :, None], 3, axis=2)
+            if col.ndim == 2:  # grey: the same value in the three channels
+                col = np.stack([col, col, col], axis=-1)
             col = np.ascontiguousarray(col)
             ov = np.ascontiguousarray(np.asarray(list(im.overlapping), np.int32).reshape(-1))
             self._keep += [d, col, ov]
