@@ -1248,6 +1248,33 @@ __global__ void __launch_bounds__(256) k_gather_vis(const int32_t* __restrict__ 
   for (int i = lane; i < nv; i += 64) out_vis[at + i] = vis_pool[no + i];
 }
 
+// The window's seeds with a graph (tickets, in seed order) and the seeds
+// whose walks run alone (barriers, each with the number of tickets before
+// it), listed on the device: flags, exclusive prefix sums, a scatter.  Seeds
+// from the first one that found the pool full (no_room) on are left out.
+__global__ void __launch_bounds__(256) k_ticket_flags(const int32_t* __restrict__ n, int32_t count,
+                                                      const int32_t* __restrict__ no_room, int32_t* __restrict__ ft,
+                                                      int32_t* __restrict__ fb) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= count) return;
+  const int v = k < *no_room ? n[k] : 0;
+  ft[k] = v > 0 ? 1 : 0;
+  fb[k] = v == -1 ? 1 : 0;
+}
+__global__ void __launch_bounds__(256) k_ticket_scatter(const int32_t* __restrict__ n, int32_t count,
+                                                        const int32_t* __restrict__ no_room,
+                                                        const int32_t* __restrict__ pt, const int32_t* __restrict__ pb,
+                                                        int32_t* __restrict__ tickets, int32_t* __restrict__ bars) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= count || k >= *no_room) return;
+  const int v = n[k];
+  if (v > 0) tickets[pt[k]] = k;
+  if (v == -1) {
+    bars[2 * pb[k]] = k;
+    bars[2 * pb[k] + 1] = pt[k];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
@@ -1476,7 +1503,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   Buf p_n, p_nc, p_noff, p_coff, p_gid, p_img, p_cbase, p_ccnt, p_child, p_used, p_room;
   Buf w_cnt, w_end, w_lists, w_sums, w_total, w_tickets, w_queue, w_counters, w_pending, w_npred, w_preds, w_nsucc,
       w_succend, w_succ, w_depth, w_rescnt, w_resnodes, w_status, w_pts, w_has, w_rank, w_visp, w_out, w_oseed, w_onvis,
-      w_visat, w_ovis, w_kept;
+      w_visat, w_ovis, w_kept, w_bars;
   const size_t ws_b = (size_t)wcap;
   const size_t nd_b = (size_t)node_cap;
   if (pm_status s = p_n.get(ws_b * 4)) return s;
@@ -1497,6 +1524,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   if (pm_status s = w_sums.get((size_t)((scan_n + kScanTile - 1) / kScanTile + 1) * 4)) return s;
   if (pm_status s = w_total.get(8)) return s;
   if (pm_status s = w_tickets.get(ws_b * 4)) return s;
+  if (pm_status s = w_bars.get(ws_b * 8)) return s;
   if (pm_status s = w_queue.get(ws_b * 4)) return s;
   if (pm_status s = w_counters.get(32)) return s;
   if (pm_status s = w_pending.get(ws_b * 4)) return s;
@@ -1562,8 +1590,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   std::unique_ptr<pm_fusion> res(new (std::nothrow) pm_fusion());
   if (!res) return fail(PM_ERR_OUT_OF_MEMORY, "host allocation");
   pm_fusion_stats& stt = res->stats;
-  std::vector<int32_t> nodes_of, tickets;
-  std::vector<int64_t> barriers;
+  std::vector<int32_t> bars_h;
   std::vector<PointRec> pts_h;
   std::vector<int32_t> seed_h, vis_h;
   struct Alone {
@@ -1594,25 +1621,31 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
       hipLaunchKernelGGL(k_window_reset, dim3(1), dim3(1), 0, st, g.used, g.no_room, nwin, w_status.as<int32_t>());
       hipLaunchKernelGGL(k_graph, dim3(nwin), dim3(64), 0, st, (const FusImgDev*)d_tab.p, (const uint32_t*)mask,
                          (const uint8_t*)d_fused.p, prm, (int32_t)w0, nwin, g);
+      // the window's tickets and barriers (uses w_has, w_rank, w_onvis and
+      // w_visat before the points do)
+      const unsigned tgrid = (unsigned)((nwin + 255) / 256);
+      hipLaunchKernelGGL(k_ticket_flags, dim3(tgrid), dim3(256), 0, st, (const int32_t*)g.n, nwin,
+                         (const int32_t*)g.no_room, w_has.as<int32_t>(), w_onvis.as<int32_t>());
+      scan(w_has.as<const int32_t>(), w_rank.as<int32_t>(), nwin, w_sums.as<int32_t>(), w_total.as<int32_t>(), st);
+      scan(w_onvis.as<const int32_t>(), w_visat.as<int32_t>(), nwin, w_sums.as<int32_t>(), w_total.as<int32_t>() + 1,
+           st);
+      hipLaunchKernelGGL(k_ticket_scatter, dim3(tgrid), dim3(256), 0, st, (const int32_t*)g.n, nwin,
+                         (const int32_t*)g.no_room, (const int32_t*)w_rank.p, (const int32_t*)w_visat.p,
+                         w_tickets.as<int32_t>(), w_bars.as<int32_t>());
       PM_HIP(hipGetLastError());
-      int32_t m = 0;
-      PM_HIP(hipMemcpyAsync(&m, g.no_room, 4, hipMemcpyDeviceToHost, st));
+      int32_t cnts[3] = {0, 0, 0};  // seeds with room, tickets, barriers
+      PM_HIP(hipMemcpyAsync(&cnts[0], g.no_room, 4, hipMemcpyDeviceToHost, st));
+      PM_HIP(hipMemcpyAsync(&cnts[1], w_total.p, 8, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
+      const int32_t m = cnts[0], nt = cnts[1], nb = cnts[2];
       if (m == 0) {
         // the pool went to later seeds first: a window whose graphs all fit
         limit = std::max<int64_t>(1, node_cap / kNodeCap);
         continue;
       }
       const bool full = m < nwin;
-      nodes_of.resize(m);
-      PM_HIP(hipMemcpy(nodes_of.data(), g.n, (size_t)m * 4, hipMemcpyDeviceToHost));
-      tickets.clear();
-      barriers.clear();
-      for (int k = 0; k < m; ++k) {
-        if (nodes_of[k] > 0) tickets.push_back(k);
-        if (nodes_of[k] == -1) barriers.push_back(k);
-      }
-      const int nt = (int)tickets.size();
+      bars_h.resize((size_t)2 * nb);
+      if (nb > 0) PM_HIP(hipMemcpy(bars_h.data(), w_bars.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
       // the walks passing each pixel
       if (nt > 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(total)), dim3(256), 0, st, w_cnt.as<int32_t>(), total, 0);
@@ -1620,19 +1653,17 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
         scan(w_cnt.as<const int32_t>(), w_end.as<int32_t>(), total, w_sums.as<int32_t>(), w_total.as<int32_t>(), st);
         hipLaunchKernelGGL(k_scatter, dim3((m + 3) / 4), dim3(256), 0, st, g, m, w_end.as<int32_t>(),
                            w_lists.as<int32_t>());
-        PM_HIP(hipMemcpyAsync(w_tickets.p, tickets.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(m)), dim3(256), 0, st, f.res_cnt, (int64_t)m, 0);
       }
       hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(m)), dim3(256), 0, st, w_has.as<int32_t>(), (int64_t)m, 0);
       PM_HIP(hipGetLastError());
       // the dataflow, split at the walks that run alone
       PM_HIP(hipEventRecord(f0, st));
-      size_t tk = 0;
-      for (size_t bi = 0; bi <= barriers.size(); ++bi) {
-        const int64_t stop = bi < barriers.size() ? barriers[bi] : m;
-        const size_t t_begin = tk;
-        while (tk < tickets.size() && tickets[tk] < stop) ++tk;
-        if (tk > t_begin) {
+      int32_t t_begin = 0;
+      for (int32_t bi = 0; bi <= nb; ++bi) {
+        const int64_t stop = bi < nb ? bars_h[2 * bi] : m;
+        const int32_t t_end = bi < nb ? bars_h[2 * bi + 1] : nt;
+        if (t_end > t_begin) {
           // the segment's dependencies: predecessors, successor lists, the
           // walks ready at the start; then the dataflow
           Flow fs = f;
@@ -1640,8 +1671,11 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
           fs.seed_w = W;
           fs.window_first = (int32_t)w0;
           fs.tickets = f.tickets + t_begin;
-          fs.n_tickets = (int32_t)(tk - t_begin);
-          fs.seg_lo = tickets[t_begin];
+          fs.n_tickets = t_end - t_begin;
+          // the seed after the last walk run alone: no earlier seed's walk
+          // is a predecessor here, and none from it to the segment's first
+          // ticket has a graph
+          fs.seg_lo = bi == 0 ? 0 : bars_h[2 * (bi - 1)] + 1;
           const int nts = fs.n_tickets;
           const int64_t span = stop - fs.seg_lo;  // seeds [seg_lo, stop) hold every predecessor and successor
           hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(span)), dim3(256), 0, st, f.nsucc + fs.seg_lo, span, 0);
@@ -1659,9 +1693,10 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
           stt.rounds++;
           stt.walks += nts;
         }
-        if (bi == barriers.size()) break;
+        t_begin = t_end;
+        if (bi == nb) break;
         // the walk itself, pixel by pixel, with buffers for max_num_pixels
-        const int32_t bws = (int32_t)barriers[bi];
+        const int32_t bws = bars_h[2 * bi];
         if (pm_status s = b_rec.get(sizeof(WalkRec))) return s;
         if (pm_status s = b_gids.get((size_t)big_acc * sizeof(int64_t))) return s;
         if (pm_status s = b_acc.get((size_t)big_acc * sizeof(AccEntry))) return s;
