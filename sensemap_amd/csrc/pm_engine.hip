@@ -291,16 +291,18 @@ namespace {
 constexpr size_t kStageBytes = size_t(16) << 20;  // per pinned buffer
 constexpr size_t kDirectBytes = size_t(1) << 20;  // below: straight through hipMemcpyAsync
 
-// memcpy split over up to four threads (the staging copies are CPU-bound)
+// memcpy split over up to eight threads of at least 2 MB each (the staging
+// copies are CPU-bound)
 void par_memcpy(void* d, const void* s, size_t n) {
   constexpr size_t kPart = size_t(2) << 20;
-  const size_t parts = std::min<size_t>(4, (n + kPart - 1) / kPart);
+  constexpr size_t kThreads = 8;
+  const size_t parts = std::min<size_t>(kThreads, (n + kPart - 1) / kPart);
   if (parts <= 1) {
     std::memcpy(d, s, n);
     return;
   }
   const size_t per = (n + parts - 1) / parts;
-  std::thread th[3];
+  std::thread th[kThreads - 1];
   for (size_t p = 1; p < parts; ++p) {
     const size_t o = p * per;
     th[p - 1] = std::thread([=] { std::memcpy((char*)d + o, (const char*)s + o, std::min(per, n - o)); });
