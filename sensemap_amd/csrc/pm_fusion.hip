@@ -890,13 +890,17 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
   __shared__ int32_t stack[kChildCap + 1];
   __shared__ int16_t accl[kNodeCap];
   const int lane = threadIdx.x;
+  int next = -1;  // a walk this wave made ready and runs next, past the queue (wave-uniform)
   for (;;) {
+    int ws = next;
+    next = -1;
+    if (ws < 0) {
     // the slot: every lane takes part in the atomic (only lane 0 adds), the
     // result is wave-uniform.  A lane-0-only branch here lets the compiler
     // run lane 0 and the other lanes round different loops.
     const int h = __builtin_amdgcn_readfirstlane(atomicAdd(f.counters, lane == 0 ? 1 : 0));
     if (h >= f.n_tickets) break;
-    int ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ws = __hip_atomic_load(f.queue + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (ws < 0) {
       // Most waves wait here at any time: each polls only its own slot, and
       // looks at the shared counters (stop flag, walks finished) once every
@@ -910,6 +914,7 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
         if (ws >= 0 || it % kPollCheck != 0) continue;
         if (__hip_atomic_load(f.counters + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         const int fin = __hip_atomic_load(f.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (fin >= f.n_tickets) break;  // every walk has finished: this slot stays empty
         if (fin != seen) {
           seen = fin;
           t0 = wall_clock64();
@@ -920,7 +925,8 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
       }
     }
     ws = __builtin_amdgcn_readfirstlane(ws);
-    if (ws < 0) break;  // stopped
+    if (ws < 0) break;  // stopped, or every walk finished
+    }
     const int nn = g.n[ws], nch = g.nc[ws];
     const int64_t no = g.node_off[ws], co = g.child_off[ws];
     const int np = f.npred[ws], ns = f.nsucc[ws], se = f.succ_end[ws];
@@ -996,11 +1002,19 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
     }
     // publish the mask bits (a walk that took no pixel wrote none)
     if (na > 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    // the successors this walk was the last one for
+    // the successors this walk was the last one for: the first one this
+    // wave runs next itself (no queue slot and no poll on its way), the
+    // others go to the queue.  The queue then fills fewer slots than there
+    // are walks, and the waves waiting on the rest leave once every walk has
+    // finished.
     for (int k0 = 0; k0 < ns; k0 += 64) {
       const int sw = k0 == 0 ? sw0 : (k0 + lane < ns ? f.succ[se - ns + k0 + lane] : -1);
-      if (sw >= 0 && atomicSub(f.pending + sw, 1) == 1)
+      const bool ready = sw >= 0 && atomicSub(f.pending + sw, 1) == 1;
+      const uint64_t rm = __ballot(ready);
+      const int kl = next < 0 && rm ? __ffsll((long long)rm) - 1 : -1;
+      if (ready && lane != kl)
         __hip_atomic_store(f.queue + atomicAdd(f.counters + 1, 1), sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (kl >= 0) next = __builtin_amdgcn_readlane(sw, kl);
     }
     if (lane == 0) atomicAdd(f.counters + 2, 1);
   }
