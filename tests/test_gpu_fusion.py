@@ -1,8 +1,9 @@
 """Depth-map fusion on the device (pm_fusion_run) equals its transcription
 bit for bit -- positions, normals, colours, scores, visibility and the order
 of the points -- on the designed plane scenes, on an orbit of rendered views
-with noisy depths, and on a scene whose walks outgrow the per-walk buffers
-(rerun alone with buffers for max_num_pixels)."""
+with noisy depths, on a scene whose walks outgrow the per-walk buffers
+(rerun alone with buffers for max_num_pixels), and on the 11 views of the
+1920x1080 heightfield scene."""
 import numpy as np
 import pytest
 
@@ -92,4 +93,29 @@ def test_walks_that_outgrow_their_buffers():
     s = stats(eng, ims, o)
     print("[fusion] dense walks: points", len(got["score"]), "alone", s.big_walks, "rounds", s.rounds, flush=True)
     assert s.big_walks > 0
+    same(got, want)
+
+
+def test_full_size_eleven_views_equal_transcription():
+    # the 1920x1080 heightfield scene, all 11 views fused (each overlapping the
+    # other 10), depths with 0.2% noise: about 2.7 M walks over 13 dataflow
+    # segments, every point bitwise equal to the transcription's
+    from sensemap_amd import scene as sc_mod
+    sc = sc_mod.heightfield_scene(seed=2)
+    views = [sc.ref] + sc.srcs
+    n = len(views)
+    rng = np.random.default_rng(2)
+    ims = []
+    for k, v in enumerate(views):
+        d = (v.depth * (1 + 0.002 * rng.standard_normal(v.depth.shape))).astype(np.float32)
+        ims.append(FusionImage(v.camera.K, v.camera.R, v.camera.T, d, v.normal, v.image,
+                               [j for j in range(n) if j != k]))
+    o = StereoFusionOptions()
+    eng = pmm.Engine(0)
+    got = fuse(ims, o, eng)
+    s = stats(eng, ims, o)
+    print("[fusion] full size: points", len(got["score"]), "walks", s.walks, "kept", s.kept, "rounds", s.rounds,
+          "chain", s.chain, "ms", round(s.total_ms, 1), flush=True)
+    want = binding.fusion(ims, o)
+    assert len(got["score"]) > 100000
     same(got, want)
