@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
   __shared__ int16_t child[kProvChild];
   __shared__ int16_t front[kProvCap];  // the frontier; after the walk, each pixel's node (-1: fails)
   __shared__ int16_t vfirst[kProvCap];  // first stored push of each pixel
-  __shared__ uint8_t sown[64], sk[64];  // a push slot's frontier pixel (in the batch) and candidate
+  __shared__ uint8_t sown[128], sk[128];  // a push slot's frontier pixel (in the batch) and candidate
   __shared__ long long offs[2];
   const int ws = blockIdx.x, lane = threadIdx.x;
   if (ws >= count) return;
@@ -354,70 +354,91 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
   int nn = 1, nch = 0, nf = 1 < prm.max_traversal ? 1 : 0, level = 0;
   bool overflow = false;
   __syncthreads();
-  // The pushes of frontier pixel f, slots [s, s + n) of the level's pushes:
-  // candidate k is f's projection into its k-th overlapping image.  The
-  // candidates' pixels are claimed all lanes at once: a compare-and-swap on
-  // the hash key claims a new pixel, and the new ones get numbers in lane
-  // order (the numbering does not affect the walk, only the push order does,
-  // and each push keeps its slot).  Returns the pushed pixel's number (-1:
-  // none), or sets overflow when the pixels outgrow the graph.
-  auto push_slots = [&](int f, int k, bool active) __attribute__((always_inline)) {
-    int cj = -1, crow = 0, ccol = 0;
-    if (active) {
-      const FusImgDev& F = imgs[nimg[f]];
-      const int j = F.ov[k];
-      if (j >= 0 && j < prm.n_images && imgs[j].used && !fused[j]) {
-        const FusImgDev& J = imgs[j];
-        const float x = nxyz[f][0], y = nxyz[f][1], z = nxyz[f][2];
-        const float q0 = row4(J.P, x, y, z), q1 = row4(J.P + 4, x, y, z), q2 = row4(J.P + 8, x, y, z);
-        const float fc = roundf(q0 / q2), fr = roundf(q1 / q2);
-        if (fabsf(fc) < 1073741824.0f && fabsf(fr) < 1073741824.0f) {
-          const int nc = (int)fc, nr = (int)fr;
-          if (!(nc < 3 || nr < 3 || nc >= J.W - 3 || nr >= J.H - 3)) {
-            cj = j;
-            crow = nr;
-            ccol = nc;
-          }
+  // Pushes, two slots per lane (slots lane and 64 + lane of a batch): a push
+  // of frontier pixel f is its projection into its k-th overlapping image.
+  // The projections' loads are in flight together; the candidates' pixels
+  // are claimed all lanes at once, a compare-and-swap on the hash key
+  // claiming a new pixel; the new ones get numbers in slot order (the
+  // numbering does not affect the walk, only the push order does, and each
+  // push keeps its slot).  idx is the pushed pixel's number (-1: none);
+  // overflow is set when the pixels outgrow the graph.
+  auto project = [&](int f, int k, bool active, int& cj, int& crow, int& ccol) __attribute__((always_inline)) {
+    cj = -1;
+    crow = 0;
+    ccol = 0;
+    if (!active) return;
+    const FusImgDev& F = imgs[nimg[f]];
+    const int j = F.ov[k];
+    if (j >= 0 && j < prm.n_images && imgs[j].used && !fused[j]) {
+      const FusImgDev& J = imgs[j];
+      const float x = nxyz[f][0], y = nxyz[f][1], z = nxyz[f][2];
+      const float q0 = row4(J.P, x, y, z), q1 = row4(J.P + 4, x, y, z), q2 = row4(J.P + 8, x, y, z);
+      const float fc = roundf(q0 / q2), fr = roundf(q1 / q2);
+      if (fabsf(fc) < 1073741824.0f && fabsf(fr) < 1073741824.0f) {
+        const int nc = (int)fc, nr = (int)fr;
+        if (!(nc < 3 || nr < 3 || nc >= J.W - 3 || nr >= J.H - 3)) {
+          cj = j;
+          crow = nr;
+          ccol = nc;
         }
       }
     }
-    int h = 0;
-    bool mine = false;
-    if (cj >= 0) {
-      const int64_t gid = imgs[cj].base + (int64_t)crow * imgs[cj].W + ccol;
-      h = hash_slot(gid);
-      for (;;) {
-        const unsigned long long old = atomicCAS((unsigned long long*)&hkey[h], ~0ull, (unsigned long long)gid);
-        if (old == ~0ull) {
-          mine = true;
-          break;
-        }
-        if ((int64_t)old == gid) break;
-        h = (h + 1) & (kHash - 1);
+  };
+  auto claim = [&](int cj, int crow, int ccol, int& h, bool& mine) __attribute__((always_inline)) {
+    h = 0;
+    mine = false;
+    if (cj < 0) return;
+    const int64_t gid = imgs[cj].base + (int64_t)crow * imgs[cj].W + ccol;
+    h = hash_slot(gid);
+    for (;;) {
+      const unsigned long long old = atomicCAS((unsigned long long*)&hkey[h], ~0ull, (unsigned long long)gid);
+      if (old == ~0ull) {
+        mine = true;
+        break;
       }
+      if ((int64_t)old == gid) break;
+      h = (h + 1) & (kHash - 1);
     }
-    const uint64_t mm = __ballot(mine);
-    const int fresh = __popcll(mm);
+  };
+  auto push2 = [&](int f0, int k0, bool a0, int f1, int k1, bool a1, int& idx0, int& idx1)
+                   __attribute__((always_inline)) {
+    int cj0, r0, c0, cj1, r1, c1, h0, h1;
+    bool m0, m1;
+    project(f0, k0, a0, cj0, r0, c0);
+    project(f1, k1, a1, cj1, r1, c1);
+    claim(cj0, r0, c0, h0, m0);
+    claim(cj1, r1, c1, h1, m1);
+    const uint64_t mm0 = __ballot(m0), mm1 = __ballot(m1);
+    const int fresh0 = __popcll(mm0), fresh = fresh0 + __popcll(mm1);
+    idx0 = -1;
+    idx1 = -1;
     if (nn + fresh > kProvCap) {
       overflow = true;
-      return -1;
+      return;
     }
-    int idx = -1;
-    if (mine) {
-      idx = nn + __popcll(mm & ((1ull << lane) - 1));
-      hval[h] = (int16_t)idx;
-      nimg[idx] = (uint16_t)cj;
-      nrc[idx] = (crow << 16) | ccol;
+    const uint64_t lt = (1ull << lane) - 1;
+    if (m0) {
+      idx0 = nn + __popcll(mm0 & lt);
+      hval[h0] = (int16_t)idx0;
+      nimg[idx0] = (uint16_t)cj0;
+      nrc[idx0] = (r0 << 16) | c0;
+    }
+    if (m1) {
+      idx1 = nn + fresh0 + __popcll(mm1 & lt);
+      hval[h1] = (int16_t)idx1;
+      nimg[idx1] = (uint16_t)cj1;
+      nrc[idx1] = (r1 << 16) | c1;
     }
     nn += fresh;
     __builtin_amdgcn_wave_barrier();
-    if (cj >= 0 && !mine) idx = hval[h];
-    return idx;
+    if (cj0 >= 0 && !m0) idx0 = hval[h0];
+    if (cj1 >= 0 && !m1) idx1 = hval[h1];
   };
+  constexpr int kBatch = 128;  // push slots of one batch
   while (nf > 0 && !overflow) {
     const int level_start = nn;
     for (int fi = 0; fi < nf && !overflow;) {
-      // the next frontier pixels whose pushes fit the wave's 64 lanes
+      // the next frontier pixels whose pushes fit the batch's 128 slots
       // together, in frontier order: their pushes are the same slots as
       // one pixel after another
       const int bf = fi + lane < nf ? front[fi + lane] : -1;
@@ -427,9 +448,9 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
         const int t = __shfl_up(cum, d);
         if (lane >= d) cum += t;
       }
-      const int nb = __popcll(__ballot(bf >= 0 && cum <= 64));  // a prefix of the lanes
+      const int nb = __popcll(__ballot(bf >= 0 && cum <= kBatch));  // a prefix of the lanes
       if (nb == 0) {
-        // a pixel with more pushes than lanes: 64 at a time
+        // a pixel with more pushes than a batch: 128 at a time
         const int f = __builtin_amdgcn_readfirstlane(bf);
         const int nov = __builtin_amdgcn_readfirstlane(bnov);
         if (nch + nov > kProvChild) {
@@ -440,10 +461,13 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
           cbase[f] = (int16_t)nch;
           ccnt[f] = (int16_t)nov;
         }
-        for (int k0 = 0; k0 < nov && !overflow; k0 += 64) {
-          const int idx = push_slots(f, k0 + lane, k0 + lane < nov);
+        for (int k0 = 0; k0 < nov && !overflow; k0 += kBatch) {
+          const bool a0 = k0 + lane < nov, a1 = k0 + 64 + lane < nov;
+          int idx0, idx1;
+          push2(f, k0 + lane, a0, f, k0 + 64 + lane, a1, idx0, idx1);
           if (overflow) break;
-          if (k0 + lane < nov) child[nch + k0 + lane] = (int16_t)idx;
+          if (a0) child[nch + k0 + lane] = (int16_t)idx0;
+          if (a1) child[nch + k0 + 64 + lane] = (int16_t)idx1;
           __builtin_amdgcn_wave_barrier();
         }
         nch += nov;
@@ -464,12 +488,14 @@ __global__ void __launch_bounds__(64) k_graph(const FusImgDev* __restrict__ imgs
           }
         }
         __builtin_amdgcn_wave_barrier();
-        const bool act = lane < total;
-        const int b = act ? sown[lane] : 0, k = act ? sk[lane] : 0;
-        const int f = act ? front[fi + b] : 0;
-        const int idx = push_slots(f, k, act);
+        const bool a0 = lane < total, a1 = 64 + lane < total;
+        const int f0 = a0 ? front[fi + sown[lane]] : 0, k0 = a0 ? sk[lane] : 0;
+        const int f1 = a1 ? front[fi + sown[64 + lane]] : 0, k1 = a1 ? sk[64 + lane] : 0;
+        int idx0, idx1;
+        push2(f0, k0, a0, f1, k1, a1, idx0, idx1);
         if (overflow) break;
-        if (act) child[nch + lane] = (int16_t)idx;
+        if (a0) child[nch + lane] = (int16_t)idx0;
+        if (a1) child[nch + 64 + lane] = (int16_t)idx1;
         __builtin_amdgcn_wave_barrier();
         nch += total;
         fi += nb;
