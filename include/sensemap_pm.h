@@ -519,12 +519,12 @@ typedef struct pm_fusion_image {
   int32_t height;
   const float* depth;            /* width*height, <= 0 = no depth */
   const float* normal;           /* 3 planar width*height slices in the camera frame; may be NULL without normals */
-  const uint8_t* rgb;            /* width*height*3, interleaved r, g, b (Bitmap::GetPixel) */
+  const uint8_t* rgb;            /* width*height*channels: interleaved r, g, b (Bitmap::GetPixel), or grey */
   const int32_t* overlapping;    /* overlapping_images_[i]: GetMaxOverlappingImages(check_num_images, 0) */
   int32_t num_overlapping;
   int32_t is_rig;                /* Image::IsRig: FindNextImage takes non-rig images first */
   int32_t used;                  /* used_images_: the image has maps and takes part */
-  int32_t reserved0;
+  int32_t channels;              /* 3 (or 0): rgb holds r, g, b; 1: grey, the same value for all three */
 } pm_fusion_image;
 
 typedef struct pm_fusion pm_fusion;  /* the fused points of one call */

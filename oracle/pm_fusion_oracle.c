@@ -154,16 +154,18 @@ static int fuse(walk_ctx* w, struct pmo_fusion* out, int img0, int row0, int col
     const float* iP = w->iP[e.img];
     const float u = (float)e.col * depth, v = (float)e.row * depth;
     const float xyz[3] = {row4(iP, u, v, depth), row4(iP + 4, u, v, depth), row4(iP + 8, u, v, depth)};
-    const uint8_t* px = I->rgb + 3 * p;
+    /* channels 1: a grey image, its value as r, g and b */
+    const int ch = I->channels == 1 ? 1 : 3;
+    const uint8_t* px = I->rgb + (int64_t)ch * p;
     float grad = 0.0f;
     if (e.col > 1 && e.col < W - 1 && e.row > 1 && e.row < H - 1) {
-      const int l = I->rgb[3 * (p - 1)], r = I->rgb[3 * (p + 1)];
-      const int t = I->rgb[3 * (p - W)], b = I->rgb[3 * (p + W)];
+      const int l = I->rgb[ch * (p - 1)], r = I->rgb[ch * (p + 1)];
+      const int t = I->rgb[ch * (p - W)], b = I->rgb[ch * (p + W)];
       const float gx = (float)(l - r) / 255.0f, gy = (float)(t - b) / 255.0f;
       grad = sqrtf(gx * gx + gy * gy);
     }
     w->mask[e.img][p] = 1;
-    const float col_r = px[0], col_g = px[1], col_b = px[2];
+    const float col_r = px[0], col_g = px[ch == 3 ? 1 : 0], col_b = px[ch == 3 ? 2 : 0];
     if (vec_push(&w->fc, &grad) || vec_push(&w->fx, &xyz[0]) || vec_push(&w->fy, &xyz[1]) ||
         vec_push(&w->fz, &xyz[2]) || vec_push(&w->fr, &col_r) || vec_push(&w->fg, &col_g) ||
         vec_push(&w->fb, &col_b))

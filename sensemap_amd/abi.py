@@ -209,7 +209,7 @@ class pm_fusion_image(C.Structure):
     _fields_ = [("cam", pm_camera), ("width", C.c_int32), ("height", C.c_int32), ("depth", C.POINTER(C.c_float)),
                 ("normal", C.POINTER(C.c_float)), ("rgb", C.POINTER(C.c_uint8)),
                 ("overlapping", C.POINTER(C.c_int32)), ("num_overlapping", C.c_int32), ("is_rig", C.c_int32),
-                ("used", C.c_int32), ("reserved0", C.c_int32)]
+                ("used", C.c_int32), ("channels", C.c_int32)]
 
 
 class pm_fusion_stats(C.Structure):

@@ -82,7 +82,7 @@ struct FusImgDev {
   int64_t base;  // first global pixel id of the image
   int32_t W, H, nov, used;
   float P[12], iP[12], Rt[9];
-  int32_t pad[1];
+  int32_t ch;  // colour channels: 3 (r, g, b interleaved) or 1 (grey)
 };
 
 struct FusParams {
@@ -171,8 +171,8 @@ __global__ void __launch_bounds__(64) k_walk(const FusImgDev* __restrict__ imgs,
     const float xyz[3] = {row4(I.iP, u, v, depth), row4(I.iP + 4, u, v, depth), row4(I.iP + 8, u, v, depth)};
     float grad = 0.0f;
     if (e.col > 1 && e.col < W - 1 && e.row > 1 && e.row < H - 1) {
-      const int l = I.rgb[3 * (p - 1)], r = I.rgb[3 * (p + 1)];
-      const int t = I.rgb[3 * (p - W)], b = I.rgb[3 * (p + W)];
+      const int l = I.rgb[I.ch * (p - 1)], r = I.rgb[I.ch * (p + 1)];
+      const int t = I.rgb[I.ch * (p - W)], b = I.rgb[I.ch * (p + W)];
       const float gx = (float)(l - r) / 255.0f, gy = (float)(t - b) / 255.0f;
       grad = sqrtf(gx * gx + gy * gy);
     }
@@ -185,9 +185,10 @@ __global__ void __launch_bounds__(64) k_walk(const FusImgDev* __restrict__ imgs,
     a.v[3] = nrm[0];
     a.v[4] = nrm[1];
     a.v[5] = nrm[2];
-    a.v[6] = (float)I.rgb[3 * p];
-    a.v[7] = (float)I.rgb[3 * p + 1];
-    a.v[8] = (float)I.rgb[3 * p + 2];
+    const uint8_t* cp = I.rgb + I.ch * p;  // grey: channel 0 three times
+    a.v[6] = (float)cp[0];
+    a.v[7] = (float)cp[I.ch == 3 ? 1 : 0];
+    a.v[8] = (float)cp[I.ch == 3 ? 2 : 0];
     a.v[9] = grad;
     a.v[10] = depth_error;
     if (e.depth == 0) {
@@ -260,13 +261,14 @@ __device__ bool test_pixel(const FusImgDev& I, const uint32_t* __restrict__ mask
   v[3] = nrm[0];
   v[4] = nrm[1];
   v[5] = nrm[2];
-  v[6] = (float)I.rgb[3 * p];
-  v[7] = (float)I.rgb[3 * p + 1];
-  v[8] = (float)I.rgb[3 * p + 2];
+  const uint8_t* cp = I.rgb + I.ch * p;  // grey: channel 0 three times
+  v[6] = (float)cp[0];
+  v[7] = (float)cp[I.ch == 3 ? 1 : 0];
+  v[8] = (float)cp[I.ch == 3 ? 2 : 0];
   float grad = 0.0f;
   if (col > 1 && col < W - 1 && row > 1 && row < H - 1) {
-    const int l = I.rgb[3 * (p - 1)], r = I.rgb[3 * (p + 1)];
-    const int t = I.rgb[3 * (p - W)], b = I.rgb[3 * (p + W)];
+    const int l = I.rgb[I.ch * (p - 1)], r = I.rgb[I.ch * (p + 1)];
+    const int t = I.rgb[I.ch * (p - W)], b = I.rgb[I.ch * (p + W)];
     const float gx = (float)(l - r) / 255.0f, gy = (float)(t - b) / 255.0f;
     grad = sqrtf(gx * gx + gy * gy);
   }
@@ -1428,7 +1430,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
     if (!x.used) continue;
     if (x.width <= 6 || x.height <= 6 || x.width > 65535 || x.height > 32767 || !x.depth || !x.rgb ||
         (o->with_normal && !x.normal) || x.num_overlapping < 0 || (x.num_overlapping > 0 && !x.overlapping) ||
-        !finite_camera(x.cam))
+        !(x.channels == 0 || x.channels == 1 || x.channels == 3) || !finite_camera(x.cam))
       return fail(PM_ERR_INVALID_ARG, "invalid fusion image " + std::to_string(i));
     max_nov = std::max(max_nov, x.num_overlapping);
     const int64_t sc = (x.width - 6 + o->step_size - 1) / o->step_size;
@@ -1453,12 +1455,14 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
 
   // every used image's maps on the device, mask bits zero
   std::vector<FusImgDev> tab(n);
-  std::vector<int64_t> base(n, 0);
-  int64_t total = 0, ov_total = 0;
+  std::vector<int64_t> base(n, 0), cbase(n, 0);  // first pixel, first colour byte
+  int64_t total = 0, ov_total = 0, col_total = 0;
   for (int32_t i = 0; i < n; ++i) {
     if (!im[i].used) continue;
     base[i] = total;
+    cbase[i] = col_total;
     total += (int64_t)im[i].width * im[i].height;
+    col_total += (int64_t)im[i].width * im[i].height * (im[i].channels == 1 ? 1 : 3);
     ov_total += im[i].num_overlapping;
   }
   const int64_t mask_words = (total + 31) / 32;
@@ -1466,7 +1470,7 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
   if (pm_status s = d_depth.get((size_t)total * 4)) return s;
   if (o->with_normal)
     if (pm_status s = d_normal.get((size_t)total * 12)) return s;
-  if (pm_status s = d_rgb.get((size_t)total * 3)) return s;
+  if (pm_status s = d_rgb.get((size_t)col_total)) return s;
   if (pm_status s = d_mask.get((size_t)mask_words * 4)) return s;
   if (pm_status s = d_ov.get((size_t)std::max<int64_t>(ov_total, 1) * 4)) return s;
   if (pm_status s = d_fused.get((size_t)n)) return s;
@@ -1481,11 +1485,12 @@ PM_API pm_status pm_fusion_run(pm_engine* e, const pm_fusion_options* o, int32_t
     const int64_t px = (int64_t)t.W * t.H;
     t.base = base[i];
     t.depth = d_depth.as<const float>() + base[i];
-    t.rgb = d_rgb.as<const uint8_t>() + 3 * base[i];
+    t.ch = im[i].channels == 1 ? 1 : 3;
+    t.rgb = d_rgb.as<const uint8_t>() + cbase[i];
     t.ov = d_ov.as<const int32_t>() + ov_at;
     t.nov = im[i].num_overlapping;
     if (pm_status s = copy_h2d(e, d_depth.as<float>() + base[i], im[i].depth, (size_t)px * 4, st)) return s;
-    if (pm_status s = copy_h2d(e, d_rgb.as<uint8_t>() + 3 * base[i], im[i].rgb, (size_t)px * 3, st)) return s;
+    if (pm_status s = copy_h2d(e, d_rgb.as<uint8_t>() + cbase[i], im[i].rgb, (size_t)(px * t.ch), st)) return s;
     if (o->with_normal) {
       t.normal = d_normal.as<const float>() + 3 * base[i];
       if (pm_status s = copy_h2d(e, d_normal.as<float>() + 3 * base[i], im[i].normal, (size_t)px * 12, st)) return s;

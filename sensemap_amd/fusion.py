@@ -70,10 +70,7 @@ class FusionInputs:
         for i, im in enumerate(images):
             d = np.ascontiguousarray(im.depth, np.float32)
             H, W = d.shape
-            col = np.asarray(im.color, np.uint8)
-            if col.ndim == 2:  # grey: the same value in the three channels
-                col = np.stack([col, col, col], axis=-1)
-            col = np.ascontiguousarray(col)
+            col = np.ascontiguousarray(np.asarray(im.color, np.uint8))  # (H, W, 3) or grey (H, W)
             ov = np.ascontiguousarray(np.asarray(list(im.overlapping), np.int32).reshape(-1))
             self._keep += [d, col, ov]
             c = self.images[i]
@@ -85,6 +82,7 @@ class FusionInputs:
                 self._keep.append(n)
                 c.normal = n.ctypes.data_as(C.POINTER(C.c_float))
             c.rgb = col.ctypes.data_as(C.POINTER(C.c_uint8))
+            c.channels = 1 if col.ndim == 2 else 3
             c.overlapping = ov.ctypes.data_as(C.POINTER(C.c_int32))
             c.num_overlapping = len(ov)
             c.is_rig = int(bool(im.is_rig))

@@ -119,3 +119,15 @@ def test_full_size_eleven_views_equal_transcription():
     want = binding.fusion(ims, o)
     assert len(got["score"]) > 100000
     same(got, want)
+
+
+def test_grey_and_three_channel_images_equal_transcription():
+    import dataclasses
+    ims = orbit_images(cams=8)
+    rgb = [dataclasses.replace(im, color=np.stack([im.color] * 3, axis=-1)) for im in ims]
+    o = StereoFusionOptions()
+    eng = pmm.Engine(0)
+    grey, col = fuse(ims, o, eng), fuse(rgb, o, eng)
+    print("[fusion] grey vs three channels: points", len(grey["score"]), flush=True)
+    same(grey, col)
+    same(grey, binding.fusion(ims, o))

@@ -145,3 +145,16 @@ def test_first_non_rig_image_seeds():
     assert np.median(out0["score"]) < 1e-3
     out1 = B.fusion(scene(depth_scale=sc, rig=[1, 0, 0, 0, 0]))
     assert np.allclose(np.median(out1["score"]), 0.6 * 0.4, rtol=5e-3)
+
+
+def test_grey_images_equal_their_three_channel_copies():
+    # a grey image (channels 1) reads its value as r, g and b: the points equal
+    # those of the same images with the grey value stacked into three channels
+    import dataclasses
+    ims = scene(color_off=[0, 3, -2, 5, 1])
+    rgb = [dataclasses.replace(im, color=np.stack([im.color] * 3, axis=-1)) for im in ims]
+    o = StereoFusionOptions()
+    a, b = B.fusion(ims, o), B.fusion(rgb, o)
+    assert len(a["score"]) == len(b["score"]) > 0
+    for k in ("xyz", "normal", "rgb", "score", "vis_offsets", "vis_images"):
+        assert np.array_equal(a[k], b[k]), k
