@@ -741,26 +741,26 @@ struct Flow {
 // array: a dynamically indexed array would live in scratch memory).
 struct NodeSet {
   uint64_t w0, w1, w2, w3;
-  __device__ bool has(int i) const {  // any lane's i (masks, not selects: no lookup table)
-    const uint32_t sh = (uint32_t)i & 63u;
-    const uint64_t q = (uint64_t)((uint32_t)i >> 6);
-    const uint64_t b = ((w0 >> sh) & (uint64_t)(q == 0)) | ((w1 >> sh) & (uint64_t)(q == 1)) |
-                       ((w2 >> sh) & (uint64_t)(q == 2)) | ((w3 >> sh) & (uint64_t)(q == 3));
-    return (b & 1ull) != 0;
+  __device__ bool has(int i) const {  // any lane's i: the word by two selects on its bits, one shift
+    const bool hi = ((uint32_t)i & 128u) != 0, odd = ((uint32_t)i & 64u) != 0;
+    const uint64_t a = odd ? w1 : w0, b = odd ? w3 : w2;
+    const uint64_t w = hi ? b : a;
+    return ((w >> ((uint32_t)i & 63u)) & 1ull) != 0;
   }
-  __device__ void add(int i) {
-    const uint64_t b = 1ull << (i & 63);
+  __device__ void remove(int i) {  // a wave-uniform i
+    const uint64_t b = ~(1ull << (i & 63));
     const int q = i >> 6;
-    w0 |= q == 0 ? b : 0ull;
-    w1 |= q == 1 ? b : 0ull;
-    w2 |= q == 2 ? b : 0ull;
-    w3 |= q == 3 ? b : 0ull;
+    w0 &= q == 0 ? b : ~0ull;
+    w1 &= q == 1 ? b : ~0ull;
+    w2 &= q == 2 ? b : ~0ull;
+    w3 &= q == 3 ? b : ~0ull;
   }
 };
 
 // The depth-first walk of Fuse over a graph in LDS, run by the whole wave:
 // the accepted nodes in visiting order (accl), their count returned.
-// `ok` holds the nodes not masked; the taken set lives in scalar registers.
+// `ok` holds the nodes not masked; the available set (not masked and not yet
+// taken) lives in scalar registers.
 // A push of a node that is masked or already taken would end in `continue`
 // when popped (a node is never untaken), so it is not stacked; the others
 // are stacked in push order, one lane per push.  Every node is taken at
@@ -777,10 +777,10 @@ __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm
                                                              const int32_t info[4], const int16_t* child,
                                                              int32_t* stack, int16_t* accl) {
   const int lane = threadIdx.x % 64;
-  NodeSet taken{0ull, 0ull, 0ull, 0ull};
+  NodeSet avail = ok;
   int sp = 0, na = 0;
   int e = 0;                 // the entry to visit next: node 0 at depth 0
-  bool have = ok.has(0);
+  bool have = avail.has(0);
   for (;;) {
     if (!have) {
       while (sp > 0) {
@@ -789,7 +789,7 @@ __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm
         bool v = false;
         if (k >= 0) {
           x = stack[k];
-          v = ok.has(x & 0xffff) && !taken.has(x & 0xffff);
+          v = avail.has(x & 0xffff);
         }
         const uint64_t m = __ballot(v);
         if (m) {
@@ -805,7 +805,7 @@ __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm
     }
     have = false;
     const int node = e & 0xffff, d = e >> 16;
-    taken.add(node);
+    avail.remove(node);
     if (lane == 0) accl[na] = (int16_t)node;
     ++na;
     if (na >= prm.max_num_pixels) break;
@@ -821,7 +821,7 @@ __device__ __attribute__((always_inline)) inline int replay(const FusParams& prm
       bool push = false;
       if (k < cnt) {
         c = child[b + k];
-        push = ok.has(c) && !taken.has(c);
+        push = avail.has(c);
       }
       const uint64_t m = __ballot(push);
       if (push) stack[sp + __popcll(m & ((1ull << lane) - 1))] = val0 | c;
@@ -966,37 +966,38 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
     for (int k = lane; k < np; k += 64)
       dep = max(dep, __hip_atomic_load(f.depth + f.preds[no + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const int sw0 = lane < ns ? f.succ[se - ns + lane] : -1;
-    // the seed first (node 0, its pixel from the seed's number): taken by an
-    // earlier walk, this walk accepts nothing, and its graph is not needed.
-    // The masks are final for the walk's pixels: every earlier walk that
-    // could take one has finished.
+    // the graph into LDS, every load in flight at once and together with
+    // the seed's mask word: a graph's nodes and pushes start at multiples of
+    // 8 elements, so 16-byte loads cover them (the tails past nn and nch are
+    // never read)
+    const int ol = lane & 31;
+    const bool lg0 = 2 * lane < nn, lg1 = 2 * (64 + lane) < nn, lc = 8 * ol < nn;
+    const bool lh0 = 8 * lane < nch, lh1 = 8 * (64 + lane) < nch, lh2 = 8 * (128 + lane) < nch,
+               lh3 = 8 * (192 + lane) < nch;
+    const uint4 z4 = {0u, 0u, 0u, 0u};
+    const uint4 g0 = lg0 ? *(const uint4*)(g.gid + no + 2 * lane) : z4;
+    const uint4 g1 = lg1 ? *(const uint4*)(g.gid + no + 2 * (64 + lane)) : z4;
+    const uint4 c0 = lc ? *(const uint4*)((lane < 32 ? g.cbase : g.ccnt) + no + 8 * ol) : z4;
+    const uint4 h0 = lh0 ? *(const uint4*)(g.child + co + 8 * lane) : z4;
+    const uint4 h1 = lh1 ? *(const uint4*)(g.child + co + 8 * (64 + lane)) : z4;
+    const uint4 h2 = lh2 ? *(const uint4*)(g.child + co + 8 * (128 + lane)) : z4;
+    const uint4 h3 = lh3 ? *(const uint4*)(g.child + co + 8 * (192 + lane)) : z4;
+    // the seed (node 0, its pixel from the seed's number): taken by an
+    // earlier walk, this walk accepts nothing.  The masks are final for the
+    // walk's pixels: every earlier walk that could take one has finished.
     const int sd = f.window_first + ws;
     const int64_t seed_gid = f.seed_base + (int64_t)(3 + (sd / prm.seed_cols) * prm.step) * f.seed_w +
                              (3 + (sd % prm.seed_cols) * prm.step);
+    const bool seed_free = !masked_now(mask, seed_gid);
+    if (lg0) *(uint4*)(ngid + 2 * lane) = g0;
+    if (lg1) *(uint4*)(ngid + 2 * (64 + lane)) = g1;
+    if (lc) *(uint4*)((lane < 32 ? nbase : ncnt) + 8 * ol) = c0;
+    if (lh0) *(uint4*)(child + 8 * lane) = h0;
+    if (lh1) *(uint4*)(child + 8 * (64 + lane)) = h1;
+    if (lh2) *(uint4*)(child + 8 * (128 + lane)) = h2;
+    if (lh3) *(uint4*)(child + 8 * (192 + lane)) = h3;
     int na = 0;
-    if (!masked_now(mask, seed_gid)) {
-      // the graph into LDS, every load in flight at once: a graph's nodes
-      // and pushes start at multiples of 8 elements, so 16-byte loads cover
-      // them (the tails past nn and nch are never read)
-      const int ol = lane & 31;
-      const bool lg0 = 2 * lane < nn, lg1 = 2 * (64 + lane) < nn, lc = 8 * ol < nn;
-      const bool lh0 = 8 * lane < nch, lh1 = 8 * (64 + lane) < nch, lh2 = 8 * (128 + lane) < nch,
-                 lh3 = 8 * (192 + lane) < nch;
-      const uint4 z4 = {0u, 0u, 0u, 0u};
-      const uint4 g0 = lg0 ? *(const uint4*)(g.gid + no + 2 * lane) : z4;
-      const uint4 g1 = lg1 ? *(const uint4*)(g.gid + no + 2 * (64 + lane)) : z4;
-      const uint4 c0 = lc ? *(const uint4*)((lane < 32 ? g.cbase : g.ccnt) + no + 8 * ol) : z4;
-      const uint4 h0 = lh0 ? *(const uint4*)(g.child + co + 8 * lane) : z4;
-      const uint4 h1 = lh1 ? *(const uint4*)(g.child + co + 8 * (64 + lane)) : z4;
-      const uint4 h2 = lh2 ? *(const uint4*)(g.child + co + 8 * (128 + lane)) : z4;
-      const uint4 h3 = lh3 ? *(const uint4*)(g.child + co + 8 * (192 + lane)) : z4;
-      if (lg0) *(uint4*)(ngid + 2 * lane) = g0;
-      if (lg1) *(uint4*)(ngid + 2 * (64 + lane)) = g1;
-      if (lc) *(uint4*)((lane < 32 ? nbase : ncnt) + 8 * ol) = c0;
-      if (lh0) *(uint4*)(child + 8 * lane) = h0;
-      if (lh1) *(uint4*)(child + 8 * (64 + lane)) = h1;
-      if (lh2) *(uint4*)(child + 8 * (128 + lane)) = h2;
-      if (lh3) *(uint4*)(child + 8 * (192 + lane)) = h3;
+    if (seed_free) {
       __builtin_amdgcn_wave_barrier();
       int32_t info[4];
 #pragma unroll
