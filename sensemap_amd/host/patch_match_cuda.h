@@ -9,6 +9,8 @@
 //   PatchMatchACMMCuda       src/mvs/patch_match_cuda.h:154-203
 //   Model (sparse model)     src/mvs/model.h:22-101: GetMaxOverlappingImages,
 //                            ComputeDepthRanges
+//   StereoFusion, PlyPoint   src/mvs/fusion.h:43-159, src/util/ply.h:14-25
+//                            (maps in memory instead of a workspace on disk)
 // so PatchMatchController::ProcessProblem-style code
 // (src/controllers/patch_match_controller.cc:4676-4686) reads the same:
 //
@@ -681,6 +683,144 @@ struct Model {
     for (size_t i = 0; i < I; ++i) lists[i].assign(out.begin() + off[i], out.begin() + off[i + 1]);
     return lists;
   }
+};
+
+// util/ply.h:14-25.
+struct PlyPoint {
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  uint8_t r = 0, g = 0, b = 0;
+  uint8_t s_id = 0xff;
+};
+
+// StereoFusion (src/mvs/fusion.h:43-159): the depth-first walk of
+// FuseImage / Fuse (fusion.cc:1730-2183) over the images in Run()'s order,
+// with the reference's Options names and defaults for the fields the walk
+// reads, GetFusedPoints and GetFusedPointsVisibility.  The reference reads
+// its images, maps and overlapping lists from a workspace on disk
+// (ReadWorkspace); this mirror takes them in memory: per image its Image
+// (cameras and grey bitmap), DepthMap, NormalMap (planar, camera frame) and
+// overlapping image list (Model::GetMaxOverlappingImages).  The result is
+// the reference's with one fusing thread per image.
+class StereoFusion {
+ public:
+  struct Options {
+    int step_size = 1;
+    int min_num_pixels = 5;
+    int max_num_pixels = 10000;
+    int min_num_visible_images = 3;
+    int max_traversal_depth = 100;
+    double max_reproj_error = 2.0;
+    double max_depth_error = 0.01f;
+    bool with_normal = true;
+    double max_normal_error = 10.0f;
+    bool Check() const {
+      return step_size >= 1 && min_num_pixels >= 2 && max_num_pixels >= 1 && max_traversal_depth >= 1 &&
+             max_reproj_error >= 0.0 && max_depth_error > 0.0 && max_normal_error >= 0.0;
+    }
+  };
+
+  // used_images[i] false leaves image i out (used_images_ of the reference);
+  // an empty list uses every image.
+  StereoFusion(const Options& options, const std::vector<Image>* images, const std::vector<DepthMap>* depth_maps,
+               const std::vector<NormalMap>* normal_maps, const std::vector<std::vector<int>>* overlapping_images,
+               std::vector<bool> used_images = {}, int gpu_index = 0)
+      : options_(options), images_(images), depth_maps_(depth_maps), normal_maps_(normal_maps),
+        overlapping_(overlapping_images), used_(std::move(used_images)) {
+    PmCheck(pm_engine_create(gpu_index, &engine_));
+  }
+  ~StereoFusion() { pm_engine_destroy(engine_); }
+  StereoFusion(const StereoFusion&) = delete;
+  StereoFusion& operator=(const StereoFusion&) = delete;
+
+  void Run() {
+    if (!options_.Check()) throw std::runtime_error("sensemap_pm: invalid StereoFusion options");
+    const size_t n = images_->size();
+    std::vector<pm_fusion_image> ims(n);
+    for (size_t i = 0; i < n; ++i) {
+      const Image& im = (*images_)[i];
+      pm_fusion_image& x = ims[i];
+      x.cam = ToCamera(im);
+      x.width = (int32_t)im.GetWidth();
+      x.height = (int32_t)im.GetHeight();
+      x.depth = (*depth_maps_)[i].GetPtr();
+      x.normal = options_.with_normal ? (*normal_maps_)[i].GetPtr() : nullptr;
+      x.rgb = im.GetBitmap().data();
+      x.channels = 1;
+      x.overlapping = (*overlapping_)[i].data();
+      x.num_overlapping = (int32_t)(*overlapping_)[i].size();
+      x.is_rig = im.IsRig() ? 1 : 0;
+      x.used = used_.empty() || used_[i] ? 1 : 0;
+    }
+    pm_fusion_options o;
+    PmCheck(pm_fusion_options_init(&o));
+    o.step_size = options_.step_size;
+    o.min_num_pixels = options_.min_num_pixels;
+    o.max_num_pixels = options_.max_num_pixels;
+    o.min_num_visible_images = options_.min_num_visible_images;
+    o.max_traversal_depth = options_.max_traversal_depth;
+    o.with_normal = options_.with_normal ? 1 : 0;
+    o.max_reproj_error = options_.max_reproj_error;
+    o.max_depth_error = options_.max_depth_error;
+    o.max_normal_error = options_.max_normal_error;
+    pm_fusion* f = nullptr;
+    PmCheck(pm_fusion_run(engine_, &o, (int32_t)n, ims.data(), &f));
+    int64_t np = 0, nv = 0;
+    pm_status st = pm_fusion_get_counts(f, &np, &nv);
+    std::vector<float> xyz((size_t)np * 3), nrm((size_t)np * 3), score((size_t)np);
+    std::vector<uint8_t> rgb((size_t)np * 3);
+    std::vector<int64_t> off((size_t)np + 1);
+    std::vector<int32_t> vis((size_t)std::max<int64_t>(nv, 1));
+    if (st == PM_OK)
+      st = pm_fusion_get_points(f, xyz.data(), nrm.data(), rgb.data(), score.data(), off.data(), vis.data());
+    pm_fusion_destroy(f);
+    PmCheck(st);
+    points_.assign((size_t)np, PlyPoint());
+    visibility_.assign((size_t)np, std::vector<uint32_t>());
+    scores_ = score;
+    for (int64_t k = 0; k < np; ++k) {
+      PlyPoint& q = points_[(size_t)k];
+      q.x = xyz[3 * k];
+      q.y = xyz[3 * k + 1];
+      q.z = xyz[3 * k + 2];
+      q.nx = nrm[3 * k];
+      q.ny = nrm[3 * k + 1];
+      q.nz = nrm[3 * k + 2];
+      q.r = rgb[3 * k];
+      q.g = rgb[3 * k + 1];
+      q.b = rgb[3 * k + 2];
+      for (int64_t j = off[k]; j < off[k + 1]; ++j) visibility_[(size_t)k].push_back((uint32_t)vis[(size_t)j]);
+    }
+  }
+
+  const std::vector<PlyPoint>& GetFusedPoints() const { return points_; }
+  const std::vector<std::vector<uint32_t>>& GetFusedPointsVisibility() const { return visibility_; }
+  // fused_points_score_ (fusion.h:256), which the reference keeps without a getter
+  const std::vector<float>& GetFusedPointsScore() const { return scores_; }
+
+ private:
+  static pm_camera ToCamera(const Image& im) {
+    pm_camera c{};
+    const float* K = im.GetK();
+    c.K[0] = K[0];
+    c.K[1] = K[2];
+    c.K[2] = K[4];
+    c.K[3] = K[5];
+    for (int i = 0; i < 9; ++i) c.R[i] = im.GetR()[i];
+    for (int i = 0; i < 3; ++i) c.T[i] = im.GetT()[i];
+    return c;
+  }
+
+  Options options_;
+  const std::vector<Image>* images_;
+  const std::vector<DepthMap>* depth_maps_;
+  const std::vector<NormalMap>* normal_maps_;
+  const std::vector<std::vector<int>>* overlapping_;
+  std::vector<bool> used_;
+  pm_engine* engine_ = nullptr;
+  std::vector<PlyPoint> points_;
+  std::vector<std::vector<uint32_t>> visibility_;
+  std::vector<float> scores_;
 };
 
 }  // namespace mvs

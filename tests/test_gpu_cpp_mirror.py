@@ -146,3 +146,61 @@ def test_cpp_model_selection_matches_transcription(tmp_path, subset):
     rng = np.array([[int(x) for x in l.split()] for l in (tmp_path / "ranges.txt").read_text().splitlines()],
                    np.uint32)
     assert np.array_equal(rng, ob.depth_ranges(m).view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_cpp_stereo_fusion_matches_transcription(tmp_path):
+    """mvs::StereoFusion through the C++ mirror (tests/cpp/shim_fusion.cpp):
+    points, scores and visibility equal the transcription's bit for bit."""
+    from oracle import binding
+    from sensemap_amd import io as pmio
+    from sensemap_amd.fusion import FusionImage, StereoFusionOptions
+    from tests.test_gpu_fusion import orbit_images
+    binary = os.path.join(ROOT, "tests", "cpp", "bin", "shim_fusion")
+    assert os.path.exists(binary), "tests/cpp/bin/shim_fusion is built by __graft_entry__.build()"
+    src = orbit_images(cams=8)
+    lines = [str(len(src))]
+    ims = []
+    for k, im in enumerate(src):
+        H, W = im.depth.shape
+        K = [float(np.float32(x)) for x in im.K]
+        R = [float(x) for x in np.asarray(im.R, np.float32).reshape(9)]
+        T = [float(x) for x in np.asarray(im.T, np.float32).reshape(3)]
+        ov = list(im.overlapping)
+        lines.append(" ".join([str(W), str(H)] + [repr(x) for x in K + R + T] +
+                              ["0", "1", str(len(ov))] + [str(j) for j in ov]))
+        pmio.write_depth_map(str(tmp_path / f"depth{k}.bin"), im.depth)
+        pmio.write_normal_map(str(tmp_path / f"normal{k}.bin"), im.normal)
+        (tmp_path / f"grey{k}.raw").write_bytes(np.ascontiguousarray(im.color, np.uint8).tobytes())
+        # the inputs as the C++ side reads them: float32 cameras, normals
+        # decoded from their file
+        ims.append(FusionImage(tuple(K), np.asarray(R, np.float32).reshape(3, 3), np.asarray(T, np.float32),
+                               pmio.read_depth_map(str(tmp_path / f"depth{k}.bin")),
+                               pmio.read_normal_map(str(tmp_path / f"normal{k}.bin")), im.color, ov))
+    o = StereoFusionOptions(min_num_pixels=3)
+    lines.append(f"{o.step_size} {o.min_num_pixels} {o.max_num_pixels} {o.min_num_visible_images} "
+                 f"{o.max_traversal_depth} {o.max_reproj_error!r} {o.max_depth_error!r} {int(o.with_normal)} "
+                 f"{o.max_normal_error!r}")
+    (tmp_path / "meta.txt").write_text("\n".join(lines) + "\n")
+    subprocess.run([binary, str(tmp_path)], check=True, timeout=120)
+    raw = (tmp_path / "points.bin").read_bytes()
+    n = int(np.frombuffer(raw[:8], np.int64)[0])
+    at = 8
+    fl = np.frombuffer(raw[at:at + n * 24], np.float32).reshape(n, 6)
+    at += n * 24
+    rgb = np.frombuffer(raw[at:at + n * 3], np.uint8).reshape(n, 3)
+    at += n * 3
+    score = np.frombuffer(raw[at:at + n * 4], np.float32)
+    at += n * 4
+    off = np.frombuffer(raw[at:at + (n + 1) * 8], np.int64)
+    at += (n + 1) * 8
+    vis = np.frombuffer(raw[at:], np.uint32)
+    want = binding.fusion(ims, o)
+    print("[cpp mirror] fusion points", n, flush=True)
+    assert n == len(want["score"]) and n > 100
+    assert np.array_equal(fl[:, :3].view(np.uint32), want["xyz"].view(np.uint32))
+    assert np.array_equal(fl[:, 3:].view(np.uint32), want["normal"].view(np.uint32))
+    assert np.array_equal(rgb, want["rgb"])
+    assert np.array_equal(score.view(np.uint32), want["score"].view(np.uint32))
+    assert np.array_equal(off, want["vis_offsets"])
+    assert np.array_equal(vis.astype(np.int32), want["vis_images"])
