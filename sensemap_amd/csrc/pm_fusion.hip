@@ -23,16 +23,18 @@
 //     (that walk needed the ones before it in turn).  The walks holding each
 //     pixel are listed (count, prefix sum, scatter); k_preds turns the lists
 //     into each walk's distinct predecessors and k_succ into successor lists.
-//   * Dataflow.  k_flow's persistent waves take walks from a ready queue: a
-//     walk enters it when its last predecessor finishes, so no wave ever
-//     waits on a walk.  A wave replays its walk over the graph (the whole
-//     wave: node sets in scalar registers, one lane per push), sets the
-//     accepted pixels' mask bits, publishes them with a release fence and
-//     counts itself off at each successor.  Everything a wave reads that
-//     another wrote (queue slots, mask words) is an agent-scope atomic load,
-//     since the waves run on every XCD.  A wave waiting for a queue slot
-//     gives up only when no walk anywhere finished for 4 s, which fails the
-//     call.
+//   * Dataflow.  k_flow's persistent waves (one per CU) run a walk once its
+//     last predecessor has finished, so no wave ever waits on a walk: the
+//     wave that finishes that predecessor runs the first walk it made ready
+//     itself and puts the others in a ready queue, from which the other
+//     waves take them.  A wave replays its walk over the graph (the whole
+//     wave: the set of available nodes in scalar registers, one lane per
+//     push), sets the accepted pixels' mask bits, publishes them with a
+//     release fence and counts itself off at each successor.  Everything a
+//     wave reads that another wrote (queue slots, mask words) is an
+//     agent-scope atomic load, since the waves run on every XCD.  A wave
+//     waiting for a queue slot gives up only when no walk anywhere finished
+//     for 4 s, which fails the call.
 //   * A graph that outgrows the wave's LDS marks its walk to run alone
 //     (k_walk, pixel by pixel, with buffers for max_num_pixels) between two
 //     dataflow segments.
@@ -63,6 +65,8 @@ constexpr int kProvCap = 512;                     // pixels (passing or not) met
 constexpr int kProvChild = 4096;                  // push slots met building a graph
 constexpr int kHash = 1024;                       // open-addressing slots (gid -> pixel)
 constexpr int64_t kMaxWindow = int64_t(1) << 23;  // seeds per window
+// Build knobs of the dataflow, defaults measured on the 1920x1080 11-view
+// scene (8: one wave per CU; 16 s_sleep units between polls).
 #ifndef PM_FLOW_WAVES_PER_8CU
 #define PM_FLOW_WAVES_PER_8CU 8
 #endif
@@ -904,12 +908,13 @@ __global__ void __launch_bounds__(256) k_succ(Pool g, Flow f) {
 // written once all the walks it depends on have finished.  A wave replays
 // its walk over the graph against the masks, sets the accepted pixels' mask
 // bits, publishes them (release fence), and then counts itself off at each
-// successor, queueing those it was the last one for.  All reads of what
-// other waves wrote (queue slots, mask words, depths) are agent-scope atomic
-// loads.  The slots are written as the walks become ready, and every walk
-// becomes ready (a walk depends only on earlier walks), so every wave
-// claiming a slot below n_tickets gets its walk; a wave gives up only when no
-// walk anywhere finished for wait_ticks.
+// successor: the first one it was the last predecessor of, it runs next
+// itself; the others it queues.  All reads of what other waves wrote (queue
+// slots, mask words, depths) are agent-scope atomic loads.  Every walk
+// becomes ready (a walk depends only on earlier walks) and is either run by
+// the wave that made it ready or written to a slot, so a wave waiting on a
+// slot gets a walk or sees every walk finished; it gives up only when no walk
+// anywhere finished for wait_ticks.
 __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* mask, Flow f) {
   __shared__ __attribute__((aligned(16))) int64_t ngid[kNodeCap];
   __shared__ __attribute__((aligned(16))) int16_t nbase[kNodeCap];
