@@ -1028,12 +1028,6 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
         f.res_nodes[no + j] = (int16_t)node;
       }
     }
-    for (int d = 32; d > 0; d >>= 1) dep = max(dep, __shfl_xor(dep, d));
-    if (lane == 0) {
-      f.res_cnt[ws] = na;
-      __hip_atomic_store(f.depth + ws, dep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      atomicMax(f.counters + 4, dep + 1);
-    }
     // publish the mask bits (a walk that took no pixel wrote none)
     if (na > 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     // the successors this walk was the last one for: the first one this
@@ -1049,6 +1043,14 @@ __global__ void __launch_bounds__(64) k_flow(FusParams prm, Pool g, uint32_t* ma
       if (ready && lane != kl)
         __hip_atomic_store(f.queue + atomicAdd(f.counters + 1, 1), sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (kl >= 0) next = __builtin_amdgcn_readlane(sw, kl);
+    }
+    // the statistic after the countdown: its loads are then waited for
+    // together with the countdown's, not before it
+    for (int d = 32; d > 0; d >>= 1) dep = max(dep, __shfl_xor(dep, d));
+    if (lane == 0) {
+      f.res_cnt[ws] = na;
+      __hip_atomic_store(f.depth + ws, dep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicMax(f.counters + 4, dep + 1);
     }
     if (lane == 0) atomicAdd(f.counters + 2, 1);
   }
