@@ -64,8 +64,9 @@ def parse():
     ap.add_argument("--workload", choices=["view", "batch", "serve", "select", "filters"], default="view",
                     help="view: the 1920x1080 view (headline); batch: 64 orbit views, plain vs batched; "
                          "serve: Poisson arrivals into the serving loop; select: source-view selection "
-                         "on a 500-camera sparse model; filters: RunFilter, cross filter and normal "
-                         "encoding after the path")
+                         "on a 500-camera sparse model; filters: RunFilter, cross filter, normal "
+                         "encoding, the steps after the geometric pass, deduplication and depth-map fusion "
+                         "after the path")
     ap.add_argument("--views", type=int, default=64, help="batch workload: reference views")
     ap.add_argument("--jobs", type=int, default=24, help="serve workload: jobs per arrival rate")
     ap.add_argument("--rates", default="5,20,50,200", help="serve workload: arrival rates, requests/s")
@@ -508,8 +509,9 @@ def run_filters_workload(args) -> dict:
     """The steps after the path on the 1920x1080 heightfield scene: each of the
     11 views' photometric maps (untimed), then RunFilter of the reference
     view against its 10 sources (geometric consistency), the cross-view
-    filter of all 11 views against the other 10, and the NormalMap encoding
-    of the reference's normals on the device.  Each is timed as the call a
+    filter of all 11 views against the other 10, the NormalMap encoding of
+    the reference's normals on the device, the steps after the geometric
+    pass, deduplication and depth-map fusion of the 11 refined maps.  Each is timed as the call a
     caller makes (device-resident inputs for RunFilter and the encoding; the
     cross filter takes host maps, so its call includes their upload) and
     beside its CPU transcription on the same inputs, with the results
